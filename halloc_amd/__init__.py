@@ -1,0 +1,203 @@
+"""Host-side Python mirror of the MI355X halloc allocator's C ABI.
+
+The allocator itself is device code (hamalloc / hafree in include/halloc.h,
+implemented in halloc_amd/csrc/halloc.hip).  This module loads the C-ABI
+library ``libhalloc_amd.so`` with ctypes and mirrors the reference's allocator
+plug-in interface for host code (tst/common/halloc-wrapper.h:9-37:
+``init`` / ``extfrag`` / ``shutdown``), plus the workload drivers its tests
+and benchmarks run (the counterparts of tst/common/common.h:361-412 and the
+tst/perf kernels).  There is no CPU fallback: if the library is missing,
+loading fails with an error naming the build command.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+__all__ = ["HallocOpts", "Halloc", "HallocError", "lib", "LIB_PATH"]
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libhalloc_amd.so")
+NSIZES = 16
+SIZE_TABLE_FIELDS = ("nchunks", "chunk_sz", "chunk_id", "ncib", "sparse", "hash_step",
+                     "roomy", "busy", "block_sz", "nblocks", "tmod", "chunk_bytes")
+# every symbol include/halloc_amd.h declares
+EXPORTS = ("ha_default_opts", "ha_init_c", "ha_shutdown_c", "ha_extfrag_c", "ha_free_bytes_c",
+           "ha_last_error", "ha_geometry_c", "ha_replay", "ha_dump_words", "ha_dump",
+           "ha_run_plain", "ha_malloc_sizes", "ha_free_ptrs", "ha_read_words", "ha_probe_atomics")
+
+
+class HallocError(RuntimeError):
+    pass
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [("memory", ctypes.c_uint64),
+                ("halloc_fraction", ctypes.c_double),
+                ("busy_fraction", ctypes.c_double),
+                ("roomy_fraction", ctypes.c_double),
+                ("sparse_fraction", ctypes.c_double),
+                ("sb_sz_sh", ctypes.c_int32),
+                ("nheads", ctypes.c_int32)]
+
+
+class _Geometry(ctypes.Structure):
+    _fields_ = [("nsbs", ctypes.c_uint32), ("sb_sz", ctypes.c_uint32), ("sb_sz_sh", ctypes.c_uint32),
+                ("nheads", ctypes.c_uint32), ("nsw", ctypes.c_uint32), ("bit_words", ctypes.c_uint32),
+                ("nib_words", ctypes.c_uint32), ("pad", ctypes.c_uint32),
+                ("arena_bytes", ctypes.c_uint64), ("cuda_memory", ctypes.c_uint64),
+                ("size_table", ctypes.c_uint32 * (16 * 12))]
+
+
+class _PlainResult(ctypes.Structure):
+    _fields_ = [("errors", ctypes.c_uint64), ("nulls", ctypes.c_uint64), ("total_ms", ctypes.c_double),
+                ("wall_ms", ctypes.c_double)]
+
+
+@dataclass
+class HallocOpts:
+    """Fields and defaults of halloc_opts_t (src/halloc.h:15-34), plus nheads."""
+    memory: int = 512 << 20
+    halloc_fraction: float = 0.75
+    busy_fraction: float = 0.835
+    roomy_fraction: float = 0.6
+    sparse_fraction: float = 0.012
+    sb_sz_sh: int = 22
+    nheads: int = 8
+
+    def _c(self) -> _Opts:
+        return _Opts(self.memory, self.halloc_fraction, self.busy_fraction, self.roomy_fraction,
+                     self.sparse_fraction, self.sb_sz_sh, self.nheads)
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libhalloc_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HallocError(f"{LIB_PATH} is missing: build it with `python -m halloc_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    i32, u64, vp = ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p
+    sigs = {
+        "ha_default_opts": (None, [P(_Opts)]),
+        "ha_init_c": (i32, [P(_Opts)]),
+        "ha_shutdown_c": (i32, []),
+        "ha_extfrag_c": (i32, [i32, P(ctypes.c_double)]),
+        "ha_free_bytes_c": (i32, [P(u64)]),
+        "ha_last_error": (ctypes.c_char_p, []),
+        "ha_geometry_c": (i32, [P(_Geometry)]),
+        "ha_replay": (i32, [vp, i32, i32, vp, vp]),
+        "ha_dump_words": (ctypes.c_int64, []),
+        "ha_dump": (i32, [vp, ctypes.c_int64]),
+        "ha_run_plain": (i32, [i32, i32, i32, i32, i32, i32, vp, P(_PlainResult)]),
+        "ha_malloc_sizes": (i32, [vp, i32, i32, vp]),
+        "ha_free_ptrs": (i32, [vp, i32, i32]),
+        "ha_read_words": (i32, [vp, i32, vp]),
+        "ha_probe_atomics": (i32, [vp]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        raise HallocError(f"halloc error {rc}: {lib().ha_last_error().decode()}")
+    return rc
+
+
+class Halloc:
+    """The allocator on the current device (mirrors tst/common/halloc-wrapper.h:9-37)."""
+
+    @staticmethod
+    def init(opts: HallocOpts | None = None) -> None:
+        o = (opts or HallocOpts())._c()
+        _check(lib().ha_init_c(ctypes.byref(o)))
+
+    @staticmethod
+    def shutdown() -> None:
+        _check(lib().ha_shutdown_c())
+
+    @staticmethod
+    def extfrag(ideal: bool) -> float:
+        v = ctypes.c_double(0)
+        _check(lib().ha_extfrag_c(1 if ideal else 0, ctypes.byref(v)))
+        return v.value
+
+    @staticmethod
+    def free_bytes() -> int:
+        v = ctypes.c_uint64(0)
+        _check(lib().ha_free_bytes_c(ctypes.byref(v)))
+        return int(v.value)
+
+    @staticmethod
+    def geometry() -> dict:
+        g = _Geometry()
+        _check(lib().ha_geometry_c(ctypes.byref(g)))
+        table = np.frombuffer(bytes(g.size_table), dtype=np.uint32).reshape(NSIZES, 12).copy()
+        return {"nsbs": g.nsbs, "sb_sz": g.sb_sz, "sb_sz_sh": g.sb_sz_sh, "nheads": g.nheads,
+                "nsw": g.nsw, "bit_words": g.bit_words, "nib_words": g.nib_words,
+                "arena_bytes": g.arena_bytes, "cuda_memory": g.cuda_memory, "size_table": table}
+
+    # ---------------------------------------------------------- workloads
+    @staticmethod
+    def replay(ops: np.ndarray):
+        """ops: (nsteps, lanes, 2) uint32; returns (recs (nsteps, lanes, 4), xcc (nsteps,))."""
+        ops = np.ascontiguousarray(ops, dtype=np.uint32)
+        nsteps, lanes = ops.shape[0], ops.shape[1]
+        recs = np.zeros((nsteps, lanes, 4), dtype=np.uint32)
+        xcc = np.zeros(nsteps, dtype=np.uint32)
+        _check(lib().ha_replay(ops.ctypes.data, nsteps, lanes, recs.ctypes.data, xcc.ctypes.data))
+        return recs, xcc
+
+    @staticmethod
+    def dump() -> np.ndarray:
+        n = _check(lib().ha_dump_words())
+        out = np.zeros(n, dtype=np.uint64)
+        _check(lib().ha_dump(out.ctypes.data, n))
+        return out
+
+    @staticmethod
+    def run_plain(nthreads=1 << 20, bs=256, size=16, iters=1, warmup=1, launches=10):
+        ms = np.zeros(launches, dtype=np.float32)
+        r = _PlainResult()
+        _check(lib().ha_run_plain(nthreads, bs, size, iters, warmup, launches, ms.ctypes.data,
+                                  ctypes.byref(r)))
+        return {"errors": int(r.errors), "nulls": int(r.nulls), "total_ms": r.total_ms,
+                "wall_ms": r.wall_ms, "ms": ms}
+
+    @staticmethod
+    def malloc_sizes(sizes, bs=256) -> np.ndarray:
+        s = np.ascontiguousarray(sizes, dtype=np.uint32)
+        ptrs = np.zeros(len(s), dtype=np.uint64)
+        _check(lib().ha_malloc_sizes(s.ctypes.data, len(s), bs, ptrs.ctypes.data))
+        return ptrs
+
+    @staticmethod
+    def free_ptrs(ptrs, bs=256) -> None:
+        p = np.ascontiguousarray(ptrs, dtype=np.uint64)
+        _check(lib().ha_free_ptrs(p.ctypes.data, len(p), bs))
+
+    @staticmethod
+    def read_words(ptrs) -> np.ndarray:
+        p = np.ascontiguousarray(ptrs, dtype=np.uint64)
+        out = np.zeros(len(p), dtype=np.uint32)
+        _check(lib().ha_read_words(p.ctypes.data, len(p), out.ctypes.data))
+        return out
+
+    @staticmethod
+    def probe_atomics() -> np.ndarray:
+        out = np.zeros(3, dtype=np.float64)
+        _check(lib().ha_probe_atomics(out.ctypes.data))
+        return out

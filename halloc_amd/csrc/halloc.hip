@@ -1,0 +1,906 @@
+// halloc.hip -- MI355X (gfx950, wave64) device allocator: hamalloc / hafree and
+// the host API ha_init / ha_shutdown / ha_extfrag.
+//
+// Built with -fgpu-rdc into libhalloc.a so that user device code links against
+// hamalloc / hafree (the drop-in boundary of src/halloc.h:38-55).
+//
+// Scheme (canonizer/halloc): size classes of 2 or 3 units, slabs with one bit
+// per 16 B chunk searched by a hash, per-class head slabs with a cached next
+// head, and busy / roomy / sparse / free slab sets.  MI355X design:
+//   * a wave processes its lanes one size class at a time; inside a class group
+//     the head slab, the size info and the probe counter are wave-uniform;
+//   * a class group takes ONE ticket and its lanes fill one aligned run of 64
+//     blocks, so a probe is one atomic per 64-bit bit word touched, its union
+//     computed from the wave-uniform lane bitmap with scalar bit arithmetic;
+//   * head slabs and tickets exist per XCD (head = xcc_id % nheads), so the
+//     hot counters split 8 ways (one address sustains ~88 M atomics/s);
+//   * one contiguous arena: slab id = offset >> sb_sz_sh, no address grid;
+//   * slab counters, hot words and tickets each own a 128 B line.
+// All shared metadata is accessed with agent-scope atomics (coherent across
+// XCDs).  Fences (which write back / invalidate the L2) appear only on the slow
+// path, around the slab and head locks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/halloc.h"
+#include "../../include/halloc_amd.h"
+#include "ha_internal.h"
+
+namespace ha {
+
+__constant__ Params ha_p;
+
+#define HA_DEV __device__ __forceinline__
+
+// ------------------------------------------------------------ wave helpers
+HA_DEV u32 lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+HA_DEV u64 ballot(bool p) { return __ballot(p); }
+HA_DEV u32 rdl(u32 v, u32 lane) { return (u32)__builtin_amdgcn_readlane((int)v, (int)lane); }
+HA_DEV u64 rdl64(u64 v, u32 lane) {
+  return ((u64)rdl((u32)(v >> 32), lane) << 32) | rdl((u32)v, lane);
+}
+HA_DEV u32 rfl(u32 v) { return (u32)__builtin_amdgcn_readfirstlane((int)v); }
+HA_DEV u64 shfl64(u64 v, u32 src) {
+  int a = (int)(src << 2);
+  u32 lo = (u32)__builtin_amdgcn_ds_bpermute(a, (int)(u32)v);
+  u32 hi = (u32)__builtin_amdgcn_ds_bpermute(a, (int)(u32)(v >> 32));
+  return ((u64)hi << 32) | lo;
+}
+HA_DEV u32 xcc_id() { return __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)); }
+
+// ------------------------------------------------ agent-scope atomic helpers
+#define HA_SCOPE __HIP_MEMORY_SCOPE_AGENT
+HA_DEV u64 a_or(u64* p, u64 v) { return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, HA_SCOPE); }
+HA_DEV u64 a_and(u64* p, u64 v) { return __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, HA_SCOPE); }
+HA_DEV u64 a_add(u64* p, u64 v) { return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, HA_SCOPE); }
+HA_DEV u64 a_sub(u64* p, u64 v) { return __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, HA_SCOPE); }
+HA_DEV u32 a_add32(u32* p, u32 v) { return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, HA_SCOPE); }
+HA_DEV u32 a_xchg32(u32* p, u32 v) { return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, HA_SCOPE); }
+HA_DEV u64 ld64(u64* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, HA_SCOPE); }
+HA_DEV u32 ld32(u32* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, HA_SCOPE); }
+HA_DEV void st32(u32* p, u32 v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, HA_SCOPE); }
+
+HA_DEV u64* ctr_of(u32 sb) { return ha_p.ctr + (u64)sb * CTR_STRIDE; }
+HA_DEV u32* hot(u32* base, u32 ihead, u32 sid) { return base + (ihead * NSIZES + sid) * HOT_STRIDE; }
+HA_DEV u64* set_words(u32 set) { return ha_p.sets + (u64)set * ha_p.nsw; }
+
+// ----------------------------------------------------- single-lane slow path
+// Everything below until malloc_group runs on exactly one lane of a wave.
+
+HA_DEV bool try_lock(u32* m) {
+  bool ok = a_xchg32(m, 1u) == 0;
+  if (ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return ok;
+}
+HA_DEV void lock(u32* m) {
+  while (!try_lock(m)) __builtin_amdgcn_s_sleep(2);
+}
+HA_DEV void unlock(u32* m) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  st32(m, 0u);
+}
+
+HA_DEV void set_add(u32 set, u32 sb) { a_or(set_words(set) + sb / 64, 1ull << (sb % 64)); }
+HA_DEV void set_remove(u32 set, u32 sb) { a_and(set_words(set) + sb / 64, ~(1ull << (sb % 64))); }
+// first-fit claim over every word holding a slab (src/sbset.cuh:5-35)
+__device__ __noinline__ u32 set_pop(u32 set) {
+  u64* w = set_words(set);
+  for (u32 i = 0; i < ha_p.nsw; ++i) {
+    u64 word = ld64(w + i);
+    while (word) {
+      u32 b = (u32)__builtin_ctzll(word);
+      u64 m = 1ull << b;
+      if (a_and(w + i, ~m) & m) return i * 64 + b;
+      word &= ~m;
+    }
+  }
+  return SB_NONE;
+}
+
+// sb_try_mark_free (src/slab.cuh:89-113); caller holds sb_lock[sb]
+__device__ __noinline__ void try_mark_free(u32 sb, u32 size_id, u32 chunk_id, bool from_head) {
+  Desc* d = ha_p.desc + sb;
+  if (ld32(&d->is_head) || chunk_id == SZ_NONE) return;
+  u64 delta = ctr_chunk_delta(chunk_id), old2;
+  u64* c = ctr_of(sb);
+  do {
+    u64 old = a_add(c, delta);
+    if (ctr_count(old) == 0) {
+      st32(&d->size_id, SZ_NONE);
+      st32(&d->chunk_id, SZ_NONE);
+      st32(&d->chunk_sz, 0);
+      set_add(0, sb);
+      set_remove(set_sparse(chunk_id), sb);
+      if (size_id != SZ_NONE) {
+        set_remove(set_roomy(size_id), sb);
+        set_remove(set_busy(size_id), sb);
+      }
+      return;
+    }
+    old2 = a_sub(c, delta);
+  } while (ctr_count(old2) == 0);
+  if (from_head && size_id != SZ_NONE) {
+    u32 cls = sb_class(ctr_count(old2), ha_p.si[size_id]);
+    if (cls != CLS_FREE) set_add(class_set(cls, size_id, chunk_id), sb);
+  }
+}
+
+// count transitions after a decrement (src/slab.cuh:176-221): `old` is the
+// counter before the decrement of `change` chunks
+__device__ __noinline__ void ctr_transitions(u32 sb, u64 old, u32 change) {
+  if (ctr_is_head(old)) return;
+  Desc* d = ha_p.desc + sb;
+  u32 size_id = ld32(&d->size_id), chunk_id = ld32(&d->chunk_id);
+  if (size_id == SZ_NONE || chunk_id == SZ_NONE) return;
+  u32 oc = ctr_count(old), nc = oc - change;
+  const SizeInfo& s = ha_p.si[size_id];
+  u32 co = sb_class(oc, s), cn = sb_class(nc, s);
+  if (co == cn) return;
+  if (cn == CLS_FREE) {
+    if (ctr_chunk(old) != CTR_CHUNK_NONE) {
+      u32* l = ha_p.sb_lock + sb;
+      lock(l);
+      size_id = ld32(&d->size_id);
+      chunk_id = ld32(&d->chunk_id);
+      try_mark_free(sb, size_id, chunk_id, false);
+      unlock(l);
+    }
+  } else {
+    set_add(class_set(cn, size_id, chunk_id), sb);
+    if (co != CLS_FREE) set_remove(class_set(co, size_id, chunk_id), sb);
+  }
+}
+
+HA_DEV bool needs_transition(u32 sb, u64 old, u32 change) {
+  if (ctr_is_head(old)) return false;
+  Desc* d = ha_p.desc + sb;
+  u32 size_id = ld32(&d->size_id), chunk_id = ld32(&d->chunk_id);
+  if (size_id >= NSIZES || chunk_id == SZ_NONE) return false;
+  u32 oc = ctr_count(old);
+  const SizeInfo& s = ha_p.si[size_id];
+  return sb_class(oc, s) != sb_class(oc - change, s);
+}
+
+// detach_head (src/slab.cuh:227-255); caller holds sb_lock[sb]
+__device__ __noinline__ void detach_head(u32 sb) {
+  Desc* d = ha_p.desc + sb;
+  st32(&d->is_head, 0);
+  u64 old = a_sub(ctr_of(sb), CTR_HEAD);
+  u32 count = ctr_count(old);
+  u32 size_id = ld32(&d->size_id), chunk_id = ld32(&d->chunk_id);
+  if (size_id == SZ_NONE) return;
+  u32 cls = sb_class(count, ha_p.si[size_id]);
+  if (cls == CLS_FREE) try_mark_free(sb, size_id, chunk_id, true);
+  else set_add(class_set(cls, size_id, chunk_id), sb);
+}
+
+// Takes an empty head or cached head of another (head, class) pair and files
+// its slab as free; false if there is none.  Caller holds the head lock of
+// (ihead, size_id); the victim's head lock is only try-locked.
+__device__ __noinline__ bool steal_idle_head(u32 ihead, u32 size_id) {
+  for (u32 ih = 0; ih < ha_p.nheads; ++ih) {
+    for (u32 c = 0; c < NSIZES; ++c) {
+      if (ih == ihead && c == size_id) continue;
+      u32* lp = hot(ha_p.head_lock, ih, c);
+      for (int which = 0; which < 2; ++which) {
+        u32* wp = hot(which == 0 ? ha_p.cached : ha_p.head, ih, c);
+        u32 w = ld32(wp);
+        if (w == SB_NONE) continue;
+        u32 sb = w & ~HEAD_EXT;
+        if (ctr_count(ld64(ctr_of(sb))) != 0) continue;
+        if (!try_lock(lp)) continue;
+        bool ok = ld32(wp) == w && ctr_count(ld64(ctr_of(sb))) == 0;
+        if (ok) {
+          st32(wp, SB_NONE);
+          u32* l = ha_p.sb_lock + sb;
+          lock(l);
+          detach_head(sb);
+          unlock(l);
+        }
+        unlock(lp);
+        if (ok) return true;
+      }
+    }
+  }
+  return false;
+}
+
+// find_sb_for_size (src/slab.cuh:259-372): roomy -> sparse -> free -> busy,
+// then (for a head needed now) an idle head of another class
+__device__ __noinline__ u32 find_sb_for_size(u32 size_id, u32 chunk_id, u32 ihead, bool steal) {
+  const SizeInfo& s = ha_p.si[size_id];
+  u32 sb;
+again:
+  while ((sb = set_pop(set_roomy(size_id))) != SB_NONE) {
+    Desc* d = ha_p.desc + sb;
+    u32* l = ha_p.sb_lock + sb;
+    lock(l);
+    bool found = false;
+    if (!ld32(&d->is_head) && ld32(&d->size_id) == size_id) {
+      if (ctr_count(ld64(ctr_of(sb))) <= s.roomy_thr) {
+        found = true;
+        st32(&d->is_head, 1);
+        a_add(ctr_of(sb), CTR_HEAD);
+      } else {
+        set_add(set_busy(size_id), sb);
+      }
+    }
+    unlock(l);
+    if (found) return sb;
+  }
+  while ((sb = set_pop(set_sparse(chunk_id))) != SB_NONE) {
+    Desc* d = ha_p.desc + sb;
+    u32* l = ha_p.sb_lock + sb;
+    lock(l);
+    bool found = false;
+    if (!ld32(&d->is_head) && ld32(&d->chunk_id) == chunk_id) {
+      if (ctr_count(ld64(ctr_of(sb))) <= s.sparse_thr) {
+        found = true;
+        st32(&d->is_head, 1);
+        st32(&d->size_id, size_id);
+        a_add(ctr_of(sb), CTR_HEAD);
+      } else {
+        set_add(set_roomy(ld32(&d->size_id)), sb);
+      }
+    }
+    unlock(l);
+    if (found) return sb;
+  }
+  while ((sb = set_pop(0)) != SB_NONE) {
+    Desc* d = ha_p.desc + sb;
+    u32* l = ha_p.sb_lock + sb;
+    lock(l);
+    bool found = false;
+    if (!ld32(&d->is_head)) {
+      found = true;
+      st32(&d->is_head, 1);
+      st32(&d->size_id, size_id);
+      st32(&d->chunk_id, chunk_id);
+      st32(&d->chunk_sz, s.chunk_sz);
+      a_add(ctr_of(sb), CTR_HEAD);
+      a_sub(ctr_of(sb), ctr_chunk_delta(chunk_id));
+    }
+    unlock(l);
+    if (found) return sb;
+  }
+  u64* w = set_words(set_busy(size_id));
+  u32 limit = (s.busy_thr + s.nchunks) / 2;
+  for (u32 i = 0; i < ha_p.nsw; ++i) {
+    u64 word = ld64(w + i);
+    while (word) {
+      u32 b = (u32)__builtin_ctzll(word);
+      u64 m = 1ull << b;
+      word &= ~m;
+      sb = i * 64 + b;
+      if (ctr_count(ld64(ctr_of(sb))) >= limit) continue;  // a head is swapped out at limit
+      if (!(a_and(w + i, ~m) & m)) continue;
+      Desc* d = ha_p.desc + sb;
+      u32* l = ha_p.sb_lock + sb;
+      lock(l);
+      bool found = false;
+      if (!ld32(&d->is_head) && ld32(&d->size_id) == size_id) {
+        found = true;
+        st32(&d->is_head, 1);
+        a_add(ctr_of(sb), CTR_HEAD);
+      }
+      unlock(l);
+      if (found) return sb | HEAD_EXT;
+    }
+  }
+  if (steal && steal_idle_head(ihead, size_id)) {
+    steal = false;
+    goto again;
+  }
+  return SB_NONE;
+}
+
+// new_sb_for_size (src/slab.cuh:378-445)
+__device__ __noinline__ u32 new_sb_for_size(u32 size_id, u32 ihead) {
+  const SizeInfo& s = ha_p.si[size_id];
+  u32* hp = hot(ha_p.head, ihead, size_id);
+  u32* lp = hot(ha_p.head_lock, ihead, size_id);
+  u32* cp = hot(ha_p.cached, ihead, size_id);
+  u32 cur = ld32(hp);
+  if (try_lock(lp)) {
+    cur = ld32(hp);
+    u32 nh;
+    if (cur == SB_NONE || ctr_count(ld64(ctr_of(cur & ~HEAD_EXT))) >= head_limit(cur, s)) {
+      nh = ld32(cp);
+      st32(cp, SB_NONE);
+      if (nh == SB_NONE) nh = find_sb_for_size(size_id, s.chunk_id, ihead, true);
+      if (nh != SB_NONE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        st32(hp, nh);
+        if (cur != SB_NONE) {
+          u32 csb = cur & ~HEAD_EXT;
+          u32* l = ha_p.sb_lock + csb;
+          lock(l);
+          detach_head(csb);
+          unlock(l);
+        }
+        st32(cp, find_sb_for_size(size_id, s.chunk_id, ihead, false));
+      }
+    } else {
+      nh = ld32(hp);
+    }
+    unlock(lp);
+    return nh;
+  }
+  while (ld32(hp) == cur && ld32(lp) != 0) __builtin_amdgcn_s_sleep(2);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return ld32(hp);
+}
+
+// ------------------------------------------------------------- malloc path
+// One size class of one wave (hamalloc_small, src/halloc.cu:125-182).  The
+// active lanes are exactly the bitmap g; sid, g and ihead are wave-uniform.
+//
+// Lane l takes ticket cv = old + l, the group adding (highest lane + 1).  With
+// runs of 64 blocks (ichunk_init, src/halloc.cu:107-122, tmod = 64) the lanes
+// fill the tail of run A = (old >> 6) from slot (old & 63) and the head of run
+// B = A + 1, contiguously: lane l sits at chunk st + l*k of its segment, where
+// st is the segment's (virtual) start.  Each probe is then one atomic per
+// 64-bit word touched, its mask the union of the segment's lanes in the word.
+HA_DEV void* malloc_group(u32 sid, u64 g, u32 ihead, u32 me) {
+  const SizeInfo& s = ha_p.si[sid];
+  const u32 k = s.ncib, nchunks = s.nchunks, nwb = nchunks >> 6, nwn = nchunks >> 4;
+  const u32 lead = (u32)__builtin_ctzll(g);
+  const u32 last = 63u - (u32)__builtin_clzll(g);
+  u32 hword = rfl(ld32(hot(ha_p.head, ihead, sid)));
+  // tickets (size_ctr_inc, src/halloc.cu:71-98): one atomic per group
+  u32 old = 0;
+  if (me == lead) old = a_add32(hot(ha_p.ticket, ihead, sid), last + 1);
+  old = rdl(old, lead);
+  const u32 slot0 = old & (RUN - 1), split = RUN - slot0;
+  const u32 runA = (((old & ~(RUN - 1)) * THREAD_FREQ) * k) % nchunks;
+  const u32 runB = ((((old & ~(RUN - 1)) + RUN) * THREAD_FREQ) * k) % nchunks;
+  const bool inA = me < split;
+  const u64 amask = split >= 64 ? ~0ull : ((1ull << split) - 1);
+  const u64 myseg = inA ? amask : ~amask;
+  u32 itry = 0;
+  u64 want = g;
+  void* p = nullptr;
+  u32 my_ich = 0;
+  while (want) {
+    bool need = false;
+    u64 reserved = 0;
+    const u32 hsb = hword == SB_NONE ? SB_NONE : (hword & ~HEAD_EXT);
+    const u32 limit = head_limit(hword, s);
+    if (hsb == SB_NONE) {
+      need = true;
+    } else {
+      u64* bw = ha_p.bits + (u64)hsb * ha_p.bit_words;
+      u64* nw = ha_p.nibs + (u64)hsb * ha_p.nib_words;
+      u64 P = want;
+      u64 off = 0;
+      // sb_alloc_in (src/slab.cuh:453-535), all probing lanes in lock step
+      for (;;) {
+        const u32 pA = (u32)(((u64)runA + off) % nchunks), pB = (u32)(((u64)runB + off) % nchunks);
+        const u32 st = inA ? pA + slot0 * k : pB - split * k;
+        u32 wb, ib;
+        u64 ub;
+        run_word_bits(P & myseg, me, st, k, wb, ub, ib);
+        const u32 wreal = wb >= nwb ? wb - nwb : wb;
+        const bool inP = (P >> me) & 1;
+        u64 oldw = 0;
+        if (inP && ib == me) oldw = a_or(bw + wreal, ub);
+        oldw = shfl64(oldw, ib & 63);
+        u32 c = st + me * k;
+        c = c >= nchunks ? c - nchunks : c;
+        const u64 mym = ((1ull << k) - 1) << (c & 63);
+        const bool ok = inP && !(oldw & mym);
+        const u64 S = ballot(ok);
+        const u64 F = P & ~S;
+        if (F) {
+          // give back the bits the failing lanes set (src/slab.cuh:480-483)
+          u32 wf, iff;
+          u64 uf;
+          run_word_bits(F & myseg, me, st, k, wf, uf, iff);
+          const u64 clr = uf & ~oldw;
+          if (((F >> me) & 1) && iff == me && clr) a_and(bw + wreal, ~clr);
+        }
+        if (S) {
+          // record the block lengths (sb_set_alloc_size, src/slab.cuh:60-70)
+          u32 wn, inn;
+          u64 un;
+          run_word_nibs(S & myseg, me, st, k, (u64)k, wn, un, inn);
+          const u32 wnreal = wn >= nwn ? wn - nwn : wn;
+          if (ok && inn == me) a_or(nw + wnreal, un);
+          if (ok) my_ich = c;
+        }
+        reserved |= S;
+        P = F;
+        if (!P) break;
+        if (itry % CHECK_NTRIES == CHECK_NTRIES - 1) {
+          if (ctr_count(ld64(ctr_of(hsb))) >= limit) break;
+        }
+        const u32 step = ((itry + 1) * STEP_FREQ * k) % s.hash_step;
+        off = (u64)(itry + 1) * step;
+        if (++itry % MAX_NTRIES >= MAX_NTRIES - 1) break;
+      }
+      if (P) need = true;
+      if (reserved) {
+        // sb_ctr_inc (src/slab.cuh:122-155): one atomic for the group
+        const u32 change = k * (u32)__builtin_popcountll(reserved);
+        u64 oldc = 0;
+        if (me == lead) oldc = a_add(ctr_of(hsb), (u64)change << CTR_COUNT_SH);
+        oldc = rdl64(oldc, lead);
+        const bool mine = (reserved >> me) & 1;
+        if (ctr_chunk(oldc) != s.chunk_id) {
+          // the slab was freed under us: give the chunks back
+          if (mine) {
+            a_and(nw + my_ich / 16, ~(0xFull << (4 * (my_ich % 16))));
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            a_and(bw + my_ich / 64, ~(((1ull << k) - 1) << (my_ich % 64)));
+          }
+          if (me == lead) {
+            u64 o = a_sub(ctr_of(hsb), (u64)change << CTR_COUNT_SH);
+            ctr_transitions(hsb, o, change);
+          }
+          need = true;
+        } else {
+          const u32 oc = ctr_count(oldc);
+          if (oc < limit && oc + change >= limit) need = true;
+          if (mine) p = ha_p.arena + (u64)hsb * ha_p.sb_sz + chunk_mul(my_ich, s.chunk_sz);
+          want &= ~reserved;
+        }
+      }
+    }
+    if (need) {
+      u32 nh = 0;
+      if (me == lead) nh = new_sb_for_size(sid, ihead);
+      hword = rdl(nh, lead);
+      if (hword == SB_NONE) want = 0;
+    }
+  }
+  return p;
+}
+
+HA_DEV void* malloc_small(u32 nbytes) {
+  const u32 me = lane_id();
+  const u32 sid = size_id_from_nbytes(nbytes);
+  const u32 ihead = rfl(xcc_id() % ha_p.nheads);
+  void* res = nullptr;
+  u64 rem = ballot(true);
+  while (rem) {
+    u32 l0 = (u32)__builtin_ctzll(rem);
+    u32 gsid = rdl(sid, l0);
+    u64 g = ballot(sid == gsid) & rem;
+    if ((g >> me) & 1) res = malloc_group(gsid, g, ihead, me);
+    rem &= ~g;
+  }
+  return res;
+}
+
+// --------------------------------------------------------------- free path
+// hafree_small (src/halloc.cu:211-230) for the active lanes; off = p - arena.
+HA_DEV void free_small(u64 off) {
+  const u32 me = lane_id();
+  const u32 sb = (u32)(off >> ha_p.sb_sz_sh);
+  Desc* d = ha_p.desc + sb;
+  const u32 cs = ld32(&d->chunk_sz);
+  const bool valid = cs != 0;
+  const u32 ich = valid ? chunk_div((u32)(off & (ha_p.sb_sz - 1)), cs) : 0u;
+  u64* bw = ha_p.bits + (u64)sb * ha_p.bit_words;
+  u64* nw = ha_p.nibs + (u64)sb * ha_p.nib_words;
+  const u64 A = ballot(valid);
+  u32 nch = 0;
+  // fast path: the lanes free one slab's blocks laid out as at most two runs
+  // (what one class group of a wave allocates): one atomic per word touched
+  bool fast = false;
+  u32 k = 0, st = 0;
+  u64 myseg = 0;
+  if (A) {
+    const u32 l0 = (u32)__builtin_ctzll(A);
+    const u32 sb0 = rdl(sb, l0);
+    if (ballot(valid && sb == sb0) == A) {
+      const u32 sid0 = rfl(ld32(&ha_p.desc[sb0].size_id));
+      if (sid0 < NSIZES) {
+        k = ha_p.si[sid0].ncib;
+        const u32 sa = rdl(ich, l0) - l0 * k;
+        const bool ina = valid && ich == sa + me * k;
+        const u64 Am = ballot(ina);
+        const u64 R = A & ~Am;
+        u32 sbb = 0;
+        u64 Bm = 0;
+        if (R) {
+          const u32 l1 = (u32)__builtin_ctzll(R);
+          sbb = rdl(ich, l1) - l1 * k;
+          Bm = ballot(valid && !ina && ich == sbb + me * k);
+        }
+        fast = (Am | Bm) == A;
+        st = ina ? sa : sbb;
+        myseg = ina ? Am : Bm;
+      }
+    }
+  }
+  if (fast) {
+    u32 wn, inn;
+    u64 un;
+    run_word_nibs(myseg, me, st, k, 0xFull, wn, un, inn);
+    u64 oldn = 0;
+    if (valid && inn == me) oldn = a_and(nw + wn, ~un);
+    oldn = shfl64(oldn, inn & 63);
+    nch = valid ? (u32)((oldn >> (4 * (ich & 15))) & 0xF) : 0u;
+    if (ballot(valid && nch == k) == A) {
+      u32 wb, ib;
+      u64 ub;
+      run_word_bits(myseg, me, st, k, wb, ub, ib);
+      if (valid && ib == me) a_and(bw + wb, ~ub);
+    } else if (nch) {
+      a_and(bw + ich / 64, ~(((1ull << nch) - 1) << (ich % 64)));
+    }
+  } else if (valid) {
+    const u64 o = a_and(nw + ich / 16, ~(0xFull << (4 * (ich % 16))));
+    nch = (u32)((o >> (4 * (ich % 16))) & 0xF);
+    if (nch) a_and(bw + ich / 64, ~(((1ull << nch) - 1) << (ich % 64)));
+  }
+  // sb_ctr_dec (src/slab.cuh:161-224): one atomic per slab touched
+  const u64 D = ballot(nch != 0);
+  u64 rem = D;
+  u32 my_change = 0;
+  bool leader = false;
+  while (rem) {
+    u32 l = (u32)__builtin_ctzll(rem);
+    u32 b = rdl(sb, l);
+    u64 G = ballot(sb == b) & rem;
+    u32 ch = 0;
+    u64 Gr = G;
+    while (Gr) {
+      u32 l2 = (u32)__builtin_ctzll(Gr);
+      u32 v = rdl(nch, l2);
+      u64 V = ballot(nch == v) & Gr;
+      ch += v * (u32)__builtin_popcountll(V);
+      Gr &= ~V;
+    }
+    if (me == l) { leader = true; my_change = ch; }
+    rem &= ~G;
+  }
+  u64 oldc = 0;
+  if (leader) oldc = a_sub(ctr_of(sb), (u64)my_change << CTR_COUNT_SH);
+  u64 T = ballot(leader && needs_transition(sb, oldc, my_change));
+  while (T) {
+    u32 l = (u32)__builtin_ctzll(T);
+    if (me == l) ctr_transitions(sb, oldc, my_change);
+    T &= T - 1;
+  }
+}
+
+}  // namespace ha
+
+// ---------------------------------------------------------- public device API
+__device__ void* hamalloc(size_t nbytes) {
+  // requests above 3072 B go to the HIP device heap (src/halloc.cu:185-195)
+  if (nbytes > ha::MAX_BLOCK_SZ) return malloc(nbytes);
+  return ha::malloc_small((ha::u32)nbytes);
+}
+
+__device__ void hafree(void* p) {
+  if (!p) return;  // src/halloc.cu:240
+  ha::u64 off = (ha::u64)((char*)p - ha::ha_p.arena);
+  if (off < ha::ha_p.arena_bytes) ha::free_small(off);
+  else free(p);
+}
+
+// ------------------------------------------------------------------ host side
+namespace ha {
+
+static std::mutex g_mu;
+static Params g_host;          // host copy of ha_p
+static void* g_meta = nullptr;  // one allocation for all metadata
+static thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+#define HA_HIP(call)                                                                              \
+  do {                                                                                            \
+    hipError_t e_ = (call);                                                                       \
+    if (e_ != hipSuccess) return ::ha::fail(HA_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// largest prime p in [3, n) with n % p != 0 and nb % p != 0 (src/utils.cu:8-24)
+static u32 max_prime_below(u32 n, u32 nb) {
+  for (u32 p = n - 1; p >= 3 && p < n; p--) {
+    u32 max_d = (u32)std::floor(std::sqrt((double)p));
+    bool is_prime = true;
+    for (u32 dd = 2; dd <= max_d; dd++)
+      if (p % dd == 0) { is_prime = false; break; }
+    if (is_prime && n % p && nb % p) return p;
+  }
+  return 0;
+}
+static u32 chunk_val(u32 chunk_sz) {
+  u32 div3 = chunk_sz % 3 ? 1 : 3;
+  if (chunk_sz % 3 == 0) chunk_sz /= 3;
+  u32 sh = 0;
+  for (; (1u << sh) < chunk_sz; sh++) {}
+  return div3 << 16 | sh;
+}
+// size table (src/halloc.cu:363-394) with blocks per slab rounded to RUN
+static bool build_size_infos(u32 sb_sz, const ha_opts_c& o, SizeInfo* out) {
+  for (u32 sid = 0; sid < NSIZES; ++sid) {
+    SizeInfo& s = out[sid];
+    std::memset(&s, 0, sizeof s);
+    u32 unit = 1u << (sid / 2 + 3);
+    s.block_sz = (sid % 2 ? 3 : 2) * unit;
+    s.tmod = RUN;
+    s.nblocks = sb_sz / s.block_sz / RUN * RUN;
+    s.chunk_id = sid % 2 + (sid < NSIZES / 2 ? 0 : 2);
+    s.chunk_bytes = (s.chunk_id % 2 ? 3 : 2) * (s.chunk_id / 2 ? 128 : 8);
+    s.chunk_sz = chunk_val(s.chunk_bytes);
+    s.ncib = s.block_sz / s.chunk_bytes;
+    s.nchunks = s.nblocks * s.ncib;
+    u32 p = max_prime_below(s.nblocks / 256 + s.nblocks / 64, s.nblocks);
+    if (!p || !s.nblocks) return false;
+    s.hash_step = s.ncib * p;
+    s.roomy_thr = (u32)(o.roomy_fraction * s.nchunks);
+    s.busy_thr = (u32)(o.busy_fraction * s.nchunks);
+    s.sparse_thr = (u32)(o.sparse_fraction * s.nchunks);
+  }
+  return true;
+}
+
+__global__ void init_state_k(Params P) {
+  u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < P.nsbs) {
+    P.ctr[(u64)i * CTR_STRIDE] = CTR_CHUNK_NONE << CTR_CHUNK_SH;
+    P.desc[i] = Desc{SZ_NONE, SZ_NONE, 0u, 0u};
+    P.sb_lock[i] = 0;
+  }
+  if (i < P.nsw) {
+    u32 lo = i * 64;
+    u64 w = 0;
+    if (lo < P.nsbs) w = P.nsbs - lo >= 64 ? ~0ull : ((1ull << (P.nsbs - lo)) - 1);
+    P.sets[i] = w;  // the free set
+  }
+  if (i < P.nheads * NSIZES) {
+    P.head[i * HOT_STRIDE] = SB_NONE;
+    P.cached[i * HOT_STRIDE] = SB_NONE;
+    P.head_lock[i * HOT_STRIDE] = 0;
+    P.ticket[i * HOT_STRIDE] = TICKET_INIT;
+  }
+}
+
+static u64 align_up(u64 x, u64 a) { return (x + a - 1) / a * a; }
+
+static int shutdown_locked() {
+  if (!g_host.initialized) return HA_OK;
+  Params zero;
+  std::memset(&zero, 0, sizeof zero);
+  HA_HIP(hipDeviceSynchronize());
+  HA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(ha_p), &zero, sizeof zero));
+  if (g_host.arena) HA_HIP(hipFree(g_host.arena));
+  if (g_meta) HA_HIP(hipFree(g_meta));
+  g_meta = nullptr;
+  std::memset(&g_host, 0, sizeof g_host);
+  return HA_OK;
+}
+
+static int init_locked(const ha_opts_c& o) {
+  if (g_host.initialized) {
+    int rc = shutdown_locked();
+    if (rc) return rc;
+  }
+  if (o.sb_sz_sh < (int)MIN_SB_SZ_SH || o.sb_sz_sh > (int)MAX_SB_SZ_SH)
+    return fail(HA_E_INVALID, "sb_sz_sh must be in 21..26");
+  if (o.nheads < 1 || o.nheads > (int)MAX_NHEADS) return fail(HA_E_INVALID, "nheads must be in 1..64");
+  if (!(o.halloc_fraction > 0 && o.halloc_fraction <= 1) || !(o.sparse_fraction >= 0) ||
+      !(o.sparse_fraction <= o.roomy_fraction) || !(o.roomy_fraction <= o.busy_fraction) ||
+      !(o.busy_fraction <= 1))
+    return fail(HA_E_INVALID, "fractions must satisfy 0 <= sparse <= roomy <= busy <= 1, 0 < halloc <= 1");
+  int dev = 0;
+  HA_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HA_HIP(hipGetDeviceProperties(&prop, dev));
+  u64 memory = std::min<u64>(o.memory, 3ull * (u64)prop.totalGlobalMem / 4ull);  // src/halloc.cu:282
+  u64 halloc_memory = (u64)(o.halloc_fraction * (double)memory);
+  u64 cuda_memory = memory - halloc_memory;                                        // src/halloc.cu:285-289
+  Params P;
+  std::memset(&P, 0, sizeof P);
+  P.sb_sz_sh = (u32)o.sb_sz_sh;
+  P.sb_sz = 1u << P.sb_sz_sh;
+  P.nsbs = (u32)(halloc_memory / P.sb_sz);
+  if (P.nsbs == 0 || P.nsbs > MAX_NSBS) return fail(HA_E_INVALID, "memory gives no slab or too many slabs");
+  P.nheads = (u32)o.nheads;
+  P.nsw = (P.nsbs + 63) / 64;
+  P.bit_words = P.sb_sz / BLOCK_STEP / 64;
+  P.nib_words = P.sb_sz / BLOCK_STEP / 16;
+  P.cuda_memory = cuda_memory;
+  if (!build_size_infos(P.sb_sz, o, P.si)) return fail(HA_E_INVALID, "no size table for this slab size");
+  if (cuda_memory) {
+    hipError_t e = hipDeviceSetLimit(hipLimitMallocHeapSize, cuda_memory);
+    if (e != hipSuccess) return fail(HA_E_HIP, std::string("hipDeviceSetLimit: ") + hipGetErrorString(e));
+  }
+  P.arena_bytes = (u64)P.nsbs * P.sb_sz;
+  // metadata offsets, each section 256 B aligned
+  u64 off = 0;
+  u64 o_bits = off; off = align_up(off + (u64)P.nsbs * P.bit_words * 8, 256);
+  u64 o_nibs = off; off = align_up(off + (u64)P.nsbs * P.nib_words * 8, 256);
+  u64 o_ctr = off; off = align_up(off + (u64)P.nsbs * CTR_STRIDE * 8, 256);
+  u64 o_desc = off; off = align_up(off + (u64)P.nsbs * sizeof(Desc), 256);
+  u64 o_lock = off; off = align_up(off + (u64)P.nsbs * 4, 256);
+  u64 o_sets = off; off = align_up(off + (u64)NSETS * P.nsw * 8, 256);
+  u64 hot_bytes = (u64)P.nheads * NSIZES * HOT_STRIDE * 4;
+  u64 o_head = off; off = align_up(off + hot_bytes, 256);
+  u64 o_cached = off; off = align_up(off + hot_bytes, 256);
+  u64 o_hlock = off; off = align_up(off + hot_bytes, 256);
+  u64 o_ticket = off; off = align_up(off + hot_bytes, 256);
+  void* arena = nullptr;
+  if (hipMalloc(&arena, P.arena_bytes) != hipSuccess) return fail(HA_E_NOMEM, "hipMalloc of the arena failed");
+  void* meta = nullptr;
+  if (hipMalloc(&meta, off) != hipSuccess) {
+    (void)hipFree(arena);
+    return fail(HA_E_NOMEM, "hipMalloc of the metadata failed");
+  }
+  char* m = (char*)meta;
+  P.arena = (char*)arena;
+  P.bits = (u64*)(m + o_bits);
+  P.nibs = (u64*)(m + o_nibs);
+  P.ctr = (u64*)(m + o_ctr);
+  P.desc = (Desc*)(m + o_desc);
+  P.sb_lock = (u32*)(m + o_lock);
+  P.sets = (u64*)(m + o_sets);
+  P.head = (u32*)(m + o_head);
+  P.cached = (u32*)(m + o_cached);
+  P.head_lock = (u32*)(m + o_hlock);
+  P.ticket = (u32*)(m + o_ticket);
+  P.initialized = 1;
+  HA_HIP(hipMemset(meta, 0, off));
+  u32 n = std::max(std::max(P.nsbs, P.nsw), P.nheads * NSIZES);
+  init_state_k<<<(n + 255) / 256, 256>>>(P);
+  HA_HIP(hipGetLastError());
+  HA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(ha_p), &P, sizeof P));
+  HA_HIP(hipDeviceSynchronize());
+  g_host = P;
+  g_meta = meta;
+  return HA_OK;
+}
+
+const Params& host_params() { return g_host; }
+
+// free bytes and free-slab bytes over the heap (src/statistics.cuh:34-48)
+__global__ void free_mem_k(u64* out) {
+  u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ha_p.nsbs) return;
+  u32 cs = ha_p.desc[i].chunk_sz;
+  u32 used = cs ? chunk_mul(ctr_count(ha_p.ctr[(u64)i * CTR_STRIDE]), cs) : 0u;
+  atomicAdd((unsigned long long*)&out[0], (unsigned long long)(ha_p.sb_sz - used));
+  if (cs == 0) atomicAdd((unsigned long long*)&out[1], (unsigned long long)ha_p.sb_sz);
+}
+// largest block the device heap can serve now, by bisection (src/statistics.cuh:14-30)
+__global__ void max_alloc_k(u64 hi, u64* out) {
+  if (blockIdx.x || threadIdx.x) return;
+  u64 lo = 0;
+  const u64 min_diff = 1ull << 20;
+  while (hi - lo > min_diff) {
+    u64 mid = lo + (hi - lo) / 2;
+    void* q = malloc(mid);
+    if (q) { lo = mid; free(q); } else hi = mid;
+  }
+  out[0] = lo;
+}
+
+static int stats_locked(u64* free_mem, u64* free_slab_mem) {
+  if (!g_host.initialized) return fail(HA_E_STATE, "allocator not initialized");
+  u64* d = nullptr;
+  HA_HIP(hipMalloc(&d, 16));
+  HA_HIP(hipMemset(d, 0, 16));
+  free_mem_k<<<(g_host.nsbs + 127) / 128, 128>>>(d);
+  HA_HIP(hipGetLastError());
+  u64 h[2];
+  HA_HIP(hipMemcpy(h, d, 16, hipMemcpyDeviceToHost));
+  HA_HIP(hipFree(d));
+  *free_mem = h[0];
+  if (free_slab_mem) *free_slab_mem = h[1];
+  return HA_OK;
+}
+
+}  // namespace ha
+
+// --------------------------------------------------------------------- C ABI
+extern "C" {
+
+void ha_default_opts(ha_opts_c* o) {
+  halloc_opts_t d;
+  o->memory = d.memory;
+  o->halloc_fraction = d.halloc_fraction;
+  o->busy_fraction = d.busy_fraction;
+  o->roomy_fraction = d.roomy_fraction;
+  o->sparse_fraction = d.sparse_fraction;
+  o->sb_sz_sh = d.sb_sz_sh;
+  o->nheads = 8;
+}
+
+int ha_init_c(const ha_opts_c* o) {
+  if (!o) return ha::fail(HA_E_INVALID, "null options");
+  std::lock_guard<std::mutex> g(ha::g_mu);
+  return ha::init_locked(*o);
+}
+
+int ha_shutdown_c(void) {
+  std::lock_guard<std::mutex> g(ha::g_mu);
+  return ha::shutdown_locked();
+}
+
+int ha_free_bytes_c(uint64_t* out) {
+  if (!out) return ha::fail(HA_E_INVALID, "null output");
+  std::lock_guard<std::mutex> g(ha::g_mu);
+  return ha::stats_locked((ha::u64*)out, nullptr);
+}
+
+int ha_extfrag_c(int ideal, double* out) {
+  if (!out) return ha::fail(HA_E_INVALID, "null output");
+  std::lock_guard<std::mutex> g(ha::g_mu);
+  ha::u64 free_mem = 0, free_slabs = 0;
+  int rc = ha::stats_locked(&free_mem, &free_slabs);
+  if (rc) return rc;
+  ha::u64 max_alloc = 0;
+  if (ha::g_host.cuda_memory) {
+    ha::u64* d = nullptr;
+    HA_HIP(hipMalloc(&d, 8));
+    HA_HIP(hipMemset(d, 0, 8));
+    ha::max_alloc_k<<<1, 64>>>(ha::g_host.cuda_memory, d);
+    HA_HIP(hipGetLastError());
+    HA_HIP(hipMemcpy(&max_alloc, d, 8, hipMemcpyDeviceToHost));
+    HA_HIP(hipFree(d));
+  }
+  if (ideal) max_alloc += free_slabs;
+  free_mem += ha::g_host.cuda_memory;
+  *out = free_mem ? 1.0 - (double)max_alloc / (double)free_mem : 0.0;
+  return HA_OK;
+}
+
+int ha_geometry_c(ha_geometry* g) {
+  if (!g) return ha::fail(HA_E_INVALID, "null output");
+  std::lock_guard<std::mutex> lk(ha::g_mu);
+  const ha::Params& P = ha::g_host;
+  if (!P.initialized) return ha::fail(HA_E_STATE, "allocator not initialized");
+  std::memset(g, 0, sizeof *g);
+  g->nsbs = P.nsbs; g->sb_sz = P.sb_sz; g->sb_sz_sh = P.sb_sz_sh; g->nheads = P.nheads;
+  g->nsw = P.nsw; g->bit_words = P.bit_words; g->nib_words = P.nib_words;
+  g->arena_bytes = P.arena_bytes; g->cuda_memory = P.cuda_memory;
+  for (ha::u32 s = 0; s < ha::NSIZES; ++s) std::memcpy(g->size_table + 12 * s, &P.si[s], 12 * sizeof(uint32_t));
+  return HA_OK;
+}
+
+const char* ha_last_error(void) { return ha::g_err.c_str(); }
+
+}  // extern "C"
+
+// ------------------------------------------------ C++ host API of halloc.h
+void ha_init(halloc_opts_t opts) {
+  ha_opts_c o;
+  ha_default_opts(&o);
+  o.memory = opts.memory;
+  o.halloc_fraction = opts.halloc_fraction;
+  o.busy_fraction = opts.busy_fraction;
+  o.roomy_fraction = opts.roomy_fraction;
+  o.sparse_fraction = opts.sparse_fraction;
+  o.sb_sz_sh = opts.sb_sz_sh;
+  if (ha_init_c(&o) != HA_OK) {
+    std::fprintf(stderr, "ha_init: %s\n", ha_last_error());
+    std::exit(-1);
+  }
+}
+
+void ha_shutdown(void) {
+  if (ha_shutdown_c() != HA_OK) std::fprintf(stderr, "ha_shutdown: %s\n", ha_last_error());
+}
+
+double ha_extfrag(bool ideal) {
+  double v = 0;
+  if (ha_extfrag_c(ideal ? 1 : 0, &v) != HA_OK) {
+    std::fprintf(stderr, "ha_extfrag: %s\n", ha_last_error());
+    std::exit(-1);
+  }
+  return v;
+}

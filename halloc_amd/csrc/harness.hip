@@ -1,0 +1,370 @@
+// harness.hip -- workload drivers of libhalloc_amd.so: the counterparts of the
+// reference's test harness kernels (tst/common/common.h:361-412,
+// tst/perf/priv-throughput/priv-throughput.cu:14-27, tst/corr/test/test.cu:38-95).
+// Each driver copies host inputs to device memory once, launches on a stream
+// it owns, times its launches with HIP events on that stream, and copies
+// results back.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/halloc.h"
+#include "../../include/halloc_amd.h"
+#include "ha_internal.h"
+
+namespace ha {
+const Params& host_params();
+int fail(int code, const std::string& msg);
+}
+
+using ha::u32;
+using ha::u64;
+
+namespace {
+
+int hfail(int code, const std::string& m) { return ha::fail(code, m); }
+
+#define HH(call)                                                                                 \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess) return hfail(HA_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+__device__ __forceinline__ u32 xcc_id_h() { return __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)); }
+
+// Device buffer that frees itself.
+struct DBuf {
+  void* p = nullptr;
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t n) { return hipMalloc(&p, n ? n : 1); }
+};
+
+struct Stream {
+  hipStream_t s = nullptr;
+  ~Stream() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+// ------------------------------------------------------------------ replay
+// One wavefront replays ops step by step: malloc lanes, then free lanes.
+__global__ void __launch_bounds__(64) replay_k(const u32* ops, int nsteps, int lanes, u32* recs, u32* xcc,
+                                               void** ptrs) {
+  const u32 l = threadIdx.x;
+  const ha::Params& P = ha::ha_p;
+  for (int st = 0; st < nsteps; ++st) {
+    const size_t i = (size_t)st * lanes + l;
+    const u32 kind = ops[2 * i], arg = ops[2 * i + 1];
+    if (l == 0) xcc[st] = xcc_id_h();
+    if (kind == 1) {
+      void* p = hamalloc(arg);
+      ptrs[i] = p;
+      u32 r0 = 0xffffffffu, r1 = 0xffffffffu, r2 = 0xffffffffu;
+      if (p) {
+        u64 off = (u64)((char*)p - P.arena);
+        if (off < P.arena_bytes) {
+          u32 sid = ha::size_id_from_nbytes(arg);
+          r0 = sid;
+          r1 = (u32)(off >> P.sb_sz_sh);
+          r2 = ha::chunk_div((u32)(off & (P.sb_sz - 1)), P.si[sid].chunk_sz);
+        } else {
+          r0 = 16; r1 = 0; r2 = 0;
+        }
+      }
+      recs[4 * i] = r0; recs[4 * i + 1] = r1; recs[4 * i + 2] = r2; recs[4 * i + 3] = 0;
+    }
+    __syncthreads();
+    if (kind == 2) {
+      // record the block's length from its nibble, then free it
+      void* p = ptrs[arg];
+      u32 r0 = 0, r1 = 0xffffffffu, r2 = 0xffffffffu;
+      if (p) {
+        u64 off = (u64)((char*)p - P.arena);
+        if (off < P.arena_bytes) {
+          u32 sb = (u32)(off >> P.sb_sz_sh);
+          u32 cs = __hip_atomic_load(&P.desc[sb].chunk_sz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (cs) {
+            u32 ich = ha::chunk_div((u32)(off & (P.sb_sz - 1)), cs);
+            u64 w = __hip_atomic_load(P.nibs + (u64)sb * P.nib_words + ich / 16, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+            u32 nch = (u32)((w >> (4 * (ich % 16))) & 0xF);
+            if (nch) { r0 = nch; r1 = sb; r2 = ich; }
+          }
+        }
+      }
+      hafree(p);
+      recs[4 * i] = r0; recs[4 * i + 1] = r1; recs[4 * i + 2] = r2; recs[4 * i + 3] = 0;
+      ptrs[i] = nullptr;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------- dump
+__global__ void dump_k(u64* out, u64 total) {
+  const ha::Params& P = ha::ha_p;
+  const u64 n1 = P.nsbs, n2 = 2ull * P.nsbs, n3 = (u64)ha::NSETS * P.nsw, nh = (u64)P.nheads * ha::NSIZES;
+  const u64 n5 = (u64)P.nsbs * P.bit_words;
+  for (u64 i = blockIdx.x * (u64)blockDim.x + threadIdx.x; i < total; i += (u64)gridDim.x * blockDim.x) {
+    u64 j = i, v = 0;
+    if (j < n1) { v = P.ctr[j * ha::CTR_STRIDE]; }
+    else if ((j -= n1) < n2) {
+      const ha::Desc& d = P.desc[j / 2];
+      v = (j & 1) ? ((u64)d.chunk_sz | (u64)d.is_head << 32) : ((u64)d.size_id | (u64)d.chunk_id << 32);
+    } else if ((j -= n2) < n3) { v = P.sets[j]; }
+    else if ((j -= n3) < nh) { v = P.head[j * ha::HOT_STRIDE]; }
+    else if ((j -= nh) < nh) { v = P.cached[j * ha::HOT_STRIDE]; }
+    else if ((j -= nh) < nh) { v = P.ticket[j * ha::HOT_STRIDE]; }
+    else if ((j -= nh) < n5) { v = P.bits[j]; }
+    else { j -= n5; v = P.nibs[j]; }
+    out[i] = v;
+  }
+}
+
+u64 dump_words_of(const ha::Params& P) {
+  return (u64)P.nsbs * 3 + (u64)ha::NSETS * P.nsw + 3ull * P.nheads * ha::NSIZES +
+         (u64)P.nsbs * P.bit_words + (u64)P.nsbs * P.nib_words;
+}
+
+// ---------------------------------------------------------------- plain
+__global__ void plain_k(int nthreads, int size, int iters, unsigned long long* errs,
+                        unsigned long long* nulls) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nthreads) return;
+  u32 bad = 0, nul = 0;
+  const u32 t = (u32)i;
+  for (int it = 0; it < iters; ++it) {
+    uint2* p = (uint2*)hamalloc((size_t)size);
+    if (!p) {
+      nul++;
+      continue;
+    }
+    const uint2 a = make_uint2(t, ~t), b = make_uint2(t ^ 0x9E3779B9u, (u32)it);
+    p[0] = a;
+    p[1] = b;
+    __asm__ __volatile__("" ::: "memory");
+    const uint2 ra = p[0], rb = p[1];
+    if (ra.x != a.x || ra.y != a.y || rb.x != b.x || rb.y != b.y) bad++;
+    hafree(p);
+  }
+  if (bad) atomicAdd(errs, (unsigned long long)bad);
+  if (nul) atomicAdd(nulls, (unsigned long long)nul);
+}
+
+// -------------------------------------------------------- arrays of blocks
+__global__ void malloc_sizes_k(const u32* sizes, int n, void** ptrs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  void* p = hamalloc(sizes[i]);
+  ptrs[i] = p;
+  if (p) *(u32*)p = sizes[i];
+}
+__global__ void free_ptrs_k(void* const* ptrs, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  hafree(ptrs[i]);
+}
+__global__ void read_words_k(void* const* ptrs, int n, u32* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = ptrs[i] ? *(const u32*)ptrs[i] : 0u;
+}
+
+// ------------------------------------------------------------------ probe
+__device__ __forceinline__ u32 mix32(u32 x) {
+  x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+  return x;
+}
+__global__ void probe_k(u64* buf, u64 mask, int iters, int shape, u64* sink) {
+  const u32 tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const u32 lane = threadIdx.x & 63, wave = tid >> 6;
+  u64 acc = 0;
+  for (int i = 0; i < iters; ++i) {
+    u64 w;
+    if (shape == 0) w = mix32(tid * 131u + (u32)i * 7919u + 1u);
+    else if (shape == 1) w = (u64)mix32(wave * 977u + (u32)i * 31u + 3u) * 64u + lane;
+    else w = 0;
+    w &= mask;
+    acc ^= __hip_atomic_fetch_or(buf + w, 1ull << (lane & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  sink[tid] = acc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ha_replay(const uint32_t* ops, int nsteps, int lanes, uint32_t* recs, uint32_t* xcc) {
+  if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
+  if (!ops || !recs || !xcc || nsteps < 0 || lanes < 1 || lanes > 64) return hfail(HA_E_INVALID, "bad replay arguments");
+  const size_t nops = (size_t)nsteps * lanes;
+  for (size_t i = 0; i < nops; ++i) {
+    if (ops[2 * i] > 2) return hfail(HA_E_INVALID, "bad op kind");
+    if (ops[2 * i] == 2 && (ops[2 * i + 1] >= nops || ops[2 * i + 1] / lanes >= i / lanes))
+      return hfail(HA_E_INVALID, "a free must name a malloc of an earlier step");
+  }
+  DBuf dops, drecs, dxcc, dptrs;
+  HH(dops.alloc(nops * 8));
+  HH(drecs.alloc(nops * 16));
+  HH(dxcc.alloc((size_t)nsteps * 4));
+  HH(dptrs.alloc(nops * 8));
+  HH(hipMemcpy(dops.p, ops, nops * 8, hipMemcpyHostToDevice));
+  HH(hipMemset(dptrs.p, 0, nops * 8));
+  HH(hipMemset(drecs.p, 0, nops * 16));
+  replay_k<<<1, lanes>>>((const u32*)dops.p, nsteps, lanes, (u32*)drecs.p, (u32*)dxcc.p, (void**)dptrs.p);
+  HH(hipGetLastError());
+  HH(hipDeviceSynchronize());
+  HH(hipMemcpy(recs, drecs.p, nops * 16, hipMemcpyDeviceToHost));
+  HH(hipMemcpy(xcc, dxcc.p, (size_t)nsteps * 4, hipMemcpyDeviceToHost));
+  return HA_OK;
+}
+
+int64_t ha_dump_words(void) {
+  const ha::Params& P = ha::host_params();
+  if (!P.initialized) return hfail(HA_E_STATE, "allocator not initialized");
+  return (int64_t)dump_words_of(P);
+}
+
+int ha_dump(uint64_t* out, int64_t nwords) {
+  const ha::Params& P = ha::host_params();
+  if (!P.initialized) return hfail(HA_E_STATE, "allocator not initialized");
+  const u64 total = dump_words_of(P);
+  if (!out || nwords < (int64_t)total) return hfail(HA_E_INVALID, "dump buffer too small");
+  DBuf d;
+  HH(d.alloc(total * 8));
+  HH(hipDeviceSynchronize());
+  dump_k<<<1024, 256>>>((u64*)d.p, total);
+  HH(hipGetLastError());
+  HH(hipMemcpy(out, d.p, total * 8, hipMemcpyDeviceToHost));
+  return HA_OK;
+}
+
+int ha_run_plain(int nthreads, int bs, int size, int iters, int warmup, int launches, float* ms,
+                 ha_plain_result* res) {
+  if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
+  if (nthreads < 1 || bs < 1 || bs > 1024 || size < 16 || size > 3072 || iters < 1 ||
+      warmup < 0 || launches < 1 || !res)
+    return hfail(HA_E_INVALID, "bad plain-run arguments");
+  DBuf dc;
+  HH(dc.alloc(16));
+  HH(hipMemset(dc.p, 0, 16));
+  Stream s;
+  HH(hipStreamCreate(&s.s));
+  hipEvent_t e0, e1;
+  HH(hipEventCreate(&e0));
+  HH(hipEventCreate(&e1));
+  const int grid = (nthreads + bs - 1) / bs;
+  unsigned long long* c = (unsigned long long*)dc.p;
+  for (int w = 0; w < warmup; ++w) plain_k<<<grid, bs, 0, s.s>>>(nthreads, size, iters, c, c + 1);
+  HH(hipGetLastError());
+  HH(hipStreamSynchronize(s.s));
+  HH(hipMemsetAsync(dc.p, 0, 16, s.s));
+  double total = 0;
+  int rc = HA_OK;
+  HH(hipStreamSynchronize(s.s));
+  const auto w0 = std::chrono::steady_clock::now();
+  for (int l = 0; l < launches; ++l) {
+    HH(hipEventRecord(e0, s.s));
+    plain_k<<<grid, bs, 0, s.s>>>(nthreads, size, iters, c, c + 1);
+    HH(hipEventRecord(e1, s.s));
+    HH(hipEventSynchronize(e1));
+    float t = 0;
+    HH(hipEventElapsedTime(&t, e0, e1));
+    if (ms) ms[l] = t;
+    total += t;
+  }
+  HH(hipStreamSynchronize(s.s));
+  const auto w1 = std::chrono::steady_clock::now();
+  HH(hipGetLastError());
+  unsigned long long h[2];
+  HH(hipMemcpy(h, dc.p, 16, hipMemcpyDeviceToHost));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  res->errors = h[0];
+  res->nulls = h[1];
+  res->total_ms = total;
+  res->wall_ms = std::chrono::duration<double, std::milli>(w1 - w0).count();
+  return rc;
+}
+
+int ha_malloc_sizes(const uint32_t* sizes, int n, int bs, uint64_t* ptrs) {
+  if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
+  if (!sizes || !ptrs || n < 0 || bs < 1 || bs > 1024) return hfail(HA_E_INVALID, "bad arguments");
+  if (n == 0) return HA_OK;
+  DBuf ds, dp;
+  HH(ds.alloc((size_t)n * 4));
+  HH(dp.alloc((size_t)n * 8));
+  HH(hipMemcpy(ds.p, sizes, (size_t)n * 4, hipMemcpyHostToDevice));
+  malloc_sizes_k<<<(n + bs - 1) / bs, bs>>>((const u32*)ds.p, n, (void**)dp.p);
+  HH(hipGetLastError());
+  HH(hipDeviceSynchronize());
+  HH(hipMemcpy(ptrs, dp.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+  return HA_OK;
+}
+
+int ha_free_ptrs(const uint64_t* ptrs, int n, int bs) {
+  if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
+  if (!ptrs || n < 0 || bs < 1 || bs > 1024) return hfail(HA_E_INVALID, "bad arguments");
+  if (n == 0) return HA_OK;
+  DBuf dp;
+  HH(dp.alloc((size_t)n * 8));
+  HH(hipMemcpy(dp.p, ptrs, (size_t)n * 8, hipMemcpyHostToDevice));
+  free_ptrs_k<<<(n + bs - 1) / bs, bs>>>((void* const*)dp.p, n);
+  HH(hipGetLastError());
+  HH(hipDeviceSynchronize());
+  return HA_OK;
+}
+
+int ha_read_words(const uint64_t* ptrs, int n, uint32_t* out) {
+  if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
+  if (!ptrs || !out || n < 0) return hfail(HA_E_INVALID, "bad arguments");
+  if (n == 0) return HA_OK;
+  DBuf dp, dout;
+  HH(dp.alloc((size_t)n * 8));
+  HH(dout.alloc((size_t)n * 4));
+  HH(hipMemcpy(dp.p, ptrs, (size_t)n * 8, hipMemcpyHostToDevice));
+  read_words_k<<<(n + 255) / 256, 256>>>((void* const*)dp.p, n, (u32*)dout.p);
+  HH(hipGetLastError());
+  HH(hipMemcpy(out, dout.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+  return HA_OK;
+}
+
+int ha_probe_atomics(double* out) {
+  if (!out) return hfail(HA_E_INVALID, "null output");
+  const size_t BYTES = 1ull << 30;
+  const int NT = 1 << 20, BS = 256;
+  DBuf buf, sink;
+  HH(buf.alloc(BYTES));
+  HH(sink.alloc((size_t)NT * 8));
+  HH(hipMemset(buf.p, 0, BYTES));
+  hipEvent_t e0, e1;
+  HH(hipEventCreate(&e0));
+  HH(hipEventCreate(&e1));
+  const u64 mask = BYTES / 8 - 1;
+  for (int shape = 0; shape < 3; ++shape) {
+    const int iters = shape == 2 ? 1 : 16;
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+      HH(hipEventRecord(e0, 0));
+      probe_k<<<NT / BS, BS>>>((u64*)buf.p, mask, iters, shape, (u64*)sink.p);
+      HH(hipEventRecord(e1, 0));
+      HH(hipEventSynchronize(e1));
+      float t = 0;
+      HH(hipEventElapsedTime(&t, e0, e1));
+      best = t < best ? t : best;
+    }
+    out[shape] = (double)NT * iters / (best * 1e-3);
+  }
+  HH(hipGetLastError());
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return HA_OK;
+}
+
+}  // extern "C"

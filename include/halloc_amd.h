@@ -1,0 +1,114 @@
+/* halloc_amd.h -- C ABI of the MI355X halloc allocator (libhalloc_amd.so).
+ *
+ * Host code (C, C++, or a ctypes / cgo / JNI binding) reaches HIP only through
+ * these functions: plain pointers, sizes and return codes.  Device code calls
+ * hamalloc / hafree from halloc.h.  Every function returns 0 (or a count) on
+ * success and a negative HA_E* code on failure; ha_last_error() describes it.
+ * The workload drivers (ha_run_*) are the counterparts of the reference test
+ * harness kernels (tst/common/common.h:361-412 and the tst/perf tests); they take host
+ * buffers, keep their inputs in device memory for the timed launches, and
+ * time those launches with HIP events on the stream they launch on.
+ */
+#ifndef HALLOC_AMD_H_
+#define HALLOC_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  HA_OK = 0,
+  HA_E_INVALID = -1,   /* bad argument or options */
+  HA_E_STATE = -2,     /* not initialized / already initialized */
+  HA_E_HIP = -3,       /* a HIP runtime call failed */
+  HA_E_NOMEM = -4      /* device allocation of the pool failed */
+};
+
+/* Options: the fields of halloc_opts_t (src/halloc.h:15-34) plus tuning. */
+typedef struct ha_opts_c {
+  uint64_t memory;           /* bytes; capped at 3/4 of device memory */
+  double halloc_fraction;    /* default 0.75 */
+  double busy_fraction;      /* default 0.835 */
+  double roomy_fraction;     /* default 0.6 */
+  double sparse_fraction;    /* default 0.012 */
+  int32_t sb_sz_sh;          /* default 22 (4 MiB slabs), 21..26 */
+  int32_t nheads;            /* head slabs per size class; lanes pick head xcc_id % nheads; default 8 */
+} ha_opts_c;
+
+/* Fills *o with the defaults of src/halloc.h:30-32 and nheads = 8. */
+void ha_default_opts(ha_opts_c *o);
+
+/* Replaces ha_init (src/halloc.cu:267-424).  Initializes the current device. */
+int ha_init_c(const ha_opts_c *o);
+/* Replaces ha_shutdown (src/halloc.cu:426-428); frees the pool. */
+int ha_shutdown_c(void);
+/* Replaces ha_extfrag (src/statistics.cuh:50-69).  Writes the value to *out. */
+int ha_extfrag_c(int ideal, double *out);
+/* Free bytes of the slab heap, the counterpart of find_free_mem_k
+   (src/statistics.cuh:34-48), summed on the device. */
+int ha_free_bytes_c(uint64_t *out);
+/* Description of the last failure on this thread. */
+const char *ha_last_error(void);
+
+/* Geometry of the initialized heap: nsbs, sb_sz, nheads, nsw, bit_words,
+   nib_words, and the 16 x 12 size table (nchunks chunk_sz chunk_id ncib sparse
+   hash_step roomy busy block_sz nblocks tmod chunk_bytes). */
+typedef struct ha_geometry {
+  uint32_t nsbs, sb_sz, sb_sz_sh, nheads, nsw, bit_words, nib_words, pad;
+  uint64_t arena_bytes, cuda_memory;
+  uint32_t size_table[16 * 12];
+} ha_geometry;
+int ha_geometry_c(ha_geometry *g);
+
+/* ---------------------------------------------------------------- harness */
+
+/* Replay nsteps x lanes operations on ONE wavefront (lanes <= 64).
+   ops[2*i] = kind (0 none, 1 malloc, 2 free), ops[2*i+1] = bytes for malloc or
+   the index of the earlier malloc op for free.  Per step the malloc lanes call
+   hamalloc together, then the free lanes call hafree together.
+   recs[4*i..]: malloc -> size class (16 = large, 0xffffffff = NULL), slab,
+   chunk; free -> chunks freed, slab, chunk.  xcc[step] = XCD id of the wave. */
+int ha_replay(const uint32_t *ops, int nsteps, int lanes, uint32_t *recs, uint32_t *xcc);
+
+/* Canonical state dump (layout documented in oracle/model_capi.cpp). */
+int64_t ha_dump_words(void);
+int ha_dump(uint64_t *out, int64_t nwords);
+
+/* Plain 16 B throughput (tst/perf/priv-throughput/priv-throughput.cu:14-27):
+   nthreads lanes in blocks of bs; each lane does iters x { p = hamalloc(size);
+   write a 16 B pattern {tid, ~tid, tid ^ 0x9E3779B9, iter}; read it back;
+   hafree(p) }; `launches` timed launches after `warmup` untimed ones.
+   ms[i] receives launch i's duration (HIP events); total_ms is their sum and
+   wall_ms the host clock around the timed launches, the stream synchronized
+   before and after.  errors counts pattern mismatches, nulls NULL returns. */
+typedef struct ha_plain_result {
+  uint64_t errors;
+  uint64_t nulls;
+  double total_ms;
+  double wall_ms;
+} ha_plain_result;
+int ha_run_plain(int nthreads, int bs, int size, int iters, int warmup, int launches, float *ms,
+                 ha_plain_result *res);
+
+/* Allocate-only kernel (tst/common/common.h:375-388): lane i stores
+   hamalloc(sizes[i]) into ptrs[i] (host array, device addresses returned) and
+   writes the 32-bit size into the block's first word. */
+int ha_malloc_sizes(const uint32_t *sizes, int n, int bs, uint64_t *ptrs);
+/* Free kernel (tst/common/common.h:391-399): lane i calls hafree(ptrs[i]). */
+int ha_free_ptrs(const uint64_t *ptrs, int n, int bs);
+/* Reads back the first 32-bit word of each block (0 for NULL). */
+int ha_read_words(const uint64_t *ptrs, int n, uint32_t *out);
+
+/* Atomic-throughput probe: 64-bit atomic OR over a 1 GiB buffer from 2^20
+   lanes with three access shapes; rates in atomics/s:
+   out[0] lane-random (one cache line per lane), out[1] wave-run (a wave
+   covers 64 consecutive words), out[2] single hot word. */
+int ha_probe_atomics(double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HALLOC_AMD_H_ */

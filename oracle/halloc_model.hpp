@@ -1,0 +1,651 @@
+// halloc_model.hpp -- host C++ model of the halloc slab / bit-array / hash scheme.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle for the
+// MI355X device allocator in halloc_amd/csrc.  Only tests/, the smoke check and
+// the CPU-baseline leg of bench.py may load it; the product library never links it.
+//
+// It restates the reference allocator's algorithm (canonizer/halloc, src/halloc.cu,
+// src/slab.cuh, src/slab.h, src/sbset.*, src/size-info.h, src/utils.cu) on the host:
+//   * size classes, chunk arithmetic and the size table   (src/halloc.cu:53-67, 363-394;
+//     src/size-info.h:50-70; src/utils.cu:8-24)
+//   * per-class tickets and the initial hash              (src/halloc.cu:71-122)
+//   * the hashed probe over a slab's bit array           (src/slab.cuh:453-535)
+//   * slab counters with chunk id and head flag           (src/slab.h:52-132)
+//   * head replacement, cached head, slab search order    (src/slab.cuh:227-445)
+//   * busy / roomy / sparse / free slab sets              (src/sbset.h, src/sbset.cuh)
+//   * free path and count transitions                     (src/halloc.cu:211-256, src/slab.cuh:161-224)
+//
+// The model emulates one wavefront in lock step with explicit lane loops, so
+//   - one active lane reproduces the reference's per-lane semantics, and
+//   - one 64-lane wave pins the wave-aggregation rules the GPU follows:
+//     lanes are processed in size-class groups ordered by their lowest lane,
+//     and within a group every step (ticket, probe, counter update, head
+//     replacement) is group-uniform.
+// Every shared word is a std::atomic, so several host threads may drive waves
+// concurrently: that is the multithreaded CPU baseline.
+//
+// Deliberate differences from the reference, shared with the GPU:
+//   * 64-bit bit, nibble and set words (the reference uses 32-bit words).
+//   * 64-bit slab counter: count in bits 32..63, head flag bit 16,
+//     chunk id bits 8..11 (0xF = none).
+//   * one contiguous arena: slab = (offset >> sb_sz_sh); no address grid.
+//   * hash_mode 1 ("wave run"): the reference's initial hash with tmod = 64
+//     (nblocks rounded down to a multiple of 64), and lane l of a class group
+//     takes ticket cv = old + l where the group adds (highest lane + 1) to the
+//     ticket counter.  A wave's blocks then fill at most two aligned 64-block
+//     runs, contiguously by lane.  hash_mode 0 keeps the reference's tmod
+//     (4/2/1) and tickets cv = old + rank; it exists in the model only.
+//   * the probe counter itry is group-uniform, and the probe offset
+//     (itry+1)*step is added in 64-bit arithmetic before the modulo.
+//   * slab set membership is complete: every non-head, non-free slab is in a
+//     set.  Count transitions and head detaches file a slab by its count
+//     (sparse <= sparse thr < roomy <= roomy thr < busy); the reference files
+//     detached heads above the roomy threshold nowhere (src/slab.cuh:236).
+//   * the busy-set search keeps slabs it rejects (the reference drops them),
+//     and a head it returns is filled up to the limit it was accepted at,
+//     (busy + nchunks) / 2, instead of being swapped out at busy again.
+//   * a free of a pointer in a free slab, or whose nibble is 0, is a no-op.
+//   * when no set holds a slab for a new head, an EMPTY head or cached head of
+//     another (head, class) pair is taken over (under a try-lock of its head
+//     lock), so slabs parked as idle heads do not make hamalloc return NULL.
+#pragma once
+
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace hamodel {
+
+using u32 = uint32_t;
+using u64 = uint64_t;
+
+constexpr u32 NSIZES = 16;          // 8 units x {2,3}            (src/size-info.h:58)
+constexpr u32 NCHUNK_IDS = 4;       // chunk sizes 16, 24, 256, 384
+constexpr u32 SZ_NONE = 0xffffffffu;
+constexpr u32 SB_NONE = 0xffffffffu;
+constexpr u32 MIN_BLOCK_SZ = 16;    // src/size-info.h:60
+constexpr u32 MAX_BLOCK_SZ = 3072;  // src/size-info.h:62
+constexpr u32 BLOCK_STEP = 16;      // bit-array granularity, src/size-info.h:50
+constexpr u32 MAX_NTRIES = 32;      // src/slab.h:36
+constexpr u32 CHECK_NTRIES = 2;     // src/slab.h:39
+constexpr u32 STEP_FREQ = 64;       // src/slab.h:48
+constexpr u32 THREAD_FREQ = 11;     // src/halloc.cu:40
+constexpr u32 THREAD_MOD_SH = 2;    // src/halloc.cu:44
+constexpr u32 TICKET_INIT = 0x0B0B0B0Bu;  // cudaMemset(counters_g, 11), src/halloc.cu:416
+constexpr u32 RUN = 64;             // wave-run length (hash_mode 1)
+constexpr u32 WAVE = 64;
+// a head word is a slab id, plus HEAD_EXT when the slab came from the busy set:
+// such a head is filled up to (busy + nchunks) / 2 instead of busy
+constexpr u32 HEAD_EXT = 0x80000000u;
+
+// slab counter layout (64-bit)
+constexpr u32 CTR_CHUNK_SH = 8;
+constexpr u64 CTR_CHUNK_NONE = 0xF;
+constexpr u64 CTR_HEAD = 1ull << 16;
+constexpr u32 CTR_COUNT_SH = 32;
+inline u32 ctr_count(u64 c) { return (u32)(c >> CTR_COUNT_SH); }
+inline u32 ctr_chunk(u64 c) { return (u32)((c >> CTR_CHUNK_SH) & 0xF); }
+inline bool ctr_is_head(u64 c) { return (c & CTR_HEAD) != 0; }
+inline u64 ctr_chunk_delta(u32 chunk_id) { return ((CTR_CHUNK_NONE - chunk_id) & 0xF) << CTR_CHUNK_SH; }
+
+// ---------------------------------------------------------------- closed forms
+// size class from request size: blocks are 2*unit or 3*unit, unit = 8..1024
+// (src/halloc.cu:53-67)
+inline u32 size_id_from_nbytes(u32 nbytes) {
+  if (nbytes < MIN_BLOCK_SZ) nbytes = MIN_BLOCK_SZ;
+  u32 iunit = (32 - __builtin_clz(nbytes)) - 5, unit = 1u << (iunit + 3);
+  if (nbytes > 3 * unit) { iunit += 1; unit *= 2; }
+  return nbytes <= 2 * unit ? 2 * iunit : 2 * iunit + 1;
+}
+inline u32 block_sz_of(u32 sid) { u32 unit = 1u << (sid / 2 + 3); return (sid % 2 ? 3 : 2) * unit; }
+// reference thread modulus by size (src/halloc.cu:101-104)
+inline u32 tmod_ref(u32 sid) { int sh = (int)THREAD_MOD_SH - (int)sid / 2; return 1u << (sh > 0 ? sh : 0); }
+// chunk size encoding (div3 << 16 | shift) (src/size-info.h:50-70)
+inline u32 chunk_val(u32 chunk_sz) {
+  u32 div3 = chunk_sz % 3 ? 1 : 3;
+  if (chunk_sz % 3 == 0) chunk_sz /= 3;
+  u32 sh = 0;
+  for (; (1u << sh) < chunk_sz; sh++) {}
+  return div3 << 16 | sh;
+}
+inline u32 chunk_mul(u32 v, u32 cs) { return (v << (cs & 0xffffu)) * (cs >> 16); }
+inline u32 chunk_div(u32 v, u32 cs) { if ((cs >> 16) == 3u) v /= 3u; return v >> (cs & 0xffffu); }
+// largest prime p in [3, n) with n % p != 0 and nb % p != 0; 0 if none
+// (src/utils.cu:8-24 exits the process instead of returning 0)
+inline u32 max_prime_below(u32 n, u32 nb) {
+  for (u32 p = n - 1; p >= 3 && p < n; p--) {
+    u32 max_d = (u32)std::floor(std::sqrt((double)p));
+    bool is_prime = true;
+    for (u32 d = 2; d <= max_d; d++) if (p % d == 0) { is_prime = false; break; }
+    if (is_prime && n % p && nb % p) return p;
+  }
+  return 0;
+}
+
+struct Opts {
+  u64 memory = 512ull << 20;     // src/halloc.h:30-32 defaults
+  double halloc_fraction = 0.75;
+  double busy_fraction = 0.835;
+  double roomy_fraction = 0.6;
+  double sparse_fraction = 0.012;
+  int sb_sz_sh = 22;
+  int nheads = 1;
+  int hash_mode = 1;
+  u64 device_memory = 0;         // 0: no 3/4-of-device cap
+};
+
+struct SizeInfo {
+  u32 nchunks, chunk_sz, chunk_id, ncib, sparse_thr, hash_step, roomy_thr, busy_thr;
+  u32 block_sz, nblocks, tmod, chunk_bytes;
+};
+
+// the size table (src/halloc.cu:363-394); hash_mode 1 rounds nblocks to a multiple of RUN
+inline bool build_size_infos(u32 sb_sz, const Opts& o, SizeInfo out[NSIZES]) {
+  for (u32 s = 0; s < NSIZES; ++s) {
+    SizeInfo& si = out[s];
+    std::memset(&si, 0, sizeof si);
+    si.block_sz = block_sz_of(s);
+    u32 nblocks = sb_sz / si.block_sz;
+    si.tmod = o.hash_mode == 0 ? tmod_ref(s) : RUN;
+    nblocks = nblocks / si.tmod * si.tmod;
+    si.nblocks = nblocks;
+    si.chunk_id = s % 2 + (s < NSIZES / 2 ? 0 : 2);
+    si.chunk_bytes = (si.chunk_id % 2 ? 3 : 2) * (si.chunk_id / 2 ? 128 : 8);
+    si.chunk_sz = chunk_val(si.chunk_bytes);
+    si.ncib = si.block_sz / si.chunk_bytes;
+    si.nchunks = nblocks * si.ncib;
+    u32 p = max_prime_below(nblocks / 256 + nblocks / 64, nblocks);
+    if (p == 0 || nblocks == 0) return false;
+    si.hash_step = si.ncib * p;
+    si.roomy_thr = (u32)(o.roomy_fraction * si.nchunks);
+    si.busy_thr = (u32)(o.busy_fraction * si.nchunks);
+    si.sparse_thr = (u32)(o.sparse_fraction * si.nchunks);
+  }
+  return true;
+}
+
+// initial chunk of a lane (src/halloc.cu:107-122).  hash_mode 0: cv is the
+// lane's own ticket.  hash_mode 1: cv = G*RUN + rank (the group's run ticket G).
+inline u32 ichunk_init(u32 cv, const SizeInfo& si) {
+  u32 t = si.tmod, tm = t - 1;
+  u32 ich = (cv & ~tm) * THREAD_FREQ + (cv & tm);
+  return ich * si.ncib % si.nchunks;
+}
+// next probe position (src/slab.cuh:492-495)
+inline u32 probe_next(u32 ich0, u32 itry, const SizeInfo& si) {
+  u32 step = ((itry + 1) * STEP_FREQ * si.ncib) % si.hash_step;
+  return (u32)(((u64)ich0 + (u64)(itry + 1) * step) % si.nchunks);
+}
+
+// slab classes by count (complete membership)
+enum SbClass : u32 { CLS_FREE = 0, CLS_SPARSE = 1, CLS_ROOMY = 2, CLS_BUSY = 3 };
+inline u32 head_limit(u32 head_word, const SizeInfo& si) {
+  return (head_word & HEAD_EXT) ? (si.busy_thr + si.nchunks) / 2 : si.busy_thr;
+}
+inline u32 sb_class(u32 count, const SizeInfo& si) {
+  if (count == 0) return CLS_FREE;
+  if (count <= si.sparse_thr) return CLS_SPARSE;
+  if (count <= si.roomy_thr) return CLS_ROOMY;
+  return CLS_BUSY;
+}
+
+// one record per replayed operation
+struct Rec { u32 a, sb, chunk, extra; };
+constexpr u32 REC_LARGE = 16;
+constexpr u32 REC_NULL = 0xffffffffu;
+
+// -------------------------------------------------------------------- the heap
+class Heap {
+ public:
+  Opts opts;
+  u32 sb_sz = 0, sb_sz_sh = 0, nsbs = 0, nheads = 1, nsw = 0;  // nsw: words per slab set
+  u64 arena_bytes = 0;
+  u32 bit_words = 0, nib_words = 0;  // per slab
+  SizeInfo si[NSIZES];
+  bool ok = false;
+
+  std::vector<std::atomic<u64>> bits, nibs, ctr, sets;
+  std::vector<std::atomic<u32>> d_size, d_chunk, d_csz, d_head, sb_lock;
+  std::vector<std::atomic<u32>> head, cached, head_lock, ticket;
+  std::vector<uint8_t> arena;  // payload memory (allocated lazily by users that touch data)
+  std::atomic<u64> large_ctr{0};
+
+  // set index helpers: 0 = free, 1+s = roomy[s], 17+s = busy[s], 33+c = sparse[c]
+  static constexpr u32 NSETS = 1 + 2 * NSIZES + NCHUNK_IDS;
+  u32 set_roomy(u32 s) const { return 1 + s; }
+  u32 set_busy(u32 s) const { return 1 + NSIZES + s; }
+  u32 set_sparse(u32 c) const { return 1 + 2 * NSIZES + c; }
+
+  explicit Heap(const Opts& o, bool with_arena = false) : opts(o) {
+    u64 memory = o.memory;
+    if (o.device_memory) memory = std::min<u64>(memory, 3ull * o.device_memory / 4ull);
+    sb_sz_sh = (u32)o.sb_sz_sh;
+    sb_sz = 1u << sb_sz_sh;
+    u64 halloc_memory = (u64)(o.halloc_fraction * (double)memory);
+    nsbs = (u32)(halloc_memory / sb_sz);
+    nheads = (u32)(o.nheads < 1 ? 1 : o.nheads);
+    if (!build_size_infos(sb_sz, o, si) || nsbs == 0) return;
+    nsw = (nsbs + 63) / 64;
+    bit_words = sb_sz / BLOCK_STEP / 64;
+    nib_words = sb_sz / BLOCK_STEP / 16;
+    arena_bytes = (u64)nsbs * sb_sz;
+    bits = std::vector<std::atomic<u64>>((size_t)nsbs * bit_words);
+    nibs = std::vector<std::atomic<u64>>((size_t)nsbs * nib_words);
+    ctr = std::vector<std::atomic<u64>>(nsbs);
+    sets = std::vector<std::atomic<u64>>((size_t)NSETS * nsw);
+    d_size = std::vector<std::atomic<u32>>(nsbs);
+    d_chunk = std::vector<std::atomic<u32>>(nsbs);
+    d_csz = std::vector<std::atomic<u32>>(nsbs);
+    d_head = std::vector<std::atomic<u32>>(nsbs);
+    sb_lock = std::vector<std::atomic<u32>>(nsbs);
+    head = std::vector<std::atomic<u32>>((size_t)nheads * NSIZES);
+    cached = std::vector<std::atomic<u32>>((size_t)nheads * NSIZES);
+    head_lock = std::vector<std::atomic<u32>>((size_t)nheads * NSIZES);
+    ticket = std::vector<std::atomic<u32>>((size_t)nheads * NSIZES);
+    for (auto& x : bits) x.store(0, std::memory_order_relaxed);
+    for (auto& x : nibs) x.store(0, std::memory_order_relaxed);
+    for (auto& x : sets) x.store(0, std::memory_order_relaxed);
+    for (u32 b = 0; b < nsbs; ++b) {
+      ctr[b].store(CTR_CHUNK_NONE << CTR_CHUNK_SH, std::memory_order_relaxed);
+      d_size[b].store(SZ_NONE); d_chunk[b].store(SZ_NONE); d_csz[b].store(0); d_head[b].store(0);
+      sb_lock[b].store(0);
+      sets[(size_t)0 * nsw + b / 64].fetch_or(1ull << (b % 64));
+    }
+    for (size_t i = 0; i < head.size(); ++i) {
+      head[i].store(SB_NONE); cached[i].store(SB_NONE); head_lock[i].store(0); ticket[i].store(TICKET_INIT);
+    }
+    if (with_arena) arena.assign((size_t)arena_bytes, 0);
+    ok = true;
+  }
+
+  // ------------------------------------------------------------- sets (sbset)
+  std::atomic<u64>* set_words(u32 set) { return &sets[(size_t)set * nsw]; }
+  void set_add(u32 set, u32 sb) { set_words(set)[sb / 64].fetch_or(1ull << (sb % 64)); }
+  void set_remove(u32 set, u32 sb) { set_words(set)[sb / 64].fetch_and(~(1ull << (sb % 64))); }
+  // first-fit claim (src/sbset.cuh:5-35), scanning every word that holds a slab
+  u32 set_pop(u32 set) {
+    std::atomic<u64>* w = set_words(set);
+    for (u32 i = 0; i < nsw; ++i) {
+      u64 word = w[i].load();
+      while (word) {
+        u32 b = (u32)__builtin_ctzll(word);
+        u64 m = 1ull << b;
+        if (w[i].fetch_and(~m) & m) return i * 64 + b;
+        word &= ~m;
+      }
+    }
+    return SB_NONE;
+  }
+  u32 class_set(u32 cls, u32 size_id, u32 chunk_id) const {
+    return cls == CLS_SPARSE ? set_sparse(chunk_id) : cls == CLS_ROOMY ? set_roomy(size_id) : set_busy(size_id);
+  }
+
+  // --------------------------------------------------------------- locks
+  static void lock(std::atomic<u32>& m) {
+    while (m.exchange(1, std::memory_order_acquire) != 0) std::this_thread::yield();
+  }
+  static bool try_lock(std::atomic<u32>& m) { return m.exchange(1, std::memory_order_acquire) == 0; }
+  static void unlock(std::atomic<u32>& m) { m.store(0, std::memory_order_release); }
+
+  // ----------------------------------------------- slab state transitions
+  // sb_try_mark_free (src/slab.cuh:89-113); caller holds sb_lock[sb]
+  void try_mark_free(u32 sb, u32 size_id, u32 chunk_id, bool from_head) {
+    if (d_head[sb].load() || chunk_id == SZ_NONE) return;
+    u64 delta = ctr_chunk_delta(chunk_id), old2;
+    do {
+      u64 old = ctr[sb].fetch_add(delta);
+      if (ctr_count(old) == 0) {
+        d_size[sb].store(SZ_NONE); d_chunk[sb].store(SZ_NONE); d_csz[sb].store(0);
+        set_add(0, sb);
+        set_remove(set_sparse(chunk_id), sb);
+        if (size_id != SZ_NONE) { set_remove(set_roomy(size_id), sb); set_remove(set_busy(size_id), sb); }
+        return;
+      }
+      old2 = ctr[sb].fetch_sub(delta);
+    } while (ctr_count(old2) == 0);
+    if (from_head && size_id != SZ_NONE) {
+      u32 c = sb_class(ctr_count(old2), si[size_id]);
+      if (c != CLS_FREE) set_add(class_set(c, size_id, chunk_id), sb);
+    }
+  }
+
+  // sb_ctr_dec leader part (src/slab.cuh:161-224) for one slab group
+  void ctr_dec(u32 sb, u32 change) {
+    u64 old = ctr[sb].fetch_sub((u64)change << CTR_COUNT_SH);
+    if (ctr_is_head(old)) return;
+    u32 size_id = d_size[sb].load(), chunk_id = d_chunk[sb].load();
+    if (size_id == SZ_NONE || chunk_id == SZ_NONE) return;
+    u32 oc = ctr_count(old), nc = oc - change;
+    u32 co = sb_class(oc, si[size_id]), cn = sb_class(nc, si[size_id]);
+    if (co == cn) return;
+    if (cn == CLS_FREE) {
+      if (ctr_chunk(old) != CTR_CHUNK_NONE) {
+        lock(sb_lock[sb]);
+        size_id = d_size[sb].load(); chunk_id = d_chunk[sb].load();
+        try_mark_free(sb, size_id, chunk_id, false);
+        unlock(sb_lock[sb]);
+      }
+    } else {
+      set_add(class_set(cn, size_id, chunk_id), sb);
+      if (co != CLS_FREE) set_remove(class_set(co, size_id, chunk_id), sb);
+    }
+  }
+
+  // detach_head (src/slab.cuh:227-255); caller holds sb_lock[sb]
+  void detach_head(u32 sb) {
+    d_head[sb].store(0);
+    u64 old = ctr[sb].fetch_sub(CTR_HEAD);
+    u32 count = ctr_count(old);
+    u32 size_id = d_size[sb].load(), chunk_id = d_chunk[sb].load();
+    if (size_id == SZ_NONE) return;
+    u32 c = sb_class(count, si[size_id]);
+    if (c == CLS_FREE) try_mark_free(sb, size_id, chunk_id, true);
+    else set_add(class_set(c, size_id, chunk_id), sb);
+  }
+
+  // take an empty head or cached head of another (head, class) pair and put it
+  // in the free set; returns false if there is none.  Caller holds the head
+  // lock of (ihead, size_id).
+  bool steal_idle_head(u32 ihead, u32 size_id) {
+    for (u32 ih = 0; ih < nheads; ++ih) {
+      for (u32 c = 0; c < NSIZES; ++c) {
+        if (ih == ihead && c == size_id) continue;
+        size_t hi = (size_t)ih * NSIZES + c;
+        for (int which = 0; which < 2; ++which) {
+          std::atomic<u32>& word = which == 0 ? cached[hi] : head[hi];
+          u32 w = word.load();
+          if (w == SB_NONE) continue;
+          u32 sb = w & ~HEAD_EXT;
+          if (ctr_count(ctr[sb].load()) != 0) continue;
+          if (!try_lock(head_lock[hi])) continue;
+          bool ok = word.load() == w && ctr_count(ctr[sb].load()) == 0;
+          if (ok) {
+            word.store(SB_NONE);
+            lock(sb_lock[sb]);
+            detach_head(sb);
+            unlock(sb_lock[sb]);
+          }
+          unlock(head_lock[hi]);
+          if (ok) return true;
+        }
+      }
+    }
+    return false;
+  }
+
+  // find_sb_for_size (src/slab.cuh:259-372): roomy -> sparse -> free -> busy,
+  // then (for a head needed now) an idle head of another class
+  u32 find_sb_for_size(u32 size_id, u32 chunk_id, u32 ihead = 0, bool steal = false) {
+    const SizeInfo& s = si[size_id];
+    u32 sb;
+  again:
+    while ((sb = set_pop(set_roomy(size_id))) != SB_NONE) {
+      lock(sb_lock[sb]);
+      bool found = false;
+      if (!d_head[sb].load() && d_size[sb].load() == size_id) {
+        if (ctr_count(ctr[sb].load()) <= s.roomy_thr) {
+          found = true; d_head[sb].store(1); ctr[sb].fetch_add(CTR_HEAD);
+        } else {
+          set_add(set_busy(size_id), sb);
+        }
+      }
+      unlock(sb_lock[sb]);
+      if (found) return sb;
+    }
+    while ((sb = set_pop(set_sparse(chunk_id))) != SB_NONE) {
+      lock(sb_lock[sb]);
+      bool found = false;
+      if (!d_head[sb].load() && d_chunk[sb].load() == chunk_id) {
+        u32 count = ctr_count(ctr[sb].load());
+        if (count <= s.sparse_thr) {
+          found = true; d_head[sb].store(1); d_size[sb].store(size_id); ctr[sb].fetch_add(CTR_HEAD);
+        } else {
+          // not sparse for this class: hand it to its own class's roomy set, as the
+          // reference does (src/slab.cuh:307-308); re-adding it here would loop
+          set_add(set_roomy(d_size[sb].load()), sb);
+        }
+      }
+      unlock(sb_lock[sb]);
+      if (found) return sb;
+    }
+    while ((sb = set_pop(0)) != SB_NONE) {
+      lock(sb_lock[sb]);
+      bool found = false;
+      if (!d_head[sb].load()) {
+        found = true;
+        d_head[sb].store(1); d_size[sb].store(size_id); d_chunk[sb].store(chunk_id); d_csz[sb].store(s.chunk_sz);
+        ctr[sb].fetch_add(CTR_HEAD);
+        ctr[sb].fetch_sub(ctr_chunk_delta(chunk_id));
+      }
+      unlock(sb_lock[sb]);
+      if (found) return sb;
+    }
+    // busy: peek, claim only a slab with room, keep the others in the set
+    std::atomic<u64>* w = set_words(set_busy(size_id));
+    u32 limit = (s.busy_thr + s.nchunks) / 2;
+    for (u32 i = 0; i < nsw; ++i) {
+      u64 word = w[i].load();
+      while (word) {
+        u32 b = (u32)__builtin_ctzll(word);
+        u64 m = 1ull << b;
+        word &= ~m;
+        sb = i * 64 + b;
+        if (ctr_count(ctr[sb].load()) >= limit) continue;  // a head is swapped out at limit
+        if (!(w[i].fetch_and(~m) & m)) continue;
+        lock(sb_lock[sb]);
+        bool found = false;
+        if (!d_head[sb].load() && d_size[sb].load() == size_id) {
+          found = true; d_head[sb].store(1); ctr[sb].fetch_add(CTR_HEAD);
+        }
+        unlock(sb_lock[sb]);
+        if (found) return sb | HEAD_EXT;
+      }
+    }
+    if (steal && steal_idle_head(ihead, size_id)) { steal = false; goto again; }
+    return SB_NONE;
+  }
+
+  // new_sb_for_size (src/slab.cuh:378-445)
+  u32 new_sb_for_size(u32 size_id, u32 ihead) {
+    const SizeInfo& s = si[size_id];
+    size_t hi = (size_t)ihead * NSIZES + size_id;
+    u32 cur = head[hi].load();
+    if (try_lock(head_lock[hi])) {
+      cur = head[hi].load();
+      u32 nh;
+      if (cur == SB_NONE || ctr_count(ctr[cur & ~HEAD_EXT].load()) >= head_limit(cur, s)) {
+        nh = cached[hi].load();
+        cached[hi].store(SB_NONE);
+        if (nh == SB_NONE) nh = find_sb_for_size(size_id, s.chunk_id, ihead, true);
+        if (nh != SB_NONE) {
+          head[hi].store(nh);
+          if (cur != SB_NONE) {
+            u32 csb = cur & ~HEAD_EXT;
+            lock(sb_lock[csb]); detach_head(csb); unlock(sb_lock[csb]);
+          }
+          cached[hi].store(find_sb_for_size(size_id, s.chunk_id));
+        }
+      } else {
+        nh = head[hi].load();
+      }
+      unlock(head_lock[hi]);
+      return nh;
+    }
+    while (head[hi].load() == cur && head_lock[hi].load() != 0) std::this_thread::yield();
+    return head[hi].load();
+  }
+
+  // ---------------------------------------------------------- malloc (wave)
+  static u32 rank_of(u64 g, u32 l) { return (u32)__builtin_popcountll(g & ((1ull << l) - 1)); }
+
+  // one size-class group of one wave (hamalloc_small, src/halloc.cu:125-182)
+  void malloc_group(u32 ihead, u32 size_id, u64 g, u64* out, Rec* rec) {
+    const SizeInfo& s = si[size_id];
+    size_t hi = (size_t)ihead * NSIZES + size_id;
+    u32 hword = head[hi].load();
+    u32 ich0[WAVE] = {0}, ich[WAVE] = {0};
+    // tickets (size_ctr_inc, src/halloc.cu:71-98)
+    if (opts.hash_mode == 0) {
+      u32 base = ticket[hi].fetch_add((u32)__builtin_popcountll(g));
+      for (u32 l = 0; l < WAVE; ++l) if (g >> l & 1) ich0[l] = ichunk_init(base + rank_of(g, l), s);
+    } else {
+      u32 last = 63 - (u32)__builtin_clzll(g);
+      u32 base = ticket[hi].fetch_add(last + 1);
+      for (u32 l = 0; l < WAVE; ++l) if (g >> l & 1) ich0[l] = ichunk_init(base + l, s);
+    }
+    u32 itry = 0;
+    u64 want = g;
+    const u64 mask1 = (1ull << s.ncib) - 1;
+    while (want) {
+      bool need = false;
+      u64 reserved = 0;
+      const u32 hsb = hword == SB_NONE ? SB_NONE : (hword & ~HEAD_EXT);
+      const u32 limit = head_limit(hword, s);
+      // sb_alloc_in (src/slab.cuh:453-535)
+      if (hsb == SB_NONE) {
+        need = true;
+      } else {
+        std::atomic<u64>* bw = &bits[(size_t)hsb * bit_words];
+        u64 probing = want;
+        for (u32 l = 0; l < WAVE; ++l) if (probing >> l & 1) ich[l] = ich0[l];
+        for (;;) {
+          u64 oldw[WAVE];
+          for (u32 l = 0; l < WAVE; ++l) if (probing >> l & 1) {
+            u64 m = mask1 << (ich[l] % 64);
+            oldw[l] = bw[ich[l] / 64].fetch_or(m);
+          }
+          u64 now = 0;
+          for (u32 l = 0; l < WAVE; ++l) if (probing >> l & 1) {
+            u64 m = mask1 << (ich[l] % 64);
+            if (!(oldw[l] & m)) now |= 1ull << l;
+            else if (~oldw[l] & m) bw[ich[l] / 64].fetch_and(~(m & ~oldw[l]));
+          }
+          reserved |= now;
+          probing &= ~now;
+          if (!probing) break;
+          if (itry % CHECK_NTRIES == CHECK_NTRIES - 1) {
+            if (ctr_count(ctr[hsb].load()) >= limit) break;
+          }
+          for (u32 l = 0; l < WAVE; ++l) if (probing >> l & 1) ich[l] = probe_next(ich0[l], itry, s);
+          if (++itry % MAX_NTRIES >= MAX_NTRIES - 1) break;
+        }
+        if (probing) need = true;
+        if (reserved) {
+          // sb_ctr_inc (src/slab.cuh:122-155), one group per (slab, ncib)
+          u32 n = (u32)__builtin_popcountll(reserved), change = s.ncib * n;
+          u64 old = ctr[hsb].fetch_add((u64)change << CTR_COUNT_SH);
+          if (ctr_chunk(old) != s.chunk_id) {
+            for (u32 l = 0; l < WAVE; ++l) if (reserved >> l & 1)
+              bw[ich[l] / 64].fetch_and(~(mask1 << (ich[l] % 64)));
+            ctr_dec(hsb, change);
+            reserved = 0;
+            need = true;
+          } else {
+            u32 oc = ctr_count(old);
+            if (oc < limit && oc + change >= limit) need = true;
+            std::atomic<u64>* nw = &nibs[(size_t)hsb * nib_words];
+            for (u32 l = 0; l < WAVE; ++l) if (reserved >> l & 1) {
+              u64 off = (u64)hsb * sb_sz + chunk_mul(ich[l], s.chunk_sz);
+              out[l] = off + 1;
+              nw[ich[l] / 16].fetch_or((u64)s.ncib << (4 * (ich[l] % 16)));
+              if (rec) rec[l] = Rec{size_id, hsb, ich[l], 0};
+            }
+            want &= ~reserved;
+          }
+        }
+      }
+      if (need) {
+        u32 nh = new_sb_for_size(size_id, ihead);
+        hword = nh;
+        if (nh == SB_NONE) {
+          for (u32 l = 0; l < WAVE; ++l) if (want >> l & 1) {
+            out[l] = 0;
+            if (rec) rec[l] = Rec{REC_NULL, REC_NULL, REC_NULL, 0};
+          }
+          want = 0;
+        }
+      }
+    }
+  }
+
+  // hamalloc for the active lanes of one wave; out[] = arena offset + 1 (0 = NULL)
+  void wave_malloc(u32 ihead, u64 active, const u32* nbytes, u64* out, Rec* rec) {
+    u32 sid[WAVE] = {0};
+    u64 small = 0;
+    for (u32 l = 0; l < WAVE; ++l) if (active >> l & 1) {
+      if (nbytes[l] > MAX_BLOCK_SZ) {
+        out[l] = (1ull << 63) | (large_ctr.fetch_add(1) + 1);
+        if (rec) rec[l] = Rec{REC_LARGE, 0, 0, 0};
+      } else {
+        sid[l] = size_id_from_nbytes(nbytes[l]);
+        small |= 1ull << l;
+      }
+    }
+    while (small) {
+      u32 l0 = (u32)__builtin_ctzll(small);
+      u64 g = 0;
+      for (u32 l = 0; l < WAVE; ++l) if ((small >> l & 1) && sid[l] == sid[l0]) g |= 1ull << l;
+      small &= ~g;
+      malloc_group(ihead % nheads, sid[l0], g, out, rec);
+    }
+  }
+
+  // hafree for the active lanes of one wave (src/halloc.cu:211-256)
+  void wave_free(u64 active, const u64* ptr, Rec* rec) {
+    u32 sbl[WAVE] = {0};
+    u64 small = 0;
+    for (u32 l = 0; l < WAVE; ++l) if (active >> l & 1) {
+      if (rec) rec[l] = Rec{0, REC_NULL, REC_NULL, 0};
+      u64 p = ptr[l];
+      if (p == 0 || (p >> 63)) continue;   // NULL, or a large block (device heap)
+      u64 off = p - 1;
+      if (off >= arena_bytes) continue;
+      sbl[l] = (u32)(off >> sb_sz_sh);
+      small |= 1ull << l;
+    }
+    while (small) {
+      u32 l0 = (u32)__builtin_ctzll(small);
+      u32 sb = sbl[l0];
+      u64 g = 0;
+      for (u32 l = 0; l < WAVE; ++l) if ((small >> l & 1) && sbl[l] == sb) g |= 1ull << l;
+      small &= ~g;
+      u32 cs = d_csz[sb].load();
+      if (cs == 0) continue;  // slab is free: nothing is allocated in it
+      std::atomic<u64>* nw = &nibs[(size_t)sb * nib_words];
+      std::atomic<u64>* bw = &bits[(size_t)sb * bit_words];
+      u32 change = 0;
+      u32 ich[WAVE] = {0}, nch[WAVE] = {0};
+      for (u32 l = 0; l < WAVE; ++l) if (g >> l & 1) {
+        u32 in_sb = (u32)((ptr[l] - 1) & (sb_sz - 1));
+        ich[l] = chunk_div(in_sb, cs);
+        u32 sh = 4 * (ich[l] % 16);
+        nch[l] = (u32)((nw[ich[l] / 16].fetch_and(~(0xFull << sh)) >> sh) & 0xF);
+      }
+      for (u32 l = 0; l < WAVE; ++l) if ((g >> l & 1) && nch[l]) {
+        bw[ich[l] / 64].fetch_and(~(((1ull << nch[l]) - 1) << (ich[l] % 64)));
+        change += nch[l];
+        if (rec) rec[l] = Rec{nch[l], sb, ich[l], 0};
+      }
+      if (change) ctr_dec(sb, change);
+    }
+  }
+
+  // -------------------------------------------------------------- statistics
+  // free bytes over slabs (src/statistics.cuh:34-48)
+  u64 free_bytes() const {
+    u64 total = 0;
+    for (u32 b = 0; b < nsbs; ++b) {
+      u32 cs = d_csz[b].load();
+      u32 used = cs ? chunk_mul(ctr_count(ctr[b].load()), cs) : 0;
+      total += sb_sz - used;
+    }
+    return total;
+  }
+};
+
+}  // namespace hamodel

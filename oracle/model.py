@@ -1,0 +1,156 @@
+"""ctypes wrapper of the host model of the halloc scheme (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of bench.py
+import this module.  It is the checker, never the thing measured or shipped:
+the product path (``halloc_amd``) never imports it.
+
+The model restates canonizer/halloc's small-allocation path (src/halloc.cu,
+src/slab.cuh, src/slab.h, src/sbset.*, src/size-info.h, src/utils.cu); see
+oracle/halloc_model.hpp for the function-by-function citations.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "libhalloc_model.so")
+
+NSIZES = 16
+SZ_NONE = 0xFFFFFFFF
+REC_LARGE = 16
+REC_NULL = 0xFFFFFFFF
+NSETS = 1 + 2 * NSIZES + 4
+SIZE_TABLE_FIELDS = ("nchunks", "chunk_sz", "chunk_id", "ncib", "sparse", "hash_step",
+                     "roomy", "busy", "block_sz", "nblocks", "tmod", "chunk_bytes")
+
+
+class HmOpts(ctypes.Structure):
+    _fields_ = [("memory", ctypes.c_uint64),
+                ("halloc_fraction", ctypes.c_double),
+                ("busy_fraction", ctypes.c_double),
+                ("roomy_fraction", ctypes.c_double),
+                ("sparse_fraction", ctypes.c_double),
+                ("sb_sz_sh", ctypes.c_int32),
+                ("nheads", ctypes.c_int32),
+                ("hash_mode", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+
+def make_opts(memory=512 << 20, halloc_fraction=0.75, busy_fraction=0.835, roomy_fraction=0.6,
+              sparse_fraction=0.012, sb_sz_sh=22, nheads=1, hash_mode=1) -> HmOpts:
+    """Defaults follow src/halloc.h:30-32."""
+    return HmOpts(memory, halloc_fraction, busy_fraction, roomy_fraction, sparse_fraction,
+                  sb_sz_sh, nheads, hash_mode, 0)
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        u32, u64, i32, i64, vp = (ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32,
+                                  ctypes.c_int64, ctypes.c_void_p)
+        P = ctypes.POINTER
+        for name, res, args in [
+            ("hm_size_id", u32, [u32]),
+            ("hm_chunk_val", u32, [u32]),
+            ("hm_chunk_mul", u32, [u32, u32]),
+            ("hm_chunk_div", u32, [u32, u32]),
+            ("hm_max_prime_below", u32, [u32, u32]),
+            ("hm_ichunk_init", u32, [u32, u32, u32, i32]),
+            ("hm_probe_next", u32, [u32, u32, u32, u32, i32]),
+            ("hm_size_table", ctypes.c_int, [P(HmOpts), vp]),
+            ("hm_create", vp, [P(HmOpts)]),
+            ("hm_destroy", None, [vp]),
+            ("hm_nsbs", u32, [vp]),
+            ("hm_replay", ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp]),
+            ("hm_dump_words", u64, [vp]),
+            ("hm_dump", ctypes.c_int, [vp, vp]),
+            ("hm_check_dump", i64, [P(HmOpts), vp, vp]),
+            ("hm_bench_plain", ctypes.c_double, [P(HmOpts), ctypes.c_int, ctypes.c_int, ctypes.c_int, P(u64)]),
+            ("hm_bench_glibc", ctypes.c_double, [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(u64)]),
+        ]:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def size_table(opts: HmOpts) -> np.ndarray:
+    out = np.zeros((NSIZES, len(SIZE_TABLE_FIELDS)), dtype=np.uint32)
+    if lib().hm_size_table(ctypes.byref(opts), out.ctypes.data) != 0:
+        raise ValueError("size table cannot be built for these options")
+    return out
+
+
+class ModelHeap:
+    """One host model heap; replay() drives it one wave call at a time."""
+
+    def __init__(self, opts: HmOpts):
+        self.opts = opts
+        self.h = lib().hm_create(ctypes.byref(opts))
+        if not self.h:
+            raise ValueError("invalid model options")
+        self.nsbs = lib().hm_nsbs(self.h)
+
+    def close(self):
+        if self.h:
+            lib().hm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def replay(self, ops: np.ndarray, iheads: np.ndarray | None = None):
+        """ops: (nsteps, lanes, 2) uint32 [kind, arg]; returns (recs (nsteps, lanes, 4), ptrs)."""
+        ops = np.ascontiguousarray(ops, dtype=np.uint32)
+        nsteps, lanes = ops.shape[0], ops.shape[1]
+        recs = np.zeros((nsteps, lanes, 4), dtype=np.uint32)
+        ptrs = np.zeros((nsteps, lanes), dtype=np.uint64)
+        ih = None
+        if iheads is not None:
+            ih = np.ascontiguousarray(iheads, dtype=np.uint32)
+        rc = lib().hm_replay(self.h, ops.ctypes.data, ih.ctypes.data if ih is not None else None,
+                             nsteps, lanes, recs.ctypes.data, ptrs.ctypes.data)
+        if rc != 0:
+            raise ValueError(f"hm_replay failed: {rc}")
+        return recs, ptrs
+
+    def dump(self) -> np.ndarray:
+        n = lib().hm_dump_words(self.h)
+        out = np.zeros(n, dtype=np.uint64)
+        lib().hm_dump(self.h, out.ctypes.data)
+        return out
+
+
+def check_dump(opts: HmOpts, dump: np.ndarray):
+    """Invariants over a canonical dump; returns (total, per-invariant counts)."""
+    d = np.ascontiguousarray(dump, dtype=np.uint64)
+    detail = np.zeros(5, dtype=np.int64)
+    tot = lib().hm_check_dump(ctypes.byref(opts), d.ctypes.data, detail.ctypes.data)
+    return int(tot), detail
+
+
+def bench_plain(opts: HmOpts, nthreads: int, nlanes: int, iters: int):
+    v = ctypes.c_uint64(0)
+    r = lib().hm_bench_plain(ctypes.byref(opts), nthreads, nlanes, iters, ctypes.byref(v))
+    return r, int(v.value)
+
+
+def bench_glibc(nthreads: int, nlanes: int, iters: int):
+    v = ctypes.c_uint64(0)
+    r = lib().hm_bench_glibc(nthreads, nlanes, iters, ctypes.byref(v))
+    return r, int(v.value)

@@ -1,0 +1,72 @@
+"""Shared helpers of the test suite: random replay programs and block checks."""
+import numpy as np
+
+BLOCK_SIZES = [16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072]
+REC_NULL = 0xFFFFFFFF
+
+def random_ops(rng, nsteps, lanes, sizes, p_free=0.4, large_every=0):
+    """Random replay program: each step a lane mallocs, frees a live block, or idles."""
+    ops = np.zeros((nsteps, lanes, 2), dtype=np.uint32)
+    live = []
+    for st in range(nsteps):
+        freed_now = set()
+        for l in range(lanes):
+            u = rng.random()
+            if live and u < p_free:
+                j = int(rng.integers(len(live)))
+                idx = live.pop(j)
+                ops[st, l] = (2, idx)
+            elif u < 0.9:
+                n = int(rng.choice(sizes))
+                if large_every and rng.integers(large_every) == 0:
+                    n = int(rng.integers(3073, 65536))
+                ops[st, l] = (1, n)
+                freed_now.add(st * lanes + l)
+        live.extend(sorted(freed_now))
+    return ops
+
+
+def live_intervals(ops, recs, table):
+    """(start, end) byte offsets within the arena of live small blocks after a replay."""
+    nsteps, lanes = ops.shape[:2]
+    alive = {}
+    for st in range(nsteps):
+        for l in range(lanes):
+            i = st * lanes + l
+            k, a = ops[st, l]
+            if k == 1:
+                r = recs[st, l]
+                if r[0] < 16:
+                    alive[i] = (int(r[1]), int(r[2]), int(table[r[0]][11]), int(table[r[0]][8]))
+            elif k == 2:
+                alive.pop(int(a), None)
+    return list(alive.values())
+
+
+def assert_disjoint(blocks, sb_sz):
+    iv = sorted((sb * sb_sz + ch * cb, sb * sb_sz + ch * cb + bs, cb) for sb, ch, cb, bs in blocks)
+    for (s0, e0, _), (s1, _, _) in zip(iv, iv[1:]):
+        assert e0 <= s1, "overlapping live blocks"
+    for s, e, cb in iv:
+        assert s % (8 if cb % 3 == 0 else 16) == 0  # 24 B classes are 8-aligned (src/halloc.cu:377-380)
+
+
+
+
+def pressure_program(seed=3, nsteps=6500):
+    """Three phases on one wave: fill a small pool with 64/128 B blocks until
+    hamalloc returns NULL, free almost everything, then allocate 16/32 B
+    blocks, which must reuse slabs first owned by the other classes."""
+    rng = np.random.default_rng(seed)
+    steps, live = [], []
+    for st in range(nsteps):
+        row = np.zeros((64, 2), dtype=np.uint32)
+        pf, sizes = (0.05, (64, 128)) if st < 2500 else (0.98, (64, 128)) if st < 5500 else (0.3, (16, 32))
+        for l in range(64):
+            if live and rng.random() < pf:
+                row[l] = (2, live.pop(int(rng.integers(len(live)))))
+            else:
+                row[l] = (1, sizes[int(rng.integers(2))])
+        steps.append(row)
+        live.extend(st * 64 + l for l in range(64) if row[l, 0] == 1)
+    return np.stack(steps)

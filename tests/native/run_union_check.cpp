@@ -1,0 +1,123 @@
+// Brute-force check of the wave-run union arithmetic (halloc_amd/csrc/ha_bits.h)
+// used by the device malloc probe and the free fast path.  Built with g++ by
+// tests/test_native.py; prints "ok" and exits 0 when every case agrees.
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <random>
+
+#include "../../halloc_amd/csrc/ha_bits.h"
+
+using ha::u32;
+using ha::u64;
+
+static int fails = 0;
+#define CHECK(c)                                                                \
+  do {                                                                          \
+    if (!(c)) {                                                                 \
+      if (fails++ < 20) std::printf("FAIL line %d: %s\n", __LINE__, #c);        \
+    }                                                                           \
+  } while (0)
+
+int main() {
+  std::mt19937_64 rng(12345);
+  const u32 ks[4] = {1, 2, 4, 8};
+  for (int iter = 0; iter < 200000; ++iter) {
+    u32 k = ks[rng() % 4];
+    u32 nruns = 2 + rng() % 40;
+    u32 nchunks = nruns * 64 * k;
+    u32 old = (u32)rng();
+    u32 slot0 = old & 63, split = 64 - slot0;
+    u32 runA = (u32)(rng() % nruns) * 64 * k, runB = (u32)(rng() % nruns) * 64 * k;
+    u64 off = (rng() % 4 == 0) ? 0 : (rng() % (8ull * nchunks)) / k * k;
+    u64 P = rng() & rng();
+    if (rng() % 3 == 0) P = ~0ull;
+    u64 amask = split >= 64 ? ~0ull : ((1ull << split) - 1);
+    u32 pA = (u32)((runA + off) % nchunks), pB = (u32)((runB + off) % nchunks);
+    // per-lane expected positions (the model's per-lane formula)
+    u32 pos[64];
+    for (u32 l = 0; l < 64; ++l) {
+      u32 slot = l < split ? slot0 + l : l - split;
+      u32 rb = l < split ? runA : runB;
+      pos[l] = (u32)(((u64)rb + slot * k + off) % nchunks);
+    }
+    u64 ufield = (1ull << k) - 1;
+    // ---- bit array: per word union restricted to the lane's segment
+    for (u32 me = 0; me < 64; ++me) {
+      if (!((P >> me) & 1)) continue;
+      bool inA = me < split;
+      u64 myseg = inA ? amask : ~amask;
+      u32 st = inA ? pA + slot0 * k : pB - split * k;
+      u32 word, issuer;
+      u64 uni;
+      ha::run_word_bits(P & myseg, me, st, k, word, uni, issuer);
+      u32 c = st + me * k;
+      CHECK((c >= nchunks ? c - nchunks : c) == pos[me]);
+      u32 wreal = word >= nchunks / 64 ? word - nchunks / 64 : word;
+      CHECK(wreal == pos[me] / 64);
+      u64 want = 0;
+      u32 wiss = 64;
+      for (u32 l = 0; l < 64; ++l) {
+        if (!((P & myseg) >> l & 1)) continue;
+        if (pos[l] / 64 != pos[me] / 64) continue;
+        want |= ufield << (pos[l] % 64);
+        if (l < wiss) wiss = l;
+      }
+      CHECK(uni == want);
+      CHECK(issuer == wiss);
+      // ---- nibble array with value k
+      u32 nword, nissuer;
+      u64 nuni;
+      ha::run_word_nibs(P & myseg, me, st, k, (u64)k, nword, nuni, nissuer);
+      u32 nreal = nword >= nchunks / 16 ? nword - nchunks / 16 : nword;
+      CHECK(nreal == pos[me] / 16);
+      u64 nwant = 0;
+      u32 niss = 64;
+      for (u32 l = 0; l < 64; ++l) {
+        if (!((P & myseg) >> l & 1)) continue;
+        if (pos[l] / 16 != pos[me] / 16) continue;
+        nwant |= (u64)k << (4 * (pos[l] % 16));
+        if (l < niss) niss = l;
+      }
+      CHECK(nuni == nwant);
+      CHECK(nissuer == niss);
+    }
+    // ---- free fast path: real positions, virtual start ich[l0] - l0*k
+    u64 A = P & amask;
+    if (A) {
+      u32 l0 = (u32)__builtin_ctzll(A);
+      u32 sa = pos[l0] - l0 * k;
+      for (u32 me = 0; me < 64; ++me) {
+        if (!((A >> me) & 1)) continue;
+        if (pos[me] != sa + me * k) continue;  // the run wrapped around nchunks
+        u32 word, issuer;
+        u64 uni;
+        ha::run_word_nibs(A, me, sa, k, 0xFull, word, uni, issuer);
+        CHECK(word == pos[me] / 16);
+        u64 nwant = 0;
+        for (u32 l = 0; l < 64; ++l)
+          if ((A >> l & 1) && pos[l] / 16 == pos[me] / 16) nwant |= 0xFull << (4 * (pos[l] % 16));
+        bool contiguous = true;
+        for (u32 l = 0; l < 64; ++l)
+          if ((A >> l & 1) && pos[l] != sa + l * k) contiguous = false;
+        if (contiguous) CHECK(uni == nwant);
+      }
+    }
+  }
+  // spread_bits: bit i -> bit f*i
+  for (u32 f : {1u, 2u, 4u, 8u, 16u, 32u}) {
+    for (int t = 0; t < 10000; ++t) {
+      u64 x = rng() & (f == 1 ? ~0ull : ((1ull << (64 / f)) - 1));
+      u64 want = 0;
+      for (u32 i = 0; i < 64 / f; ++i)
+        if (x >> i & 1) want |= 1ull << (f * i);
+      CHECK(ha::spread_bits(x, f) == want);
+    }
+  }
+  if (fails) {
+    std::printf("%d failures\n", fails);
+    return 1;
+  }
+  std::printf("ok\n");
+  return 0;
+}

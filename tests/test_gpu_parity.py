@@ -1,0 +1,199 @@
+"""GPU parity of the MI355X allocator against the host model (oracle/).
+
+Deterministic mode: one wavefront replays a program of mallocs and frees; the
+records (size class, slab, chunk) and the whole allocator state must equal the
+model's bit for bit.  Concurrent mode: full-grid workloads must satisfy the
+reference test suite's invariants (tst/common/common.cu:412-529,
+tst/corr/freeslabs/freeslabs.cu:35-71), checked on the dumped state.
+"""
+import numpy as np
+import pytest
+
+import halloc_amd as H
+from oracle import model as M
+from tests.helpers import BLOCK_SIZES, REC_NULL, assert_disjoint, live_intervals, pressure_program, random_ops
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def model_opts(o: H.HallocOpts):
+    return M.make_opts(memory=o.memory, halloc_fraction=o.halloc_fraction, busy_fraction=o.busy_fraction,
+                       roomy_fraction=o.roomy_fraction, sparse_fraction=o.sparse_fraction,
+                       sb_sz_sh=o.sb_sz_sh, nheads=o.nheads, hash_mode=1)
+
+
+@pytest.fixture
+def halloc():
+    started = []
+
+    def start(opts):
+        H.Halloc.init(opts)
+        started.append(opts)
+        return opts
+
+    yield start
+    if started:
+        H.Halloc.shutdown()
+
+
+def replay_both(opts, ops):
+    """Replay ops on the GPU, then on the model with the GPU's XCD per step."""
+    recs_g, xcc = H.Halloc.replay(ops)
+    heap = M.ModelHeap(model_opts(opts))
+    recs_m, _ = heap.replay(ops, iheads=xcc % opts.nheads)
+    return recs_g, recs_m, heap
+
+
+def assert_same_state(heap, opts):
+    dg = H.Halloc.dump()
+    dm = heap.dump()
+    assert dg.shape == dm.shape
+    diff = np.nonzero(dg != dm)[0]
+    assert diff.size == 0, f"{diff.size} state words differ, first at {diff[:8]}"
+    tot, det = M.check_dump(model_opts(opts), dg)
+    assert tot == 0, det
+
+
+def test_geometry_matches_model(halloc):
+    opts = halloc(H.HallocOpts())
+    g = H.Halloc.geometry()
+    say(f"geometry nsbs={g['nsbs']} nheads={g['nheads']}")
+    t = M.size_table(model_opts(opts))
+    assert (g["size_table"] == t).all()
+    assert g["nsbs"] == int(0.75 * (512 << 20)) // (4 << 20)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_replay_single_lane(halloc, seed):
+    say(f"replay single lane seed={seed}")
+    opts = halloc(H.HallocOpts(memory=256 << 20, sb_sz_sh=21, nheads=1))
+    rng = np.random.default_rng(100 + seed)
+    ops = random_ops(rng, 3000, 1, BLOCK_SIZES + [0, 1, 17, 100, 1000, 3000])
+    recs_g, recs_m, heap = replay_both(opts, ops)
+    assert (recs_g == recs_m).all()
+    assert_same_state(heap, opts)
+
+
+@pytest.mark.parametrize("nheads", [1, 8])
+def test_replay_wave(halloc, nheads):
+    say(f"replay one wave nheads={nheads}")
+    opts = halloc(H.HallocOpts(memory=512 << 20, sb_sz_sh=21, nheads=nheads))
+    rng = np.random.default_rng(7 + nheads)
+    ops = random_ops(rng, 200, 64, BLOCK_SIZES + [20, 40, 200, 700, 2500])
+    recs_g, recs_m, heap = replay_both(opts, ops)
+    assert (recs_g == recs_m).all()
+    assert_same_state(heap, opts)
+    table = M.size_table(model_opts(opts))
+    assert_disjoint(live_intervals(ops, recs_g, table), 1 << 21)
+
+
+def test_replay_full_waves_of_one_class(halloc):
+    # the plain workload's shape: whole waves of 16 B, then whole waves freed
+    say("replay full waves 16 B")
+    opts = halloc(H.HallocOpts(memory=64 << 20, sb_sz_sh=21, nheads=1))
+    n = 400
+    ops = np.zeros((2 * n, 64, 2), dtype=np.uint32)
+    ops[:n, :, 0] = 1
+    ops[:n, :, 1] = 16
+    ops[n:, :, 0] = 2
+    ops[n:, :, 1] = np.arange(n * 64).reshape(n, 64)
+    recs_g, recs_m, heap = replay_both(opts, ops)
+    assert (recs_g == recs_m).all()
+    assert_same_state(heap, opts)
+
+
+def test_replay_pressure_exhaustion_and_reuse(halloc):
+    # 6 slabs: exhaustion (NULL), head replacement, busy-set heads, slabs freed
+    # and taken over by other classes
+    say("replay pressure")
+    opts = halloc(H.HallocOpts(memory=16 << 20, sb_sz_sh=21, nheads=1))
+    ops = pressure_program()
+    recs_g, recs_m, heap = replay_both(opts, ops)
+    nnull = int((recs_g[:2500, :, 0][ops[:2500, :, 0] == 1] == REC_NULL).sum())
+    say(f"pressure nulls={nnull}")
+    assert nnull > 0
+    assert (recs_g == recs_m).all()
+    assert_same_state(heap, opts)
+    phase1 = set(recs_g[:2500, :, 1][ops[:2500, :, 0] == 1].tolist()) - {REC_NULL}
+    phase3 = set(recs_g[5500:, :, 1][ops[5500:, :, 0] == 1].tolist()) - {REC_NULL}
+    assert phase3 and phase3 <= phase1
+
+
+def test_replay_large_requests(halloc):
+    say("replay large")
+    opts = halloc(H.HallocOpts(memory=256 << 20, sb_sz_sh=21, nheads=1))
+    rng = np.random.default_rng(11)
+    ops = random_ops(rng, 200, 8, BLOCK_SIZES, large_every=10)
+    recs_g, recs_m, heap = replay_both(opts, ops)
+    assert (recs_g == recs_m).all()
+    assert_same_state(heap, opts)
+
+
+def test_plain_workload_invariants(halloc):
+    say("plain 2^18 lanes x 2 iters")
+    opts = halloc(H.HallocOpts())
+    r = H.Halloc.run_plain(nthreads=1 << 18, bs=256, size=16, iters=2, warmup=1, launches=3)
+    assert r["errors"] == 0 and r["nulls"] == 0
+    d = H.Halloc.dump()
+    tot, det = M.check_dump(model_opts(opts), d)
+    assert tot == 0, det
+    g = H.Halloc.geometry()
+    assert ((d[:g["nsbs"]] >> np.uint64(32)) == 0).all()
+
+
+def test_alloc_check_and_persistence(halloc):
+    # check_alloc (tst/common/common.cu:468-529): no NULL, sorted blocks do not
+    # overlap; each block keeps its first word across kernel launches
+    say("alloc check 2^16 mixed sizes")
+    opts = halloc(H.HallocOpts())
+    rng = np.random.default_rng(21)
+    sizes = rng.integers(1, 3073, 1 << 16).astype(np.uint32)
+    ptrs = H.Halloc.malloc_sizes(sizes)
+    assert (ptrs != 0).all()
+    order = np.argsort(ptrs)
+    p, s = ptrs[order], sizes[order]
+    assert (p[:-1] + s[:-1] <= p[1:]).all()
+    align = np.where(np.isin(np.array(BLOCK_SIZES)[np.searchsorted(BLOCK_SIZES, np.maximum(s, 16))] % 3, [0]), 8, 16)
+    assert (p % align.astype(np.uint64) == 0).all()
+    assert (H.Halloc.read_words(ptrs) == sizes).all()
+    d = H.Halloc.dump()
+    tot, det = M.check_dump(model_opts(opts), d)
+    assert tot == 0, det
+    H.Halloc.free_ptrs(ptrs)
+    d = H.Halloc.dump()
+    g = H.Halloc.geometry()
+    assert ((d[:g["nsbs"]] >> np.uint64(32)) == 0).all()
+    assert M.check_dump(model_opts(opts), d)[0] == 0
+
+
+def test_freeslabs_reuse_across_classes(halloc):
+    # tst/corr/freeslabs/freeslabs.cu:35-71 with its 512 MiB pool and default
+    # options: 40% of memory at each size 16..1024 in turn, allocated and freed
+    say("freeslabs")
+    opts = halloc(H.HallocOpts())
+    size, step = 16, 8
+    while size <= 1024:
+        n = int(0.4 * opts.memory / size)
+        ptrs = H.Halloc.malloc_sizes(np.full(n, size, dtype=np.uint32))
+        assert (ptrs != 0).all(), size
+        H.Halloc.free_ptrs(ptrs)
+        size += step
+        step = 128 if size >= 256 else 16 if size >= 64 else 8
+    d = H.Halloc.dump()
+    assert M.check_dump(model_opts(opts), d)[0] == 0
+
+
+def test_extfrag_and_free_bytes(halloc):
+    say("extfrag")
+    opts = halloc(H.HallocOpts())
+    g = H.Halloc.geometry()
+    assert H.Halloc.free_bytes() == g["arena_bytes"]
+    ptrs = H.Halloc.malloc_sizes(np.full(1 << 16, 64, dtype=np.uint32))
+    assert H.Halloc.free_bytes() == g["arena_bytes"] - 64 * (1 << 16)
+    f = H.Halloc.extfrag(True)
+    assert 0.0 <= f < 1.0
+    H.Halloc.free_ptrs(ptrs)

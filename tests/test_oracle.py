@@ -1,0 +1,317 @@
+"""CPU tests of the host model (oracle/) against closed forms, brute force,
+the reference's size table and the reference test suite's invariants."""
+import json
+import math
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import model as M
+from tests.helpers import BLOCK_SIZES, assert_disjoint, live_intervals, pressure_program, random_ops
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    M.build()
+
+
+def primes_below(n):
+    if n < 3:
+        return []
+    sieve = bytearray([1]) * n
+    sieve[0:2] = b"\x00\x00"
+    for i in range(2, int(n ** 0.5) + 1):
+        if sieve[i]:
+            sieve[i * i::i] = bytearray(len(sieve[i * i::i]))
+    return [i for i in range(n) if sieve[i]]
+
+
+def test_size_id_is_smallest_fitting_block():
+    # src/halloc.cu:53-67: the class is the smallest 2*unit / 3*unit block holding n (>= 16 B)
+    L = M.lib()
+    for n in range(0, 3073):
+        want = next(i for i, b in enumerate(BLOCK_SIZES) if b >= max(n, 16))
+        assert L.hm_size_id(n) == want, n
+
+
+@pytest.mark.parametrize("sz", [16, 24, 256, 384])
+def test_chunk_arithmetic_round_trip(sz):
+    # src/size-info.h:50-70: chunk_mul(v) = v*sz, chunk_div(v*sz + r) = v for r < sz
+    L = M.lib()
+    cs = L.hm_chunk_val(sz)
+    for v in list(range(0, 4096)) + [262143, 174759, 16383, 10921]:
+        if v * sz >= 1 << 32:
+            continue
+        assert L.hm_chunk_mul(v, cs) == v * sz
+        for r in (0, 1, sz - 1):
+            if v * sz + r < 1 << 32:
+                assert L.hm_chunk_div(v * sz + r, cs) == v
+
+
+def test_max_prime_below_matches_sieve():
+    # src/utils.cu:8-24: largest prime p < n with n % p and nb % p nonzero
+    L = M.lib()
+    P = primes_below(70000)
+    rng = np.random.default_rng(0)
+    for n in list(range(4, 400)) + [int(x) for x in rng.integers(400, 70000, 300)]:
+        nb = int(rng.integers(1, 1 << 20))
+        cands = [p for p in P if 3 <= p < n and n % p and nb % p]
+        want = max(cands) if cands else 0
+        assert L.hm_max_prime_below(n, nb) == want, (n, nb)
+
+
+def test_reference_size_table_matches_golden():
+    # the reference's own table at 4 MiB slabs (src/halloc.cu:363-394)
+    gold = json.load(open(os.path.join(GOLDEN, "size_table_ref_sb22.json")))["rows"]
+    t = M.size_table(M.make_opts(hash_mode=0))
+    F = {f: i for i, f in enumerate(M.SIZE_TABLE_FIELDS)}
+    for row in gold:
+        r = t[row["id"]]
+        assert r[F["block_sz"]] == row["block_sz"]
+        assert r[F["chunk_id"]] == row["chunk_id"]
+        assert r[F["chunk_bytes"]] == row["chunk_bytes"]
+        assert r[F["ncib"]] == row["ncib"]
+        assert r[F["nblocks"]] == row["nblocks"]
+        assert r[F["nchunks"]] == row["nchunks"]
+        assert r[F["hash_step"]] == row["hash_step"]
+        assert r[F["roomy"]] == row["roomy"]
+        assert r[F["busy"]] == row["busy"]
+        assert r[F["sparse"]] == row["sparse"]
+
+
+def test_wave_run_tables_need_2mib_slabs():
+    # with 64-block runs, 1 MiB slabs leave the 3072 B class 320 blocks and no
+    # admissible hash prime below 320/256 + 320/64 = 6 (5 divides 320, 3 divides 6)
+    with pytest.raises(ValueError):
+        M.size_table(M.make_opts(sb_sz_sh=20, hash_mode=1))
+
+
+@pytest.mark.parametrize("hash_mode", [0, 1])
+@pytest.mark.parametrize("sh", [20, 21, 22, 24, 26])
+def test_size_table_properties(hash_mode, sh):
+    fr = dict(busy=0.835, roomy=0.6, sparse=0.012)
+    if hash_mode == 1 and sh == 20:
+        pytest.skip("not a valid wave-run geometry (see test_wave_run_tables_need_2mib_slabs)")
+    t = M.size_table(M.make_opts(sb_sz_sh=sh, hash_mode=hash_mode))
+    F = {f: i for i, f in enumerate(M.SIZE_TABLE_FIELDS)}
+    sb = 1 << sh
+    P = primes_below(1 << 17)
+    for sid in range(16):
+        r = {f: int(t[sid][i]) for f, i in F.items()}
+        assert r["block_sz"] == BLOCK_SIZES[sid]
+        assert r["chunk_bytes"] in (16, 24, 256, 384)
+        assert r["block_sz"] % r["chunk_bytes"] == 0 and r["ncib"] == r["block_sz"] // r["chunk_bytes"]
+        assert r["ncib"] in (1, 2, 4, 8)
+        tmod = r["tmod"]
+        assert tmod == (64 if hash_mode else 1 << max(2 - sid // 2, 0))
+        # nblocks: the largest multiple of tmod that fits
+        assert r["nblocks"] % tmod == 0
+        assert r["nblocks"] * r["block_sz"] <= sb < (r["nblocks"] + tmod) * r["block_sz"]
+        assert r["nchunks"] == r["nblocks"] * r["ncib"]
+        # hash step = ncib * the largest admissible prime
+        n, nb = r["nblocks"] // 256 + r["nblocks"] // 64, r["nblocks"]
+        p = r["hash_step"] // r["ncib"]
+        assert r["hash_step"] % r["ncib"] == 0
+        assert p == max(q for q in P if 3 <= q < n and n % q and nb % q)
+        # thresholds: fraction * nchunks truncated (double arithmetic)
+        for f in ("busy", "roomy", "sparse"):
+            assert r[f] == int(fr[f] * r["nchunks"])
+            assert abs(Fraction(r[f]) - Fraction(fr[f]) * r["nchunks"]) < 1
+        assert r["sparse"] <= r["roomy"] <= r["busy"] <= r["nchunks"]
+
+
+def test_wave_run_initial_positions():
+    # hash_mode 1: 64 consecutive tickets fill the tail of one aligned 64-block
+    # run and the head of the next, contiguously, and never collide
+    L = M.lib()
+    rng = np.random.default_rng(1)
+    for sh in (21, 22, 26):
+        t = M.size_table(M.make_opts(sb_sz_sh=sh))
+        for sid in range(16):
+            ncib, nchunks = int(t[sid][3]), int(t[sid][0])
+            for old in [0x0B0B0B0B, 0, 1, 63, 64, 0xFFFFFFFF, 0xFFFFFFC0] + [int(x) for x in rng.integers(0, 1 << 32, 10)]:
+                pos = [L.hm_ichunk_init((old + l) & 0xFFFFFFFF, sh, sid, 1) for l in range(64)]
+                split = 64 - (old & 63)
+                a, b = pos[:split], pos[split:]
+                assert a[0] % ncib == 0 and (a[0] - (old & 63) * ncib) % (64 * ncib) == 0
+                assert a == [a[0] + l * ncib for l in range(split)]
+                if b:
+                    assert b[0] % (64 * ncib) == 0
+                    assert b == [b[0] + l * ncib for l in range(len(b))]
+                assert len(set(pos)) == 64 and max(pos) < nchunks
+
+
+def test_reference_hash_initial_positions_are_block_aligned():
+    L = M.lib()
+    t = M.size_table(M.make_opts(hash_mode=0))
+    for sid in range(16):
+        ncib, nchunks = int(t[sid][3]), int(t[sid][0])
+        for cv in range(0x0B0B0B0B, 0x0B0B0B0B + 256):
+            ich = L.hm_ichunk_init(cv, 22, sid, 0)
+            assert ich % ncib == 0 and ich < nchunks
+
+
+@pytest.mark.parametrize("sid", [0, 7, 15])
+def test_probe_sequence_block_aligned_and_spreading(sid):
+    # src/slab.cuh:492-495 on a 2 MiB slab: every probe lands on a block start;
+    # the probes of one lane keep reaching new blocks (brute-force enumeration)
+    L = M.lib()
+    t = M.size_table(M.make_opts(sb_sz_sh=21))
+    ncib, nchunks, nblocks = int(t[sid][3]), int(t[sid][0]), int(t[sid][9])
+    seen = set()
+    for itry in range(0, 4 * nblocks):
+        ich = L.hm_probe_next(0, itry, 21, sid, 1)
+        assert ich % ncib == 0 and ich < nchunks
+        seen.add(ich)
+    assert len(seen) >= nblocks // 4
+
+
+# ------------------------------------------------------------ model behaviour
+@pytest.mark.parametrize("lanes", [1, 64])
+def test_model_replay_invariants(lanes):
+    rng = np.random.default_rng(lanes)
+    # 96 slabs: room for a head and a cached head for each of the 16 classes
+    opts = M.make_opts(memory=256 << 20, sb_sz_sh=21, nheads=1)
+    h = M.ModelHeap(opts)
+    table = M.size_table(opts)
+    ops = random_ops(rng, 300 if lanes == 64 else 4000, lanes, BLOCK_SIZES + [1, 17, 100, 1000, 3000])
+    recs, _ = h.replay(ops)
+    tot, det = M.check_dump(opts, h.dump())
+    assert tot == 0, det
+    assert_disjoint(live_intervals(ops, recs, table), 1 << 21)
+    assert (recs[ops[..., 0] == 1][:, 0] != M.REC_NULL).all()
+
+
+def test_model_free_everything_returns_slabs():
+    # freeslabs (tst/corr/freeslabs/freeslabs.cu:35-71): after freeing every
+    # block, every non-head slab is free and reusable by another class
+    opts = M.make_opts(memory=64 << 20, sb_sz_sh=21, nheads=1)
+    h = M.ModelHeap(opts)
+    rng = np.random.default_rng(5)
+    n = 64 * 200
+    ops = np.zeros((200 * 2, 64, 2), dtype=np.uint32)
+    ops[:200, :, 0] = 1
+    ops[:200, :, 1] = rng.choice(BLOCK_SIZES[:10], size=(200, 64))
+    ops[200:, :, 0] = 2
+    ops[200:, :, 1] = np.arange(n).reshape(200, 64)
+    h.replay(ops)
+    d = h.dump()
+    tot, det = M.check_dump(opts, d)
+    assert tot == 0, det
+    nsbs = h.nsbs
+    ctr = d[:nsbs]
+    assert ((ctr >> np.uint64(32)) == 0).all()
+    desc = d[nsbs:3 * nsbs].reshape(nsbs, 2)
+    heads = (desc[:, 1] >> np.uint64(32)) != 0
+    free = (ctr >> np.uint64(8)) & np.uint64(0xF) == 0xF
+    assert (heads | free).all()
+
+
+def test_model_exhaustion_only_when_full():
+    # NULL only when no slab of the class has room left by the scheme's rules
+    opts = M.make_opts(memory=8 << 20, sb_sz_sh=21, nheads=1)
+    h = M.ModelHeap(opts)
+    t = M.size_table(opts)
+    nsbs = h.nsbs
+    busy, nchunks = int(t[0][7]), int(t[0][0])
+    cap = nsbs * nchunks
+    ops = np.zeros((cap, 1, 2), dtype=np.uint32)
+    ops[:, 0] = (1, 16)
+    recs, _ = h.replay(ops)
+    nulls = np.nonzero(recs[:, 0, 0] == M.REC_NULL)[0]
+    assert len(nulls) > 0
+    first = nulls[0]
+    # replay the prefix up to the first NULL on a fresh heap and inspect it
+    h2 = M.ModelHeap(opts)
+    h2.replay(ops[:first + 1])
+    d = h2.dump()
+    ctr = d[:nsbs] >> np.uint64(32)
+    desc = d[nsbs:3 * nsbs].reshape(nsbs, 2)
+    is_head = (desc[:, 1] >> np.uint64(32)) != 0
+    assert (desc[:, 0] & np.uint64(0xFFFFFFFF) == 0).all()   # every slab serves class 0
+    assert (ctr[is_head] >= busy).all()
+    assert (ctr[~is_head] >= (busy + nchunks) // 2).all()     # the last-resort limit
+    assert int(ctr.sum()) == first                              # every success is counted
+
+
+def test_checker_catches_planted_mistakes():
+    opts = M.make_opts(memory=32 << 20, sb_sz_sh=21, nheads=1)
+    h = M.ModelHeap(opts)
+    ops = np.zeros((50, 64, 2), dtype=np.uint32)
+    ops[:, :, 0] = 1
+    ops[:, :, 1] = 48
+    h.replay(ops)
+    d = h.dump()
+    assert M.check_dump(opts, d)[0] == 0
+    nsbs = h.nsbs
+    nsw = (nsbs + 63) // 64
+    off_bits = 3 * nsbs + M.NSETS * nsw + 3 * 16
+    # 1: a reserved bit cleared
+    bad = d.copy()
+    w = off_bits + int(np.nonzero(d[off_bits:] != 0)[0][0])
+    bad[w] &= bad[w] - np.uint64(1)
+    assert M.check_dump(opts, bad)[1][0] > 0
+    # 2: a counter off by one
+    bad = d.copy()
+    b = int(np.nonzero((d[:nsbs] >> np.uint64(32)) > 0)[0][0])
+    bad[b] += np.uint64(1 << 32)
+    assert M.check_dump(opts, bad)[1][0] > 0
+    # 3: a free slab dropped from the free set
+    bad = d.copy()
+    free_b = int(np.nonzero(((d[:nsbs] >> np.uint64(8)) & np.uint64(0xF)) == 0xF)[0][0])
+    bad[3 * nsbs + free_b // 64] &= ~np.uint64(1 << (free_b % 64))
+    assert M.check_dump(opts, bad)[1][2] > 0
+
+
+def test_concurrent_model_and_glibc_baselines_run_clean():
+    rate, bad = M.bench_plain(M.make_opts(nheads=8), 8, 1 << 14, 2)
+    assert bad == 0 and rate > 0
+    rate, bad = M.bench_glibc(4, 1 << 14, 2)
+    assert bad == 0 and rate > 0
+
+
+def test_model_pressure_program_reuses_slabs_across_classes():
+    opts = M.make_opts(memory=16 << 20, sb_sz_sh=21, nheads=1)
+    h = M.ModelHeap(opts)
+    ops = pressure_program()
+    recs, _ = h.replay(ops)
+    m = ops[..., 0] == 1
+    assert (recs[:2500][m[:2500]][:, 0] == M.REC_NULL).sum() > 0      # the pool ran out
+    assert (recs[5500:][m[5500:]][:, 0] != M.REC_NULL).all()          # and came back
+    first = set(recs[:2500, :, 1][m[:2500]].tolist()) - {M.REC_NULL}
+    later = set(recs[5500:, :, 1][m[5500:]].tolist()) - {M.REC_NULL}
+    assert later and later <= first
+    assert M.check_dump(opts, h.dump())[0] == 0
+
+
+def freeslabs_sizes():
+    """Sizes of the reference's freeslabs sweep (tst/corr/freeslabs/freeslabs.cu:39-69)."""
+    out, size, step = [], 16, 8
+    while size <= 1024:
+        out.append(size)
+        size += step
+        step = 128 if size >= 256 else 16 if size >= 64 else 8
+    return out
+
+
+def test_model_freeslabs_with_idle_head_stealing():
+    # 24 slabs, 8 heads per class: the sweep parks idle heads of 13 classes
+    # (up to 16 slabs per class); each class must still get 40% of memory
+    opts = M.make_opts(memory=64 << 20, sb_sz_sh=21, nheads=8)
+    h = M.ModelHeap(opts)
+    for size in freeslabs_sizes():
+        n = int(0.4 * (64 << 20) / size) // 64 * 64
+        steps = n // 64
+        ops = np.zeros((2 * steps, 64, 2), dtype=np.uint32)
+        ops[:steps, :, 0] = 1
+        ops[:steps, :, 1] = size
+        ops[steps:, :, 0] = 2
+        ops[steps:, :, 1] = np.arange(n).reshape(steps, 64)
+        iheads = np.arange(2 * steps, dtype=np.uint32) % 8
+        recs, _ = h.replay(ops, iheads)
+        assert (recs[:steps, :, 0] != M.REC_NULL).all(), size
+    assert M.check_dump(opts, h.dump())[0] == 0
