@@ -94,7 +94,7 @@ def lib() -> ctypes.CDLL:
         "ha_free_bytes_c": (i32, [P(u64)]),
         "ha_last_error": (ctypes.c_char_p, []),
         "ha_geometry_c": (i32, [P(_Geometry)]),
-        "ha_replay": (i32, [vp, i32, i32, vp, vp]),
+        "ha_replay": (i32, [vp, i32, i32, vp, vp, vp]),
         "ha_dump_words": (ctypes.c_int64, []),
         "ha_dump": (i32, [vp, ctypes.c_int64]),
         "ha_run_plain": (i32, [i32, i32, i32, i32, i32, i32, vp, P(_PlainResult)]),
@@ -152,13 +152,16 @@ class Halloc:
 
     # ---------------------------------------------------------- workloads
     @staticmethod
-    def replay(ops: np.ndarray):
-        """ops: (nsteps, lanes, 2) uint32; returns (recs (nsteps, lanes, 4), xcc (nsteps,))."""
+    def replay(ops: np.ndarray, iheads=None):
+        """ops: (nsteps, lanes, 2) uint32; iheads: optional head set per step.
+        Returns (recs (nsteps, lanes, 4), head set used per step)."""
         ops = np.ascontiguousarray(ops, dtype=np.uint32)
         nsteps, lanes = ops.shape[0], ops.shape[1]
         recs = np.zeros((nsteps, lanes, 4), dtype=np.uint32)
         xcc = np.zeros(nsteps, dtype=np.uint32)
-        _check(lib().ha_replay(ops.ctypes.data, nsteps, lanes, recs.ctypes.data, xcc.ctypes.data))
+        ih = None if iheads is None else np.ascontiguousarray(iheads, dtype=np.uint32)
+        _check(lib().ha_replay(ops.ctypes.data, nsteps, lanes, None if ih is None else ih.ctypes.data,
+                               recs.ctypes.data, xcc.ctypes.data))
         return recs, xcc
 
     @staticmethod

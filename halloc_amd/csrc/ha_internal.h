@@ -4,8 +4,9 @@
 //   arena       nsbs * sb_sz bytes, one allocation; slab b = [b*sb_sz, (b+1)*sb_sz)
 //   bits        nsbs * (sb_sz/1024) u64: one bit per 16 B chunk      (src/halloc.cu:345)
 //   nibs        nsbs * (sb_sz/256)  u64: 4-bit block length per chunk (src/slab.cuh:60-83)
-//   ctr         nsbs * CTR_STRIDE   u64: slab counter, one per 128 B line (src/slab.h:52-132)
-//   desc        nsbs Desc: size id, chunk id, chunk size, head flag     (src/slab.h:16-29)
+//   ctr         nsbs * CTR_STRIDE   u64: slab counter, one per 128 B line (src/slab.h:52-132):
+//               count bits 32..63, head references 16..23, chunk id 8..11
+//   desc        nsbs Desc: size id, chunk id, chunk size, head references (src/slab.h:16-29)
 //   sb_lock     nsbs u32                                                (src/slab.cuh:18)
 //   sets        NSETS * nsw u64 slab bitmaps: free, roomy[16], busy[16], sparse[4]
 //   head/cached/head_lock/ticket: nheads * 16 u32 each, one per 128 B line
@@ -49,7 +50,7 @@ constexpr u32 HEAD_EXT = 0x80000000u;
 
 constexpr u32 CTR_CHUNK_SH = 8;
 constexpr u64 CTR_CHUNK_NONE = 0xF;
-constexpr u64 CTR_HEAD = 1ull << 16;
+constexpr u64 CTR_HEAD = 1ull << 16;  // one head reference (bits 16..23 count them)
 constexpr u32 CTR_COUNT_SH = 32;
 
 struct SizeInfo {
@@ -99,7 +100,7 @@ __host__ __device__ inline u32 chunk_div(u32 v, u32 cs) {
 }
 __host__ __device__ inline u32 ctr_count(u64 c) { return (u32)(c >> CTR_COUNT_SH); }
 __host__ __device__ inline u32 ctr_chunk(u64 c) { return (u32)((c >> CTR_CHUNK_SH) & 0xF); }
-__host__ __device__ inline bool ctr_is_head(u64 c) { return (c & CTR_HEAD) != 0; }
+__host__ __device__ inline bool ctr_is_head(u64 c) { return ((c >> 16) & 0xFF) != 0; }
 __host__ __device__ inline u64 ctr_chunk_delta(u32 chunk_id) {
   return ((CTR_CHUNK_NONE - chunk_id) & 0xF) << CTR_CHUNK_SH;
 }

@@ -168,22 +168,55 @@ HA_DEV bool needs_transition(u32 sb, u64 old, u32 change) {
   return sb_class(oc, s) != sb_class(oc - change, s);
 }
 
-// detach_head (src/slab.cuh:227-255); caller holds sb_lock[sb]
-__device__ __noinline__ void detach_head(u32 sb) {
+// drop one head reference (detach_head, src/slab.cuh:227-255); the last one
+// files the slab by its count.  Caller holds sb_lock[sb].  True if the slab
+// became free.
+__device__ __noinline__ bool detach_head(u32 sb) {
   Desc* d = ha_p.desc + sb;
-  st32(&d->is_head, 0);
+  const u32 refs = ld32(&d->is_head) - 1;
+  st32(&d->is_head, refs);
   u64 old = a_sub(ctr_of(sb), CTR_HEAD);
+  if (refs) return false;
   u32 count = ctr_count(old);
   u32 size_id = ld32(&d->size_id), chunk_id = ld32(&d->chunk_id);
-  if (size_id == SZ_NONE) return;
+  if (size_id == SZ_NONE) return false;
   u32 cls = sb_class(count, ha_p.si[size_id]);
-  if (cls == CLS_FREE) try_mark_free(sb, size_id, chunk_id, true);
-  else set_add(class_set(cls, size_id, chunk_id), sb);
+  if (cls == CLS_FREE) {
+    try_mark_free(sb, size_id, chunk_id, true);
+    return ld32(&d->chunk_id) == SZ_NONE;
+  }
+  set_add(class_set(cls, size_id, chunk_id), sb);
+  return false;
 }
 
-// Takes an empty head or cached head of another (head, class) pair and files
-// its slab as free; false if there is none.  Caller holds the head lock of
-// (ihead, size_id); the victim's head lock is only try-locked.
+// Shares the head of another XCD's pair of the same class if it has room.
+// Caller holds the head lock of (ihead, size_id).
+__device__ __noinline__ u32 share_head(u32 ihead, u32 size_id, u32 chunk_id) {
+  const SizeInfo& s = ha_p.si[size_id];
+  for (u32 ih = 0; ih < ha_p.nheads; ++ih) {
+    if (ih == ihead) continue;
+    u32 w = ld32(hot(ha_p.head, ih, size_id));
+    if (w == SB_NONE) continue;
+    u32 sb = w & ~HEAD_EXT;
+    if (ctr_count(ld64(ctr_of(sb))) >= head_limit(w, s)) continue;
+    Desc* d = ha_p.desc + sb;
+    u32* l = ha_p.sb_lock + sb;
+    lock(l);
+    u32 refs = ld32(&d->is_head);
+    bool ok = refs > 0 && refs < 255 && ld32(&d->size_id) == size_id && ctr_chunk(ld64(ctr_of(sb))) == chunk_id;
+    if (ok) {
+      st32(&d->is_head, refs + 1);
+      a_add(ctr_of(sb), CTR_HEAD);
+    }
+    unlock(l);
+    if (ok) return w;
+  }
+  return SB_NONE;
+}
+
+// Takes over empty heads and cached heads of other (head, class) pairs until
+// one slab becomes free; false if none does.  Caller holds the head lock of
+// (ihead, size_id); the victims' head locks are only try-locked.
 __device__ __noinline__ bool steal_idle_head(u32 ihead, u32 size_id) {
   for (u32 ih = 0; ih < ha_p.nheads; ++ih) {
     for (u32 c = 0; c < NSIZES; ++c) {
@@ -197,26 +230,39 @@ __device__ __noinline__ bool steal_idle_head(u32 ihead, u32 size_id) {
         if (ctr_count(ld64(ctr_of(sb))) != 0) continue;
         if (!try_lock(lp)) continue;
         bool ok = ld32(wp) == w && ctr_count(ld64(ctr_of(sb))) == 0;
+        bool freed = false;
         if (ok) {
           st32(wp, SB_NONE);
           u32* l = ha_p.sb_lock + sb;
           lock(l);
-          detach_head(sb);
+          freed = detach_head(sb);
           unlock(l);
         }
         unlock(lp);
-        if (ok) return true;
+        if (freed) return true;
       }
     }
   }
   return false;
 }
 
-// find_sb_for_size (src/slab.cuh:259-372): roomy -> sparse -> free -> busy,
-// then (for a head needed now) an idle head of another class
-__device__ __noinline__ u32 find_sb_for_size(u32 size_id, u32 chunk_id, u32 ihead, bool steal) {
+// With several head sets, free slabs are scarce below nheads * 16 / 2.
+HA_DEV bool free_scarce() {
+  if (ha_p.nheads <= 1) return false;
+  u32 n = 0;
+  u64* w = set_words(0);
+  for (u32 i = 0; i < ha_p.nsw; ++i) n += (u32)__builtin_popcountll(ld64(w + i));
+  return n < ha_p.nheads * NSIZES / 2;
+}
+
+// find_sb_for_size (src/slab.cuh:259-372): roomy -> sparse -> free -> busy;
+// then, for a head needed now, another XCD's head of the class, then an idle
+// head of another class.  A cache refill (now == false) skips the free set
+// when free slabs are scarce; a head needed now then tries sharing first.
+__device__ __noinline__ u32 find_sb_for_size(u32 size_id, u32 chunk_id, u32 ihead, bool now) {
   const SizeInfo& s = ha_p.si[size_id];
   u32 sb;
+  bool tried_share = false, stole = false;
 again:
   while ((sb = set_pop(set_roomy(size_id))) != SB_NONE) {
     Desc* d = ha_p.desc + sb;
@@ -252,6 +298,13 @@ again:
     }
     unlock(l);
     if (found) return sb;
+  }
+  const bool scarce = free_scarce();
+  if (scarce && !now) return SB_NONE;
+  if (scarce && now && !tried_share) {
+    tried_share = true;
+    u32 shared = share_head(ihead, size_id, chunk_id);
+    if (shared != SB_NONE) return shared;
   }
   while ((sb = set_pop(0)) != SB_NONE) {
     Desc* d = ha_p.desc + sb;
@@ -294,8 +347,14 @@ again:
       if (found) return sb | HEAD_EXT;
     }
   }
-  if (steal && steal_idle_head(ihead, size_id)) {
-    steal = false;
+  if (!now) return SB_NONE;
+  if (!tried_share) {
+    tried_share = true;
+    u32 shared = share_head(ihead, size_id, chunk_id);
+    if (shared != SB_NONE) return shared;
+  }
+  if (!stole && steal_idle_head(ihead, size_id)) {
+    stole = true;
     goto again;
   }
   return SB_NONE;
@@ -463,10 +522,9 @@ HA_DEV void* malloc_group(u32 sid, u64 g, u32 ihead, u32 me) {
   return p;
 }
 
-HA_DEV void* malloc_small(u32 nbytes) {
+HA_DEV void* malloc_small(u32 nbytes, u32 ihead) {
   const u32 me = lane_id();
   const u32 sid = size_id_from_nbytes(nbytes);
-  const u32 ihead = rfl(xcc_id() % ha_p.nheads);
   void* res = nullptr;
   u64 rem = ballot(true);
   while (rem) {
@@ -579,7 +637,14 @@ HA_DEV void free_small(u64 off) {
 __device__ void* hamalloc(size_t nbytes) {
   // requests above 3072 B go to the HIP device heap (src/halloc.cu:185-195)
   if (nbytes > ha::MAX_BLOCK_SZ) return malloc(nbytes);
-  return ha::malloc_small((ha::u32)nbytes);
+  return ha::malloc_small((ha::u32)nbytes, ha::rfl(ha::xcc_id() % ha::ha_p.nheads));
+}
+
+// hamalloc with the head set chosen by the caller instead of by the XCD id;
+// the replay harness uses it to pin behaviour across head sets
+__device__ void* ha_malloc_on_head(size_t nbytes, unsigned ihead) {
+  if (nbytes > ha::MAX_BLOCK_SZ) return malloc(nbytes);
+  return ha::malloc_small((ha::u32)nbytes, ha::rfl(ihead % ha::ha_p.nheads));
 }
 
 __device__ void hafree(void* p) {

@@ -16,6 +16,8 @@
 #include "../../include/halloc_amd.h"
 #include "ha_internal.h"
 
+__device__ void* ha_malloc_on_head(size_t nbytes, unsigned ihead);
+
 namespace ha {
 const Params& host_params();
 int fail(int code, const std::string& msg);
@@ -54,16 +56,17 @@ struct Stream {
 
 // ------------------------------------------------------------------ replay
 // One wavefront replays ops step by step: malloc lanes, then free lanes.
-__global__ void __launch_bounds__(64) replay_k(const u32* ops, int nsteps, int lanes, u32* recs, u32* xcc,
-                                               void** ptrs) {
+__global__ void __launch_bounds__(64) replay_k(const u32* ops, int nsteps, int lanes, const u32* iheads,
+                                               u32* recs, u32* xcc, void** ptrs) {
   const u32 l = threadIdx.x;
   const ha::Params& P = ha::ha_p;
   for (int st = 0; st < nsteps; ++st) {
     const size_t i = (size_t)st * lanes + l;
     const u32 kind = ops[2 * i], arg = ops[2 * i + 1];
-    if (l == 0) xcc[st] = xcc_id_h();
+    const u32 ih = iheads ? iheads[st] : xcc_id_h();
+    if (l == 0) xcc[st] = ih;
     if (kind == 1) {
-      void* p = hamalloc(arg);
+      void* p = iheads ? ha_malloc_on_head(arg, ih) : hamalloc(arg);
       ptrs[i] = p;
       u32 r0 = 0xffffffffu, r1 = 0xffffffffu, r2 = 0xffffffffu;
       if (p) {
@@ -200,7 +203,8 @@ __global__ void probe_k(u64* buf, u64 mask, int iters, int shape, u64* sink) {
 
 extern "C" {
 
-int ha_replay(const uint32_t* ops, int nsteps, int lanes, uint32_t* recs, uint32_t* xcc) {
+int ha_replay(const uint32_t* ops, int nsteps, int lanes, const uint32_t* iheads, uint32_t* recs,
+              uint32_t* xcc) {
   if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
   if (!ops || !recs || !xcc || nsteps < 0 || lanes < 1 || lanes > 64) return hfail(HA_E_INVALID, "bad replay arguments");
   const size_t nops = (size_t)nsteps * lanes;
@@ -209,7 +213,11 @@ int ha_replay(const uint32_t* ops, int nsteps, int lanes, uint32_t* recs, uint32
     if (ops[2 * i] == 2 && (ops[2 * i + 1] >= nops || ops[2 * i + 1] / lanes >= i / lanes))
       return hfail(HA_E_INVALID, "a free must name a malloc of an earlier step");
   }
-  DBuf dops, drecs, dxcc, dptrs;
+  DBuf dops, drecs, dxcc, dptrs, dih;
+  if (iheads) {
+    HH(dih.alloc((size_t)nsteps * 4));
+    HH(hipMemcpy(dih.p, iheads, (size_t)nsteps * 4, hipMemcpyHostToDevice));
+  }
   HH(dops.alloc(nops * 8));
   HH(drecs.alloc(nops * 16));
   HH(dxcc.alloc((size_t)nsteps * 4));
@@ -217,7 +225,8 @@ int ha_replay(const uint32_t* ops, int nsteps, int lanes, uint32_t* recs, uint32
   HH(hipMemcpy(dops.p, ops, nops * 8, hipMemcpyHostToDevice));
   HH(hipMemset(dptrs.p, 0, nops * 8));
   HH(hipMemset(drecs.p, 0, nops * 16));
-  replay_k<<<1, lanes>>>((const u32*)dops.p, nsteps, lanes, (u32*)drecs.p, (u32*)dxcc.p, (void**)dptrs.p);
+  replay_k<<<1, lanes>>>((const u32*)dops.p, nsteps, lanes, (const u32*)dih.p, (u32*)drecs.p, (u32*)dxcc.p,
+                         (void**)dptrs.p);
   HH(hipGetLastError());
   HH(hipDeviceSynchronize());
   HH(hipMemcpy(recs, drecs.p, nops * 16, hipMemcpyDeviceToHost));

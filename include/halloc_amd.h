@@ -69,8 +69,11 @@ int ha_geometry_c(ha_geometry *g);
    the index of the earlier malloc op for free.  Per step the malloc lanes call
    hamalloc together, then the free lanes call hafree together.
    recs[4*i..]: malloc -> size class (16 = large, 0xffffffff = NULL), slab,
-   chunk; free -> chunks freed, slab, chunk.  xcc[step] = XCD id of the wave. */
-int ha_replay(const uint32_t *ops, int nsteps, int lanes, uint32_t *recs, uint32_t *xcc);
+   chunk; free -> chunks freed, slab, chunk.  iheads (may be NULL) forces the
+   head set of each step's mallocs; otherwise the wave's XCD id picks it.
+   xcc[step] receives the head-set index used (before the modulo nheads). */
+int ha_replay(const uint32_t *ops, int nsteps, int lanes, const uint32_t *iheads, uint32_t *recs,
+              uint32_t *xcc);
 
 /* Canonical state dump (layout documented in oracle/model_capi.cpp). */
 int64_t ha_dump_words(void);

@@ -26,8 +26,12 @@
 //
 // Deliberate differences from the reference, shared with the GPU:
 //   * 64-bit bit, nibble and set words (the reference uses 32-bit words).
-//   * 64-bit slab counter: count in bits 32..63, head flag bit 16,
-//     chunk id bits 8..11 (0xF = none).
+//   * 64-bit slab counter: count in bits 32..63, head references bits 16..23,
+//     chunk id bits 8..11 (0xF = none).  A slab may be the head of several
+//     (head, class) pairs of ONE class at once: when no set holds a slab for a
+//     new head, the head of another XCD's pair of the same class is shared
+//     (its reference count goes up); a slab is detached when its last head
+//     reference goes.
 //   * one contiguous arena: slab = (offset >> sb_sz_sh); no address grid.
 //   * hash_mode 1 ("wave run"): the reference's initial hash with tmod = 64
 //     (nblocks rounded down to a multiple of 64), and lane l of a class group
@@ -48,6 +52,9 @@
 //   * when no set holds a slab for a new head, an EMPTY head or cached head of
 //     another (head, class) pair is taken over (under a try-lock of its head
 //     lock), so slabs parked as idle heads do not make hamalloc return NULL.
+//   * with several head sets, free slabs are "scarce" when fewer than
+//     nheads * 16 / 2 remain: then a cached head is not refilled from the free
+//     set, and a new head shares its class's head before taking a free slab.
 #pragma once
 
 #include <atomic>
@@ -86,11 +93,12 @@ constexpr u32 HEAD_EXT = 0x80000000u;
 // slab counter layout (64-bit)
 constexpr u32 CTR_CHUNK_SH = 8;
 constexpr u64 CTR_CHUNK_NONE = 0xF;
-constexpr u64 CTR_HEAD = 1ull << 16;
+constexpr u64 CTR_HEAD = 1ull << 16;  // one head reference
 constexpr u32 CTR_COUNT_SH = 32;
 inline u32 ctr_count(u64 c) { return (u32)(c >> CTR_COUNT_SH); }
 inline u32 ctr_chunk(u64 c) { return (u32)((c >> CTR_CHUNK_SH) & 0xF); }
-inline bool ctr_is_head(u64 c) { return (c & CTR_HEAD) != 0; }
+inline bool ctr_is_head(u64 c) { return ((c >> 16) & 0xFF) != 0; }
+inline u32 ctr_head_refs(u64 c) { return (u32)((c >> 16) & 0xFF); }
 inline u64 ctr_chunk_delta(u32 chunk_id) { return ((CTR_CHUNK_NONE - chunk_id) & 0xF) << CTR_CHUNK_SH; }
 
 // ---------------------------------------------------------------- closed forms
@@ -336,21 +344,49 @@ class Heap {
     }
   }
 
-  // detach_head (src/slab.cuh:227-255); caller holds sb_lock[sb]
-  void detach_head(u32 sb) {
-    d_head[sb].store(0);
+  // drop one head reference (detach_head, src/slab.cuh:227-255); the last one
+  // files the slab by its count.  Caller holds sb_lock[sb].  True if the slab
+  // became free.
+  bool detach_head(u32 sb) {
+    u32 refs = d_head[sb].load() - 1;
+    d_head[sb].store(refs);
     u64 old = ctr[sb].fetch_sub(CTR_HEAD);
+    if (refs) return false;
     u32 count = ctr_count(old);
     u32 size_id = d_size[sb].load(), chunk_id = d_chunk[sb].load();
-    if (size_id == SZ_NONE) return;
+    if (size_id == SZ_NONE) return false;
     u32 c = sb_class(count, si[size_id]);
-    if (c == CLS_FREE) try_mark_free(sb, size_id, chunk_id, true);
-    else set_add(class_set(c, size_id, chunk_id), sb);
+    if (c == CLS_FREE) {
+      try_mark_free(sb, size_id, chunk_id, true);
+      return d_chunk[sb].load() == SZ_NONE;
+    }
+    set_add(class_set(c, size_id, chunk_id), sb);
+    return false;
   }
 
-  // take an empty head or cached head of another (head, class) pair and put it
-  // in the free set; returns false if there is none.  Caller holds the head
-  // lock of (ihead, size_id).
+  // share the head of another XCD's pair of the same class if it has room
+  // (caller holds the head lock of (ihead, size_id))
+  u32 share_head(u32 ihead, u32 size_id, u32 chunk_id) {
+    const SizeInfo& s = si[size_id];
+    for (u32 ih = 0; ih < nheads; ++ih) {
+      if (ih == ihead) continue;
+      u32 w = head[(size_t)ih * NSIZES + size_id].load();
+      if (w == SB_NONE) continue;
+      u32 sb = w & ~HEAD_EXT;
+      if (ctr_count(ctr[sb].load()) >= head_limit(w, s)) continue;
+      lock(sb_lock[sb]);
+      u32 refs = d_head[sb].load();
+      bool ok = refs > 0 && refs < 255 && d_size[sb].load() == size_id && ctr_chunk(ctr[sb].load()) == chunk_id;
+      if (ok) { d_head[sb].store(refs + 1); ctr[sb].fetch_add(CTR_HEAD); }
+      unlock(sb_lock[sb]);
+      if (ok) return w;
+    }
+    return SB_NONE;
+  }
+
+  // take over empty heads and cached heads of other (head, class) pairs until
+  // one slab becomes free; false if none does.  Caller holds the head lock of
+  // (ihead, size_id).
   bool steal_idle_head(u32 ihead, u32 size_id) {
     for (u32 ih = 0; ih < nheads; ++ih) {
       for (u32 c = 0; c < NSIZES; ++c) {
@@ -364,25 +400,37 @@ class Heap {
           if (ctr_count(ctr[sb].load()) != 0) continue;
           if (!try_lock(head_lock[hi])) continue;
           bool ok = word.load() == w && ctr_count(ctr[sb].load()) == 0;
+          bool freed = false;
           if (ok) {
             word.store(SB_NONE);
             lock(sb_lock[sb]);
-            detach_head(sb);
+            freed = detach_head(sb);
             unlock(sb_lock[sb]);
           }
           unlock(head_lock[hi]);
-          if (ok) return true;
+          if (freed) return true;
         }
       }
     }
     return false;
   }
 
-  // find_sb_for_size (src/slab.cuh:259-372): roomy -> sparse -> free -> busy,
-  // then (for a head needed now) an idle head of another class
-  u32 find_sb_for_size(u32 size_id, u32 chunk_id, u32 ihead = 0, bool steal = false) {
+  u32 free_slab_count() {
+    u32 n = 0;
+    std::atomic<u64>* w = set_words(0);
+    for (u32 i = 0; i < nsw; ++i) n += (u32)__builtin_popcountll(w[i].load());
+    return n;
+  }
+  bool free_scarce() { return nheads > 1 && free_slab_count() < nheads * NSIZES / 2; }
+
+  // find_sb_for_size (src/slab.cuh:259-372): roomy -> sparse -> free -> busy;
+  // then, for a head needed now, another XCD's head of the class, then an
+  // idle head of another class.  A cache refill (now == false) skips the free
+  // set when free slabs are scarce; a head needed now then tries sharing first.
+  u32 find_sb_for_size(u32 size_id, u32 chunk_id, u32 ihead = 0, bool now = false) {
     const SizeInfo& s = si[size_id];
     u32 sb;
+    bool tried_share = false, stole = false;
   again:
     while ((sb = set_pop(set_roomy(size_id))) != SB_NONE) {
       lock(sb_lock[sb]);
@@ -412,6 +460,13 @@ class Heap {
       }
       unlock(sb_lock[sb]);
       if (found) return sb;
+    }
+    const bool scarce = free_scarce();
+    if (scarce && !now) return SB_NONE;
+    if (scarce && now && !tried_share) {
+      tried_share = true;
+      u32 shared = share_head(ihead, size_id, chunk_id);
+      if (shared != SB_NONE) return shared;
     }
     while ((sb = set_pop(0)) != SB_NONE) {
       lock(sb_lock[sb]);
@@ -446,7 +501,13 @@ class Heap {
         if (found) return sb | HEAD_EXT;
       }
     }
-    if (steal && steal_idle_head(ihead, size_id)) { steal = false; goto again; }
+    if (!now) return SB_NONE;
+    if (!tried_share) {
+      tried_share = true;
+      u32 shared = share_head(ihead, size_id, chunk_id);
+      if (shared != SB_NONE) return shared;
+    }
+    if (!stole && steal_idle_head(ihead, size_id)) { stole = true; goto again; }
     return SB_NONE;
   }
 

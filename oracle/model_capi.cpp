@@ -137,7 +137,8 @@ int hm_dump(void* hv, uint64_t* out) {
 //   2 every set nibble marks a block whose chunks are all set in the bit array
 //   3 free slabs (chunk NONE): count 0, no bits, no nibbles, in the free set
 //   4 every non-head slab with count > 0 is in a set (busy/roomy/sparse)
-//   5 every head / cached head slab carries the head flag
+//   5 a slab's head references (counter and descriptor) equal the number of
+//     head and cached-head words naming it
 // returns the number of violations; detail[0..4] counts per invariant
 int64_t hm_check_dump(const hm_opts_t* o, const uint64_t* d, int64_t* detail) {
   Heap h(to_opts(o));
@@ -180,13 +181,19 @@ int64_t hm_check_dump(const hm_opts_t* o, const uint64_t* d, int64_t* detail) {
         detail[3]++;
     }
   }
+  std::vector<u32> refs(h.nsbs, 0);
   for (u32 i = 0; i < h.nheads * NSIZES; ++i) {
     for (const u64* arr : {heads, cached}) {
       u32 sb = (u32)arr[i];
       if (sb == SB_NONE) continue;
       sb &= ~HEAD_EXT;
-      if (sb >= h.nsbs || !ctr_is_head(ctr[sb]) || !(desc[2 * sb + 1] >> 32)) detail[4]++;
+      if (sb >= h.nsbs) { detail[4]++; continue; }
+      refs[sb]++;
     }
+  }
+  for (u32 b = 0; b < h.nsbs; ++b) {
+    u32 dref = (u32)(desc[2 * b + 1] >> 32);
+    if (ctr_head_refs(ctr[b]) != refs[b] || dref != refs[b]) detail[4]++;
   }
   int64_t tot = 0;
   for (int i = 0; i < 5; ++i) tot += detail[i];

@@ -40,9 +40,9 @@ def halloc():
         H.Halloc.shutdown()
 
 
-def replay_both(opts, ops):
-    """Replay ops on the GPU, then on the model with the GPU's XCD per step."""
-    recs_g, xcc = H.Halloc.replay(ops)
+def replay_both(opts, ops, iheads=None):
+    """Replay ops on the GPU, then on the model with the GPU's head set per step."""
+    recs_g, xcc = H.Halloc.replay(ops, iheads)
     heap = M.ModelHeap(model_opts(opts))
     recs_m, _ = heap.replay(ops, iheads=xcc % opts.nheads)
     return recs_g, recs_m, heap
@@ -121,6 +121,29 @@ def test_replay_pressure_exhaustion_and_reuse(halloc):
     phase1 = set(recs_g[:2500, :, 1][ops[:2500, :, 0] == 1].tolist()) - {REC_NULL}
     phase3 = set(recs_g[5500:, :, 1][ops[5500:, :, 0] == 1].tolist()) - {REC_NULL}
     assert phase3 and phase3 <= phase1
+
+
+def test_replay_shared_and_stolen_heads(halloc):
+    # mallocs spread over 8 head sets in a 96-slab pool: pairs without a slab
+    # share another set's head of their class; later size sweeps take over
+    # idle heads of old classes
+    say("replay shared heads")
+    opts = halloc(H.HallocOpts(nheads=8))
+    rng = np.random.default_rng(5)
+    nw = 600
+    ops = np.zeros((2 * nw, 64, 2), dtype=np.uint32)
+    ops[:nw, :, 0] = 1
+    ops[:nw, :, 1] = rng.integers(1, 3073, (nw, 64))
+    ops[nw:, :, 0] = 2
+    ops[nw:, :, 1] = rng.permutation(nw * 64).reshape(nw, 64)
+    iheads = np.arange(2 * nw, dtype=np.uint32) % 8
+    recs_g, recs_m, heap = replay_both(opts, ops, iheads)
+    assert (recs_g[:nw, :, 0] != REC_NULL).all()
+    assert (recs_g == recs_m).all()
+    assert_same_state(heap, opts)
+    d = H.Halloc.dump()
+    nsbs = H.Halloc.geometry()["nsbs"]
+    assert ((d[:nsbs] >> np.uint64(16)) & np.uint64(0xFF)).max() > 1
 
 
 def test_replay_large_requests(halloc):

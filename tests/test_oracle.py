@@ -315,3 +315,21 @@ def test_model_freeslabs_with_idle_head_stealing():
         recs, _ = h.replay(ops, iheads)
         assert (recs[:steps, :, 0] != M.REC_NULL).all(), size
     assert M.check_dump(opts, h.dump())[0] == 0
+
+
+def test_model_mixed_sizes_over_eight_heads_share_instead_of_failing():
+    # 16 classes x 8 XCD heads want 128 head slabs from a 96-slab pool: pairs
+    # without a slab share another XCD's head of their class, so nothing fails
+    opts = M.make_opts(nheads=8)
+    h = M.ModelHeap(opts)
+    rng = np.random.default_rng(21)
+    nw = 1024
+    ops = np.zeros((nw, 64, 2), dtype=np.uint32)
+    ops[:, :, 0] = 1
+    ops[:, :, 1] = rng.integers(1, 3073, (nw, 64))
+    recs, _ = h.replay(ops, np.arange(nw, dtype=np.uint32) % 8)
+    assert (recs[:, :, 0] != M.REC_NULL).all()
+    d = h.dump()
+    assert M.check_dump(opts, d)[0] == 0
+    refs = (d[:h.nsbs] >> np.uint64(16)) & np.uint64(0xFF)
+    assert refs.max() > 1                      # some heads are shared
