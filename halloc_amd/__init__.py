@@ -27,7 +27,8 @@ SIZE_TABLE_FIELDS = ("nchunks", "chunk_sz", "chunk_id", "ncib", "sparse", "hash_
 # every symbol include/halloc_amd.h declares
 EXPORTS = ("ha_default_opts", "ha_init_c", "ha_shutdown_c", "ha_extfrag_c", "ha_free_bytes_c",
            "ha_last_error", "ha_geometry_c", "ha_replay", "ha_dump_words", "ha_dump",
-           "ha_run_plain", "ha_malloc_sizes", "ha_free_ptrs", "ha_read_words", "ha_probe_atomics")
+           "ha_run_plain", "ha_malloc_sizes", "ha_free_ptrs", "ha_read_words", "ha_probe_atomics",
+           "ha_run_lists", "ha_run_phase")
 
 
 class HallocError(RuntimeError):
@@ -55,6 +56,24 @@ class _Geometry(ctypes.Structure):
 class _PlainResult(ctypes.Structure):
     _fields_ = [("errors", ctypes.c_uint64), ("nulls", ctypes.c_uint64), ("total_ms", ctypes.c_double),
                 ("wall_ms", ctypes.c_double)]
+
+
+class _ListResult(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_uint64), ("nulls", ctypes.c_uint64), ("errors", ctypes.c_uint64),
+                ("malloc_ms", ctypes.c_double), ("free_ms", ctypes.c_double), ("wall_ms", ctypes.c_double)]
+
+
+class _PhaseArgs(ctypes.Structure):
+    _fields_ = [("nthreads", ctypes.c_int32), ("bs", ctypes.c_int32), ("slots", ctypes.c_int32),
+                ("nphases", ctypes.c_int32), ("max_steps", ctypes.c_int32), ("min_size", ctypes.c_int32),
+                ("max_size", ctypes.c_int32), ("seed", ctypes.c_uint32), ("p_hi", ctypes.c_float),
+                ("p_lo", ctypes.c_float), ("fill_hi", ctypes.c_double), ("fill_lo", ctypes.c_double)]
+
+
+class _PhaseResult(ctypes.Structure):
+    _fields_ = [("mallocs", ctypes.c_uint64), ("frees", ctypes.c_uint64), ("nulls", ctypes.c_uint64),
+                ("errors", ctypes.c_uint64), ("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
+                ("steps", ctypes.c_int32), ("trace_len", ctypes.c_int32), ("phase_steps", ctypes.c_int32 * 8)]
 
 
 @dataclass
@@ -102,6 +121,8 @@ def lib() -> ctypes.CDLL:
         "ha_free_ptrs": (i32, [vp, i32, i32]),
         "ha_read_words": (i32, [vp, i32, vp]),
         "ha_probe_atomics": (i32, [vp]),
+        "ha_run_lists": (i32, [vp, i32, i32, vp, i32, vp, P(_ListResult)]),
+        "ha_run_phase": (i32, [P(_PhaseArgs), P(_PhaseResult), vp, i32]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -204,3 +225,34 @@ class Halloc:
         out = np.zeros(3, dtype=np.float64)
         _check(lib().ha_probe_atomics(out.ctypes.data))
         return out
+
+    @staticmethod
+    def run_lists(sizes: np.ndarray, order=None, bs=256, want_ptrs=False):
+        """sizes: (per, n) uint32, 0 = no request; order: optional permutation of
+        the per*n slots for the frees.  Returns timings, counts and (optionally)
+        the addresses the mallocs returned."""
+        sz = np.ascontiguousarray(sizes, dtype=np.uint32)
+        per, n = sz.shape
+        od = None if order is None else np.ascontiguousarray(order, dtype=np.uint32)
+        ptrs = np.zeros(per * n, dtype=np.uint64) if want_ptrs else None
+        r = _ListResult()
+        _check(lib().ha_run_lists(sz.ctypes.data, n, per, None if od is None else od.ctypes.data, bs,
+                                  None if ptrs is None else ptrs.ctypes.data, ctypes.byref(r)))
+        out = {"requests": int(r.requests), "nulls": int(r.nulls), "errors": int(r.errors),
+               "malloc_ms": r.malloc_ms, "free_ms": r.free_ms, "wall_ms": r.wall_ms}
+        if ptrs is not None:
+            out["ptrs"] = ptrs.reshape(per, n)
+        return out
+
+    @staticmethod
+    def run_phase(nthreads=1 << 20, bs=256, slots=4, nphases=5, max_steps=400, min_size=16, max_size=256,
+                  seed=0, p_hi=0.95, p_lo=0.05, fill_hi=0.91, fill_lo=0.09):
+        a = _PhaseArgs(nthreads, bs, slots, nphases, max_steps, min_size, max_size, seed, p_hi, p_lo,
+                       fill_hi, fill_lo)
+        r = _PhaseResult()
+        cap = nphases * max_steps
+        trace = np.zeros((cap, 3), dtype=np.float64)
+        _check(lib().ha_run_phase(ctypes.byref(a), ctypes.byref(r), trace.ctypes.data, cap))
+        return {"mallocs": int(r.mallocs), "frees": int(r.frees), "nulls": int(r.nulls), "errors": int(r.errors),
+                "kernel_ms": r.kernel_ms, "wall_ms": r.wall_ms, "steps": int(r.steps),
+                "phase_steps": list(r.phase_steps)[:nphases], "trace": trace[:r.trace_len]}

@@ -199,6 +199,115 @@ __global__ void probe_k(u64* buf, u64 mask, int iters, int shape, u64* sink) {
   sink[tid] = acc;
 }
 
+// --------------------------------------------------------------- lists
+// Lane i allocates sizes[j*n + i] for j < per (0 = no request) and writes the
+// size into the block's first word (tst/common/common.h:375-388).
+__global__ void list_malloc_k(const u32* sizes, int n, int per, void** ptrs,
+                              unsigned long long* nulls) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  u32 nul = 0;
+  for (int j = 0; j < per; ++j) {
+    const size_t k = (size_t)j * n + i;
+    const u32 sz = sizes[k];
+    void* p = nullptr;
+    if (sz) {
+      p = hamalloc(sz);
+      if (p) *(u32*)p = sz;
+      else nul++;
+    }
+    ptrs[k] = p;
+  }
+  if (nul) atomicAdd(nulls, (unsigned long long)nul);
+}
+// Thread t frees ptrs[order[j*n + t]] (or ptrs[j*n + t]) after checking the
+// first word (tst/common/common.h:391-399).
+__global__ void list_free_k(void** ptrs, const u32* sizes, const u32* order, int n, int per,
+                            unsigned long long* errs) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  u32 bad = 0;
+  for (int j = 0; j < per; ++j) {
+    const size_t k = order ? (size_t)order[(size_t)j * n + t] : (size_t)j * n + t;
+    void* p = ptrs[k];
+    if (p) {
+      if (*(const u32*)p != sizes[k]) bad++;
+      hafree(p);
+    }
+  }
+  if (bad) atomicAdd(errs, (unsigned long long)bad);
+}
+
+// ---------------------------------------------------------------- phases
+__device__ __forceinline__ u32 hash3(u32 a, u32 b, u32 c) {
+  u32 x = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u ^ (c + 0x165667B1u) * 0xC2B2AE3Du;
+  x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+  return x;
+}
+// One step of the phase workload (tst/perf/phase-throughput/phase-throughput.cu:18-50):
+// each of a lane's slots, if empty, allocates with probability p_alloc a block
+// of 8*U{lo/8..hi/8} bytes and stamps it; if full, frees it with probability
+// p_free after checking the stamp.  stats: mallocs, frees, nulls, errors,
+// bytes allocated, bytes freed.
+__global__ void phase_step_k(void** ptrs, u32* sizes, int n, int slots, u32 seed, u32 step, float p_alloc,
+                             float p_free, u32 lo8, u32 hi8, unsigned long long* stats) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  u32 nm = 0, nf = 0, nn = 0, ne = 0;
+  unsigned long long ba = 0, bf = 0;
+  for (int s = 0; s < slots; ++s) {
+    const size_t k = (size_t)s * n + i;
+    void* p = ptrs[k];
+    const u32 r = hash3(seed, (u32)i * 64u + (u32)s, step);
+    const float u = (float)(r >> 8) * (1.0f / 16777216.0f);
+    if (!p) {
+      if (u < p_alloc) {
+        const u32 sz = 8u * (lo8 + hash3(r, seed, 7u) % (hi8 - lo8 + 1));
+        void* q = hamalloc(sz);
+        if (q) {
+          *(u32*)q = (u32)i ^ (sz << 20);
+          ptrs[k] = q;
+          sizes[k] = sz;
+          nm++;
+          ba += sz;
+        } else {
+          nn++;
+        }
+      }
+    } else if (u < p_free) {
+      const u32 sz = sizes[k];
+      if (*(const u32*)p != ((u32)i ^ (sz << 20))) ne++;
+      hafree(p);
+      ptrs[k] = nullptr;
+      nf++;
+      bf += sz;
+    }
+  }
+  if (nm) atomicAdd(&stats[0], (unsigned long long)nm);
+  if (nf) atomicAdd(&stats[1], (unsigned long long)nf);
+  if (nn) atomicAdd(&stats[2], (unsigned long long)nn);
+  if (ne) atomicAdd(&stats[3], (unsigned long long)ne);
+  if (ba) atomicAdd(&stats[4], ba);
+  if (bf) atomicAdd(&stats[5], bf);
+}
+// frees what is left (free_rest_k, tst/common/common.h:403-412)
+__global__ void phase_rest_k(void** ptrs, u32* sizes, int n, int slots, unsigned long long* stats) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  u32 nf = 0, ne = 0;
+  for (int s = 0; s < slots; ++s) {
+    const size_t k = (size_t)s * n + i;
+    void* p = ptrs[k];
+    if (!p) continue;
+    if (*(const u32*)p != ((u32)i ^ (sizes[k] << 20))) ne++;
+    hafree(p);
+    ptrs[k] = nullptr;
+    nf++;
+  }
+  if (nf) atomicAdd(&stats[1], (unsigned long long)nf);
+  if (ne) atomicAdd(&stats[3], (unsigned long long)ne);
+}
+
 }  // namespace
 
 extern "C" {
@@ -373,6 +482,156 @@ int ha_probe_atomics(double* out) {
   HH(hipGetLastError());
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  return HA_OK;
+}
+
+int ha_run_lists(const uint32_t* sizes, int n, int per, const uint32_t* order, int bs, uint64_t* ptrs_out,
+                 ha_list_result* res) {
+  if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
+  if (!sizes || n < 1 || per < 1 || bs < 1 || bs > 1024 || !res) return hfail(HA_E_INVALID, "bad arguments");
+  const size_t total = (size_t)n * per;
+  if (order) {
+    std::vector<uint8_t> seen(total, 0);
+    for (size_t k = 0; k < total; ++k) {
+      if (order[k] >= total || seen[order[k]]) return hfail(HA_E_INVALID, "order is not a permutation");
+      seen[order[k]] = 1;
+    }
+  }
+  DBuf ds, dp, dord, dc;
+  HH(ds.alloc(total * 4));
+  HH(dp.alloc(total * 8));
+  HH(dc.alloc(16));
+  HH(hipMemcpy(ds.p, sizes, total * 4, hipMemcpyHostToDevice));
+  if (order) {
+    HH(dord.alloc(total * 4));
+    HH(hipMemcpy(dord.p, order, total * 4, hipMemcpyHostToDevice));
+  }
+  HH(hipMemset(dc.p, 0, 16));
+  Stream s;
+  HH(hipStreamCreate(&s.s));
+  hipEvent_t e0, e1, e2;
+  HH(hipEventCreate(&e0));
+  HH(hipEventCreate(&e1));
+  HH(hipEventCreate(&e2));
+  unsigned long long* c = (unsigned long long*)dc.p;
+  const int grid = (n + bs - 1) / bs;
+  HH(hipStreamSynchronize(s.s));
+  const auto w0 = std::chrono::steady_clock::now();
+  HH(hipEventRecord(e0, s.s));
+  list_malloc_k<<<grid, bs, 0, s.s>>>((const u32*)ds.p, n, per, (void**)dp.p, c);
+  HH(hipEventRecord(e1, s.s));
+  HH(hipStreamSynchronize(s.s));
+  const auto w1 = std::chrono::steady_clock::now();
+  if (ptrs_out) HH(hipMemcpy(ptrs_out, dp.p, total * 8, hipMemcpyDeviceToHost));
+  HH(hipStreamSynchronize(s.s));
+  const auto w2 = std::chrono::steady_clock::now();
+  list_free_k<<<grid, bs, 0, s.s>>>((void**)dp.p, (const u32*)ds.p, (const u32*)dord.p, n, per, c + 1);
+  HH(hipEventRecord(e2, s.s));
+  HH(hipStreamSynchronize(s.s));
+  const auto w3 = std::chrono::steady_clock::now();
+  HH(hipGetLastError());
+  float tm = 0, tf = 0;
+  HH(hipEventElapsedTime(&tm, e0, e1));
+  HH(hipEventElapsedTime(&tf, e1, e2));
+  unsigned long long h[2];
+  HH(hipMemcpy(h, dc.p, 16, hipMemcpyDeviceToHost));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
+  res->nulls = h[0];
+  res->errors = h[1];
+  res->malloc_ms = tm;
+  res->free_ms = tf;
+  res->wall_ms = std::chrono::duration<double, std::milli>((w1 - w0) + (w3 - w2)).count();
+  uint64_t nreq = 0;
+  for (size_t k = 0; k < total; ++k) nreq += sizes[k] != 0;
+  res->requests = nreq;
+  return HA_OK;
+}
+
+int ha_run_phase(const ha_phase_args* a, ha_phase_result* res, double* trace, int trace_cap) {
+  const ha::Params& P = ha::host_params();
+  if (!P.initialized) return hfail(HA_E_STATE, "allocator not initialized");
+  if (!a || !res || a->nthreads < 1 || a->bs < 1 || a->bs > 1024 || a->slots < 1 || a->nphases < 1 ||
+      a->min_size < 8 || a->max_size > 3072 || a->min_size > a->max_size || a->max_steps < 1)
+    return hfail(HA_E_INVALID, "bad phase arguments");
+  const int n = a->nthreads, slots = a->slots;
+  const size_t total = (size_t)n * slots;
+  DBuf dp, ds, dc;
+  HH(dp.alloc(total * 8));
+  HH(ds.alloc(total * 4));
+  HH(dc.alloc(8 * 8));
+  HH(hipMemset(dp.p, 0, total * 8));
+  HH(hipMemset(dc.p, 0, 8 * 8));
+  Stream s;
+  HH(hipStreamCreate(&s.s));
+  hipEvent_t e0, e1;
+  HH(hipEventCreate(&e0));
+  HH(hipEventCreate(&e1));
+  unsigned long long* c = (unsigned long long*)dc.p;
+  const int grid = (n + a->bs - 1) / a->bs;
+  const double heap = (double)P.arena_bytes;
+  std::memset(res, 0, sizeof *res);
+  unsigned long long h[6] = {0, 0, 0, 0, 0, 0}, prev[6] = {0, 0, 0, 0, 0, 0};
+  double kern = 0, wall = 0;
+  int ntrace = 0;
+  u32 step = 0;
+  for (int ph = 0; ph < a->nphases; ++ph) {
+    const bool alloc_phase = (ph % 2) == 0;
+    int st = 0;
+    for (; st < a->max_steps; ++st, ++step) {
+      HH(hipStreamSynchronize(s.s));
+      const auto w0 = std::chrono::steady_clock::now();
+      HH(hipEventRecord(e0, s.s));
+      phase_step_k<<<grid, a->bs, 0, s.s>>>((void**)dp.p, (u32*)ds.p, n, slots, a->seed, step,
+                                              alloc_phase ? a->p_hi : a->p_lo, alloc_phase ? a->p_lo : a->p_hi,
+                                              a->min_size / 8, a->max_size / 8, c);
+      HH(hipEventRecord(e1, s.s));
+      HH(hipStreamSynchronize(s.s));
+      const auto w1 = std::chrono::steady_clock::now();
+      float t = 0;
+      HH(hipEventElapsedTime(&t, e0, e1));
+      kern += t;
+      wall += std::chrono::duration<double, std::milli>(w1 - w0).count();
+      HH(hipMemcpy(h, dc.p, sizeof h, hipMemcpyDeviceToHost));
+      const double fill = ((double)h[4] - (double)h[5]) / heap;
+      const unsigned long long dm = h[0] - prev[0], dn = h[2] - prev[2];
+      std::memcpy(prev, h, sizeof h);
+      if (trace && ntrace < trace_cap) {
+        trace[3 * ntrace] = ph;
+        trace[3 * ntrace + 1] = fill;
+        trace[3 * ntrace + 2] = (double)dn;
+        ntrace++;
+      }
+      if (alloc_phase && (fill >= a->fill_hi || dn * 100 > dm + dn)) { ++st; ++step; break; }
+      if (!alloc_phase && fill <= a->fill_lo) { ++st; ++step; break; }
+    }
+    res->phase_steps[ph < 8 ? ph : 7] += st;
+  }
+  // free what is left
+  HH(hipStreamSynchronize(s.s));
+  const auto w0 = std::chrono::steady_clock::now();
+  HH(hipEventRecord(e0, s.s));
+  phase_rest_k<<<grid, a->bs, 0, s.s>>>((void**)dp.p, (u32*)ds.p, n, slots, c);
+  HH(hipEventRecord(e1, s.s));
+  HH(hipStreamSynchronize(s.s));
+  const auto w1 = std::chrono::steady_clock::now();
+  float t = 0;
+  HH(hipEventElapsedTime(&t, e0, e1));
+  kern += t;
+  wall += std::chrono::duration<double, std::milli>(w1 - w0).count();
+  HH(hipGetLastError());
+  HH(hipMemcpy(h, dc.p, sizeof h, hipMemcpyDeviceToHost));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  res->mallocs = h[0];
+  res->frees = h[1];
+  res->nulls = h[2];
+  res->errors = h[3];
+  res->kernel_ms = kern;
+  res->wall_ms = wall;
+  res->steps = (int)step;
+  res->trace_len = ntrace;
   return HA_OK;
 }
 

@@ -104,6 +104,41 @@ int ha_free_ptrs(const uint64_t *ptrs, int n, int bs);
 /* Reads back the first 32-bit word of each block (0 for NULL). */
 int ha_read_words(const uint64_t *ptrs, int n, uint32_t *out);
 
+/* Lists (tst/common/common.h:375-399, check_alloc tst/common/common.cu:468-529):
+   lane i allocates sizes[j*n + i] for j < per (0 = none) and writes the size
+   into the block's first word; then thread t frees ptrs[order[j*n + t]]
+   (order = NULL: the same slots) after checking that word.  Both kernels are
+   timed with HIP events; ptrs_out (may be NULL) receives the device addresses
+   between the two kernels.  errors counts wrong first words at free time. */
+typedef struct ha_list_result {
+  uint64_t requests, nulls, errors;
+  double malloc_ms, free_ms, wall_ms;
+} ha_list_result;
+int ha_run_lists(const uint32_t *sizes, int n, int per, const uint32_t *order, int bs, uint64_t *ptrs_out,
+                 ha_list_result *res);
+
+/* Phase workload (tst/perf/phase-throughput/phase-throughput.cu:18-50): each
+   of nthreads lanes owns `slots` slots.  Per step an empty slot allocates with
+   probability p_alloc a block of 8*U{min/8..max/8} bytes, a full one frees with
+   probability p_free.  Even phases allocate (p_alloc = p_hi, p_free = p_lo)
+   until the requested bytes reach fill_hi of the slab heap, or more than 1% of
+   a step's requests return NULL; odd phases free (swapped probabilities) until
+   fill_lo.  Then everything left is freed.  trace (3 doubles per step: phase,
+   fill, NULLs) may be NULL. */
+typedef struct ha_phase_args {
+  int32_t nthreads, bs, slots, nphases, max_steps, min_size, max_size;
+  uint32_t seed;
+  float p_hi, p_lo;
+  double fill_hi, fill_lo;
+} ha_phase_args;
+typedef struct ha_phase_result {
+  uint64_t mallocs, frees, nulls, errors;
+  double kernel_ms, wall_ms;
+  int32_t steps, trace_len;
+  int32_t phase_steps[8];
+} ha_phase_result;
+int ha_run_phase(const ha_phase_args *a, ha_phase_result *res, double *trace, int trace_cap);
+
 /* Atomic-throughput probe: 64-bit atomic OR over a 1 GiB buffer from 2^20
    lanes with three access shapes; rates in atomics/s:
    out[0] lane-random (one cache line per lane), out[1] wave-run (a wave

@@ -70,3 +70,16 @@ def pressure_program(seed=3, nsteps=6500):
         steps.append(row)
         live.extend(st * 64 + l for l in range(64) if row[l, 0] == 1)
     return np.stack(steps)
+
+
+def sweep_sizes(rng, n, per_max=8, large_frac=0.01, lo=16, hi=3072, large_hi=65536):
+    """(per_max, n) request sizes: lane i makes U{1..per_max} requests, each
+    log-uniform on [lo, hi] bytes, or with probability large_frac uniform on
+    (hi, large_hi]; 0 marks no request."""
+    counts = rng.integers(1, per_max + 1, n)
+    logs = rng.uniform(np.log(lo), np.log(hi), (per_max, n))
+    sizes = np.floor(np.exp(logs)).astype(np.int64).clip(lo, hi)
+    large = rng.random((per_max, n)) < large_frac
+    sizes[large] = rng.integers(hi + 1, large_hi + 1, int(large.sum()))
+    sizes[np.arange(per_max)[:, None] >= counts[None, :]] = 0
+    return sizes.astype(np.uint32)

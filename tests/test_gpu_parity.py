@@ -11,7 +11,8 @@ import pytest
 
 import halloc_amd as H
 from oracle import model as M
-from tests.helpers import BLOCK_SIZES, REC_NULL, assert_disjoint, live_intervals, pressure_program, random_ops
+from tests.helpers import (BLOCK_SIZES, REC_NULL, assert_disjoint, live_intervals, pressure_program,
+                           random_ops, sweep_sizes)
 
 pytestmark = pytest.mark.gpu
 
@@ -220,3 +221,38 @@ def test_extfrag_and_free_bytes(halloc):
     f = H.Halloc.extfrag(True)
     assert 0.0 <= f < 1.0
     H.Halloc.free_ptrs(ptrs)
+
+
+def test_size_sweep_lists_random_free_order(halloc):
+    # size-class sweep shape: U{1..8} requests per lane, log-uniform 16 B..3 KiB
+    # plus 1% in (3 KiB, 64 KiB] on the device heap; frees in random order
+    say("size sweep 2^16 lanes")
+    opts = halloc(H.HallocOpts(memory=1 << 30))
+    rng = np.random.default_rng(3)
+    sizes = sweep_sizes(rng, 1 << 16)
+    order = rng.permutation(sizes.size).astype(np.uint32)
+    r = H.Halloc.run_lists(sizes, order, want_ptrs=True)
+    assert r["nulls"] == 0 and r["errors"] == 0
+    ptrs, sz = r["ptrs"].ravel(), sizes.ravel()
+    live = sz != 0
+    p, s = ptrs[live], sz[live].astype(np.uint64)
+    o = np.argsort(p)
+    assert (p[o][:-1] + s[o][:-1] <= p[o][1:]).all()
+    g = H.Halloc.geometry()
+    d = H.Halloc.dump()
+    assert ((d[:g["nsbs"]] >> np.uint64(32)) == 0).all()
+    assert M.check_dump(model_opts(opts), d)[0] == 0
+
+
+def test_phase_workload_invariants(halloc):
+    say("phase 2^16 lanes x 4 slots")
+    opts = halloc(H.HallocOpts(memory=32 << 20))
+    r = H.Halloc.run_phase(nthreads=1 << 16, slots=4, nphases=3, max_steps=60)
+    say(f"phase steps={r['phase_steps']} nulls={r['nulls']}")
+    assert r["errors"] == 0
+    assert r["mallocs"] == r["frees"]
+    assert r["trace"][:, 1].max() > 0.5
+    g = H.Halloc.geometry()
+    d = H.Halloc.dump()
+    assert ((d[:g["nsbs"]] >> np.uint64(32)) == 0).all()
+    assert M.check_dump(model_opts(opts), d)[0] == 0
