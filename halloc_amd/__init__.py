@@ -28,7 +28,7 @@ SIZE_TABLE_FIELDS = ("nchunks", "chunk_sz", "chunk_id", "ncib", "sparse", "hash_
 EXPORTS = ("ha_default_opts", "ha_init_c", "ha_shutdown_c", "ha_extfrag_c", "ha_free_bytes_c",
            "ha_last_error", "ha_geometry_c", "ha_replay", "ha_dump_words", "ha_dump",
            "ha_run_plain", "ha_malloc_sizes", "ha_free_ptrs", "ha_read_words", "ha_probe_atomics",
-           "ha_run_lists", "ha_run_phase")
+           "ha_run_lists", "ha_run_phase", "ha_replay_batch")
 
 
 class HallocError(RuntimeError):
@@ -123,6 +123,7 @@ def lib() -> ctypes.CDLL:
         "ha_probe_atomics": (i32, [vp]),
         "ha_run_lists": (i32, [vp, i32, i32, vp, i32, vp, P(_ListResult)]),
         "ha_run_phase": (i32, [P(_PhaseArgs), P(_PhaseResult), vp, i32]),
+        "ha_replay_batch": (i32, [vp, i32, i32, vp, vp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -183,6 +184,16 @@ class Halloc:
         ih = None if iheads is None else np.ascontiguousarray(iheads, dtype=np.uint32)
         _check(lib().ha_replay(ops.ctypes.data, nsteps, lanes, None if ih is None else ih.ctypes.data,
                                recs.ctypes.data, xcc.ctypes.data))
+        return recs, xcc
+
+    @staticmethod
+    def replay_batch(ops: np.ndarray):
+        """ops: (nsteps, lanes, 4) uint32 [kind, arg, count, 0]; returns (recs (nsteps, lanes, 16, 4), xcc)."""
+        ops = np.ascontiguousarray(ops, dtype=np.uint32)
+        nsteps, lanes = ops.shape[0], ops.shape[1]
+        recs = np.zeros((nsteps, lanes, 16, 4), dtype=np.uint32)
+        xcc = np.zeros(nsteps, dtype=np.uint32)
+        _check(lib().ha_replay_batch(ops.ctypes.data, nsteps, lanes, recs.ctypes.data, xcc.ctypes.data))
         return recs, xcc
 
     @staticmethod

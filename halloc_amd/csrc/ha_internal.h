@@ -35,6 +35,7 @@ constexpr u32 CHECK_NTRIES = 2;      // src/slab.h:39
 constexpr u32 STEP_FREQ = 64;        // src/slab.h:48
 constexpr u32 THREAD_FREQ = 11;      // src/halloc.cu:40
 constexpr u32 RUN = 64;              // blocks per class-group run (one per wave lane)
+constexpr u32 MAXB = 16;             // most blocks one lane takes in one batched call
 constexpr u32 TICKET_INIT = 0x0B0B0B0Bu;  // src/halloc.cu:416
 constexpr u32 NSETS = 1 + 2 * NSIZES + NCHUNK_IDS;
 constexpr u32 HOT_STRIDE = 32;       // u32 words: one 128 B line per hot word

@@ -398,18 +398,89 @@ __device__ __noinline__ u32 new_sb_for_size(u32 size_id, u32 ihead) {
 }
 
 // ------------------------------------------------------------- malloc path
+// Lane l of a class group takes ticket cv = old + l.  With runs of 64 blocks
+// (ichunk_init, src/halloc.cu:107-122, tmod = 64) the lanes fill the tail of
+// run A = (old >> 6) from slot (old & 63) and the head of run B = A + 1,
+// contiguously: lane l sits at chunk st + l*k of its segment, where st is the
+// segment's (virtual) start.  Each probe is then one atomic per 64-bit word
+// touched, its mask the union of the segment's lanes in the word.
+//
+// probe_run is one probe sequence (sb_alloc_in, src/slab.cuh:453-535) in slab
+// hsb for the lanes in P (wave-uniform), from the blocks of ticket base `old`.
+// It returns the lanes that reserved, records their block lengths in the
+// nibbles, and leaves each reserved lane's chunk in my_ich.  itry advances for
+// the whole group.
+HA_DEV u64 probe_run(const SizeInfo& s, u32 hsb, u32 limit, u32 old, u64 P, u32 me, u32& itry, u32& my_ich) {
+  const u32 k = s.ncib, nchunks = s.nchunks, nwb = nchunks >> 6, nwn = nchunks >> 4;
+  const u32 slot0 = old & (RUN - 1), split = RUN - slot0;
+  const u32 runA = (((old & ~(RUN - 1)) * THREAD_FREQ) * k) % nchunks;
+  const u32 runB = ((((old & ~(RUN - 1)) + RUN) * THREAD_FREQ) * k) % nchunks;
+  const bool inA = me < split;
+  const u64 amask = split >= 64 ? ~0ull : ((1ull << split) - 1);
+  const u64 myseg = inA ? amask : ~amask;
+  u64* bw = ha_p.bits + (u64)hsb * ha_p.bit_words;
+  u64* nw = ha_p.nibs + (u64)hsb * ha_p.nib_words;
+  u64 reserved = 0;
+  u64 off = 0;
+  for (;;) {
+    const u32 pA = (u32)(((u64)runA + off) % nchunks), pB = (u32)(((u64)runB + off) % nchunks);
+    const u32 st = inA ? pA + slot0 * k : pB - split * k;
+    u32 wb, ib;
+    u64 ub;
+    run_word_bits(P & myseg, me, st, k, wb, ub, ib);
+    const u32 wreal = wb >= nwb ? wb - nwb : wb;
+    const bool inP = (P >> me) & 1;
+    u64 oldw = 0;
+    if (inP && ib == me) oldw = a_or(bw + wreal, ub);
+    oldw = shfl64(oldw, ib & 63);
+    u32 c = st + me * k;
+    c = c >= nchunks ? c - nchunks : c;
+    const u64 mym = ((1ull << k) - 1) << (c & 63);
+    const bool ok = inP && !(oldw & mym);
+    const u64 S = ballot(ok);
+    const u64 F = P & ~S;
+    if (F) {
+      // give back the bits the failing lanes set (src/slab.cuh:480-483)
+      u32 wf, iff;
+      u64 uf;
+      run_word_bits(F & myseg, me, st, k, wf, uf, iff);
+      const u64 clr = uf & ~oldw;
+      if (((F >> me) & 1) && iff == me && clr) a_and(bw + wreal, ~clr);
+    }
+    if (S) {
+      // record the block lengths (sb_set_alloc_size, src/slab.cuh:60-70)
+      u32 wn, inn;
+      u64 un;
+      run_word_nibs(S & myseg, me, st, k, (u64)k, wn, un, inn);
+      const u32 wnreal = wn >= nwn ? wn - nwn : wn;
+      if (ok && inn == me) a_or(nw + wnreal, un);
+      if (ok) my_ich = c;
+    }
+    reserved |= S;
+    P = F;
+    if (!P) break;
+    if (itry % CHECK_NTRIES == CHECK_NTRIES - 1) {
+      if (ctr_count(ld64(ctr_of(hsb))) >= limit) break;
+    }
+    const u32 step = ((itry + 1) * STEP_FREQ * k) % s.hash_step;
+    off = (u64)(itry + 1) * step;
+    if (++itry % MAX_NTRIES >= MAX_NTRIES - 1) break;
+  }
+  return reserved;
+}
+
+// gives back one reserved block of this lane: nibble first, then the bits
+HA_DEV void release_block(u32 hsb, u32 ich, u32 k) {
+  a_and(ha_p.nibs + (u64)hsb * ha_p.nib_words + ich / 16, ~(0xFull << (4 * (ich % 16))));
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  a_and(ha_p.bits + (u64)hsb * ha_p.bit_words + ich / 64, ~(((1ull << k) - 1) << (ich % 64)));
+}
+
 // One size class of one wave (hamalloc_small, src/halloc.cu:125-182).  The
 // active lanes are exactly the bitmap g; sid, g and ihead are wave-uniform.
-//
-// Lane l takes ticket cv = old + l, the group adding (highest lane + 1).  With
-// runs of 64 blocks (ichunk_init, src/halloc.cu:107-122, tmod = 64) the lanes
-// fill the tail of run A = (old >> 6) from slot (old & 63) and the head of run
-// B = A + 1, contiguously: lane l sits at chunk st + l*k of its segment, where
-// st is the segment's (virtual) start.  Each probe is then one atomic per
-// 64-bit word touched, its mask the union of the segment's lanes in the word.
 HA_DEV void* malloc_group(u32 sid, u64 g, u32 ihead, u32 me) {
   const SizeInfo& s = ha_p.si[sid];
-  const u32 k = s.ncib, nchunks = s.nchunks, nwb = nchunks >> 6, nwn = nchunks >> 4;
+  const u32 k = s.ncib;
   const u32 lead = (u32)__builtin_ctzll(g);
   const u32 last = 63u - (u32)__builtin_clzll(g);
   u32 hword = rfl(ld32(hot(ha_p.head, ihead, sid)));
@@ -417,74 +488,19 @@ HA_DEV void* malloc_group(u32 sid, u64 g, u32 ihead, u32 me) {
   u32 old = 0;
   if (me == lead) old = a_add32(hot(ha_p.ticket, ihead, sid), last + 1);
   old = rdl(old, lead);
-  const u32 slot0 = old & (RUN - 1), split = RUN - slot0;
-  const u32 runA = (((old & ~(RUN - 1)) * THREAD_FREQ) * k) % nchunks;
-  const u32 runB = ((((old & ~(RUN - 1)) + RUN) * THREAD_FREQ) * k) % nchunks;
-  const bool inA = me < split;
-  const u64 amask = split >= 64 ? ~0ull : ((1ull << split) - 1);
-  const u64 myseg = inA ? amask : ~amask;
   u32 itry = 0;
   u64 want = g;
   void* p = nullptr;
   u32 my_ich = 0;
   while (want) {
     bool need = false;
-    u64 reserved = 0;
     const u32 hsb = hword == SB_NONE ? SB_NONE : (hword & ~HEAD_EXT);
     const u32 limit = head_limit(hword, s);
     if (hsb == SB_NONE) {
       need = true;
     } else {
-      u64* bw = ha_p.bits + (u64)hsb * ha_p.bit_words;
-      u64* nw = ha_p.nibs + (u64)hsb * ha_p.nib_words;
-      u64 P = want;
-      u64 off = 0;
-      // sb_alloc_in (src/slab.cuh:453-535), all probing lanes in lock step
-      for (;;) {
-        const u32 pA = (u32)(((u64)runA + off) % nchunks), pB = (u32)(((u64)runB + off) % nchunks);
-        const u32 st = inA ? pA + slot0 * k : pB - split * k;
-        u32 wb, ib;
-        u64 ub;
-        run_word_bits(P & myseg, me, st, k, wb, ub, ib);
-        const u32 wreal = wb >= nwb ? wb - nwb : wb;
-        const bool inP = (P >> me) & 1;
-        u64 oldw = 0;
-        if (inP && ib == me) oldw = a_or(bw + wreal, ub);
-        oldw = shfl64(oldw, ib & 63);
-        u32 c = st + me * k;
-        c = c >= nchunks ? c - nchunks : c;
-        const u64 mym = ((1ull << k) - 1) << (c & 63);
-        const bool ok = inP && !(oldw & mym);
-        const u64 S = ballot(ok);
-        const u64 F = P & ~S;
-        if (F) {
-          // give back the bits the failing lanes set (src/slab.cuh:480-483)
-          u32 wf, iff;
-          u64 uf;
-          run_word_bits(F & myseg, me, st, k, wf, uf, iff);
-          const u64 clr = uf & ~oldw;
-          if (((F >> me) & 1) && iff == me && clr) a_and(bw + wreal, ~clr);
-        }
-        if (S) {
-          // record the block lengths (sb_set_alloc_size, src/slab.cuh:60-70)
-          u32 wn, inn;
-          u64 un;
-          run_word_nibs(S & myseg, me, st, k, (u64)k, wn, un, inn);
-          const u32 wnreal = wn >= nwn ? wn - nwn : wn;
-          if (ok && inn == me) a_or(nw + wnreal, un);
-          if (ok) my_ich = c;
-        }
-        reserved |= S;
-        P = F;
-        if (!P) break;
-        if (itry % CHECK_NTRIES == CHECK_NTRIES - 1) {
-          if (ctr_count(ld64(ctr_of(hsb))) >= limit) break;
-        }
-        const u32 step = ((itry + 1) * STEP_FREQ * k) % s.hash_step;
-        off = (u64)(itry + 1) * step;
-        if (++itry % MAX_NTRIES >= MAX_NTRIES - 1) break;
-      }
-      if (P) need = true;
+      const u64 reserved = probe_run(s, hsb, limit, old, want, me, itry, my_ich);
+      if (reserved != want) need = true;
       if (reserved) {
         // sb_ctr_inc (src/slab.cuh:122-155): one atomic for the group
         const u32 change = k * (u32)__builtin_popcountll(reserved);
@@ -494,11 +510,7 @@ HA_DEV void* malloc_group(u32 sid, u64 g, u32 ihead, u32 me) {
         const bool mine = (reserved >> me) & 1;
         if (ctr_chunk(oldc) != s.chunk_id) {
           // the slab was freed under us: give the chunks back
-          if (mine) {
-            a_and(nw + my_ich / 16, ~(0xFull << (4 * (my_ich % 16))));
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            a_and(bw + my_ich / 64, ~(((1ull << k) - 1) << (my_ich % 64)));
-          }
+          if (mine) release_block(hsb, my_ich, k);
           if (me == lead) {
             u64 o = a_sub(ctr_of(hsb), (u64)change << CTR_COUNT_SH);
             ctr_transitions(hsb, o, change);
@@ -520,6 +532,73 @@ HA_DEV void* malloc_group(u32 sid, u64 g, u32 ihead, u32 me) {
     }
   }
   return p;
+}
+
+// hamalloc_batch for one class group: lane l wants cnt (<= MAXB) blocks into
+// out[0..cnt).  One ticket for all R rounds (round j based where R back-to-back
+// calls would be), every round probed in the current head, ONE counter atomic
+// for all reserved chunks; unserved lanes and rollbacks go through plain rounds.
+HA_DEV void malloc_batch_group(u32 sid, u64 g, u32 ihead, u32 me, u32 cnt, void** out) {
+  const SizeInfo& s = ha_p.si[sid];
+  const u32 k = s.ncib;
+  const u32 lead = (u32)__builtin_ctzll(g);
+  const u32 L = 64u - (u32)__builtin_clzll(g);
+  u32 R = 0;  // the group's largest cnt
+  for (u32 j = 0; j < MAXB; ++j)
+    if (ballot(cnt > j)) R = j + 1;
+  u32 hword = rfl(ld32(hot(ha_p.head, ihead, sid)));
+  if (hword == SB_NONE) {  // no head yet: get one first
+    u32 nh = 0;
+    if (me == lead) nh = new_sb_for_size(sid, ihead);
+    hword = rdl(nh, lead);
+  }
+  u32 base = 0;
+  if (me == lead) base = a_add32(hot(ha_p.ticket, ihead, sid), R * L);
+  base = rdl(base, lead);
+  for (u32 j = 0; j < cnt; ++j) out[j] = nullptr;
+  bool need = false;
+  const u32 hsb = hword == SB_NONE ? SB_NONE : (hword & ~HEAD_EXT);
+  const u32 limit = head_limit(hword, s);
+  if (hsb == SB_NONE) {
+    need = true;
+  } else {
+    u32 total = 0;
+    char* sbase = ha_p.arena + (u64)hsb * ha_p.sb_sz;
+    for (u32 j = 0; j < R; ++j) {
+      const u64 P = ballot(cnt > j);
+      u32 itry = 0, my_ich = 0;
+      const u64 S = probe_run(s, hsb, limit, base + j * L, P, me, itry, my_ich);
+      if (S != P) need = true;
+      total += (u32)__builtin_popcountll(S);
+      if ((S >> me) & 1) out[j] = sbase + chunk_mul(my_ich, s.chunk_sz);
+    }
+    if (total) {
+      const u32 change = k * total;
+      u64 oldc = 0;
+      if (me == lead) oldc = a_add(ctr_of(hsb), (u64)change << CTR_COUNT_SH);
+      oldc = rdl64(oldc, lead);
+      if (ctr_chunk(oldc) != s.chunk_id) {
+        for (u32 j = 0; j < cnt; ++j) {
+          if (out[j]) release_block(hsb, chunk_div((u32)((char*)out[j] - sbase), s.chunk_sz), k);
+          out[j] = nullptr;
+        }
+        if (me == lead) {
+          u64 o = a_sub(ctr_of(hsb), (u64)change << CTR_COUNT_SH);
+          ctr_transitions(hsb, o, change);
+        }
+        need = true;
+      } else {
+        const u32 oc = ctr_count(oldc);
+        if (oc < limit && oc + change >= limit) need = true;
+      }
+    }
+  }
+  if (need && me == lead) new_sb_for_size(sid, ihead);
+  for (u32 j = 0; j < R; ++j) {
+    const bool pend = cnt > j && out[j] == nullptr;
+    const u64 Pj = ballot(pend);
+    if (pend) out[j] = malloc_group(sid, Pj, ihead, me);
+  }
 }
 
 HA_DEV void* malloc_small(u32 nbytes, u32 ihead) {
@@ -638,6 +717,37 @@ __device__ void* hamalloc(size_t nbytes) {
   // requests above 3072 B go to the HIP device heap (src/halloc.cu:185-195)
   if (nbytes > ha::MAX_BLOCK_SZ) return malloc(nbytes);
   return ha::malloc_small((ha::u32)nbytes, ha::rfl(ha::xcc_id() % ha::ha_p.nheads));
+}
+
+// Batched allocation: lane l gets cnt (<= HA_MAX_BATCH) blocks of nbytes into
+// out[0..cnt) (NULL where memory ran out); returns the number obtained.
+__device__ int hamalloc_batch(size_t nbytes, int cnt, void** out) {
+  if (cnt <= 0) return 0;
+  ha::u32 c = cnt > (int)ha::MAXB ? ha::MAXB : (ha::u32)cnt;
+  int got = 0;
+  if (nbytes > ha::MAX_BLOCK_SZ) {
+    for (ha::u32 j = 0; j < c; ++j) got += (out[j] = malloc(nbytes)) != nullptr;
+    return got;
+  }
+  const ha::u32 me = ha::lane_id();
+  const ha::u32 sid = ha::size_id_from_nbytes((ha::u32)nbytes);
+  const ha::u32 ihead = ha::rfl(ha::xcc_id() % ha::ha_p.nheads);
+  ha::u64 rem = ha::ballot(true);
+  while (rem) {
+    ha::u32 l0 = (ha::u32)__builtin_ctzll(rem);
+    ha::u32 gsid = ha::rdl(sid, l0);
+    ha::u64 g = ha::ballot(sid == gsid) & rem;
+    if ((g >> me) & 1) ha::malloc_batch_group(gsid, g, ihead, me, c, out);
+    rem &= ~g;
+  }
+  for (ha::u32 j = 0; j < c; ++j) got += out[j] != nullptr;
+  return got;
+}
+
+// Batched free: lane l frees ptrs[0..cnt) (NULL entries are skipped), one
+// round per index.
+__device__ void hafree_batch(void* const* ptrs, int cnt) {
+  for (int j = 0; j < cnt; ++j) hafree(ptrs[j]);
 }
 
 // hamalloc with the head set chosen by the caller instead of by the XCD id;

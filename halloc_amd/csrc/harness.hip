@@ -17,6 +17,8 @@
 #include "ha_internal.h"
 
 __device__ void* ha_malloc_on_head(size_t nbytes, unsigned ihead);
+__device__ int hamalloc_batch(size_t nbytes, int cnt, void** out);
+__device__ void hafree_batch(void* const* ptrs, int cnt);
 
 namespace ha {
 const Params& host_params();
@@ -104,6 +106,67 @@ __global__ void __launch_bounds__(64) replay_k(const u32* ops, int nsteps, int l
       hafree(p);
       recs[4 * i] = r0; recs[4 * i + 1] = r1; recs[4 * i + 2] = r2; recs[4 * i + 3] = 0;
       ptrs[i] = nullptr;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ batch replay
+// One wavefront: per step the kind-1 lanes call hamalloc_batch together, then
+// the kind-2 lanes call hafree_batch on an earlier op's blocks.
+__global__ void __launch_bounds__(64) replay_batch_k(const u32* ops, int nsteps, int lanes, u32* recs, u32* xcc,
+                                                     void** ptrs) {
+  const u32 l = threadIdx.x;
+  const ha::Params& P = ha::ha_p;
+  for (int st = 0; st < nsteps; ++st) {
+    const size_t i = (size_t)st * lanes + l;
+    const u32 kind = ops[4 * i], arg = ops[4 * i + 1], cnt = ops[4 * i + 2];
+    if (l == 0) xcc[st] = xcc_id_h();
+    if (kind == 1) {
+      void** out = ptrs + i * ha::MAXB;
+      hamalloc_batch(arg, (int)cnt, out);
+      const u32 c = cnt > ha::MAXB ? ha::MAXB : cnt;
+      for (u32 j = 0; j < c; ++j) {
+        void* p = out[j];
+        u32* r = recs + 4 * (i * ha::MAXB + j);
+        u32 r0 = 0xffffffffu, r1 = 0xffffffffu, r2 = 0xffffffffu;
+        if (p) {
+          u64 off = (u64)((char*)p - P.arena);
+          if (off < P.arena_bytes) {
+            u32 sid = ha::size_id_from_nbytes(arg);
+            r0 = sid; r1 = (u32)(off >> P.sb_sz_sh);
+            r2 = ha::chunk_div((u32)(off & (P.sb_sz - 1)), P.si[sid].chunk_sz);
+          } else {
+            r0 = 16; r1 = 0; r2 = 0;
+          }
+        }
+        r[0] = r0; r[1] = r1; r[2] = r2; r[3] = 0;
+      }
+    }
+    __syncthreads();
+    if (kind == 2) {
+      void** fp = ptrs + (size_t)arg * ha::MAXB;
+      const u32 c = ops[4 * (size_t)arg + 2] > ha::MAXB ? ha::MAXB : ops[4 * (size_t)arg + 2];
+      for (u32 j = 0; j < c; ++j) {
+        void* p = fp[j];
+        if (!p) continue;
+        u64 off = (u64)((char*)p - P.arena);
+        u32* r = recs + 4 * (i * ha::MAXB + j);
+        r[0] = 0; r[1] = 0xffffffffu; r[2] = 0xffffffffu; r[3] = 0;
+        if (off < P.arena_bytes) {
+          u32 sb = (u32)(off >> P.sb_sz_sh);
+          u32 cs = __hip_atomic_load(&P.desc[sb].chunk_sz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (cs) {
+            u32 ich = ha::chunk_div((u32)(off & (P.sb_sz - 1)), cs);
+            u64 w = __hip_atomic_load(P.nibs + (u64)sb * P.nib_words + ich / 16, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+            u32 nch = (u32)((w >> (4 * (ich % 16))) & 0xF);
+            if (nch) { r[0] = nch; r[1] = sb; r[2] = ich; }
+          }
+        }
+      }
+      hafree_batch(fp, (int)c);
+      for (u32 j = 0; j < c; ++j) fp[j] = nullptr;
     }
     __syncthreads();
   }
@@ -632,6 +695,32 @@ int ha_run_phase(const ha_phase_args* a, ha_phase_result* res, double* trace, in
   res->wall_ms = wall;
   res->steps = (int)step;
   res->trace_len = ntrace;
+  return HA_OK;
+}
+
+int ha_replay_batch(const uint32_t* ops, int nsteps, int lanes, uint32_t* recs, uint32_t* xcc) {
+  if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
+  if (!ops || !recs || !xcc || nsteps < 0 || lanes < 1 || lanes > 64) return hfail(HA_E_INVALID, "bad replay arguments");
+  const size_t nops = (size_t)nsteps * lanes;
+  for (size_t i = 0; i < nops; ++i) {
+    const uint32_t kind = ops[4 * i], arg = ops[4 * i + 1];
+    if (kind > 2) return hfail(HA_E_INVALID, "bad op kind");
+    if (kind == 2 && (arg >= nops || arg / lanes >= i / lanes || ops[4 * (size_t)arg] != 1))
+      return hfail(HA_E_INVALID, "a batch free must name a batch malloc of an earlier step");
+  }
+  DBuf dops, drecs, dxcc, dptrs;
+  HH(dops.alloc(nops * 16));
+  HH(drecs.alloc(nops * ha::MAXB * 16));
+  HH(dxcc.alloc((size_t)nsteps * 4));
+  HH(dptrs.alloc(nops * ha::MAXB * 8));
+  HH(hipMemcpy(dops.p, ops, nops * 16, hipMemcpyHostToDevice));
+  HH(hipMemset(dptrs.p, 0, nops * ha::MAXB * 8));
+  HH(hipMemset(drecs.p, 0, nops * ha::MAXB * 16));
+  replay_batch_k<<<1, lanes>>>((const u32*)dops.p, nsteps, lanes, (u32*)drecs.p, (u32*)dxcc.p, (void**)dptrs.p);
+  HH(hipGetLastError());
+  HH(hipDeviceSynchronize());
+  HH(hipMemcpy(recs, drecs.p, nops * ha::MAXB * 16, hipMemcpyDeviceToHost));
+  HH(hipMemcpy(xcc, dxcc.p, (size_t)nsteps * 4, hipMemcpyDeviceToHost));
   return HA_OK;
 }
 

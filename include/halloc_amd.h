@@ -75,6 +75,13 @@ int ha_geometry_c(ha_geometry *g);
 int ha_replay(const uint32_t *ops, int nsteps, int lanes, const uint32_t *iheads, uint32_t *recs,
               uint32_t *xcc);
 
+/* Batched replay on ONE wavefront (include/halloc_batch.h): ops hold 4 u32
+   per (step, lane): kind (0 none, 1 hamalloc_batch, 2 hafree_batch), arg
+   (bytes, or the index of an earlier kind-1 op whose blocks are freed), count
+   (kind 1), 0.  recs hold 4 u32 per (op, j < 16), as for ha_replay.  xcc[step]
+   receives the wave's XCD id. */
+int ha_replay_batch(const uint32_t *ops, int nsteps, int lanes, uint32_t *recs, uint32_t *xcc);
+
 /* Canonical state dump (layout documented in oracle/model_capi.cpp). */
 int64_t ha_dump_words(void);
 int ha_dump(uint64_t *out, int64_t nwords);

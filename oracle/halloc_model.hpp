@@ -86,6 +86,7 @@ constexpr u32 THREAD_MOD_SH = 2;    // src/halloc.cu:44
 constexpr u32 TICKET_INIT = 0x0B0B0B0Bu;  // cudaMemset(counters_g, 11), src/halloc.cu:416
 constexpr u32 RUN = 64;             // wave-run length (hash_mode 1)
 constexpr u32 WAVE = 64;
+constexpr u32 MAXB = 16;            // most blocks one lane takes in one batched call
 // a head word is a slab id, plus HEAD_EXT when the slab came from the busy set:
 // such a head is filled up to (busy + nchunks) / 2 instead of busy
 constexpr u32 HEAD_EXT = 0x80000000u;
@@ -544,6 +545,54 @@ class Heap {
   // ---------------------------------------------------------- malloc (wave)
   static u32 rank_of(u64 g, u32 l) { return (u32)__builtin_popcountll(g & ((1ull << l) - 1)); }
 
+  // One probe sequence (sb_alloc_in, src/slab.cuh:453-535) in slab hsb for the
+  // lanes P with initial chunks ich0; ich receives each lane's last position.
+  // Returns the lanes that reserved; itry advances for the whole group.
+  u64 probe_in(const SizeInfo& s, u32 hsb, u32 limit, u64 P, const u32* ich0, u32* ich, u32& itry) {
+    const u64 mask1 = (1ull << s.ncib) - 1;
+    std::atomic<u64>* bw = &bits[(size_t)hsb * bit_words];
+    u64 probing = P, reserved = 0;
+    for (u32 l = 0; l < WAVE; ++l) if (probing >> l & 1) ich[l] = ich0[l];
+    for (;;) {
+      u64 oldw[WAVE];
+      for (u32 l = 0; l < WAVE; ++l) if (probing >> l & 1) {
+        u64 m = mask1 << (ich[l] % 64);
+        oldw[l] = bw[ich[l] / 64].fetch_or(m);
+      }
+      u64 now = 0;
+      for (u32 l = 0; l < WAVE; ++l) if (probing >> l & 1) {
+        u64 m = mask1 << (ich[l] % 64);
+        if (!(oldw[l] & m)) now |= 1ull << l;
+        else if (~oldw[l] & m) bw[ich[l] / 64].fetch_and(~(m & ~oldw[l]));
+      }
+      reserved |= now;
+      probing &= ~now;
+      if (!probing) break;
+      if (itry % CHECK_NTRIES == CHECK_NTRIES - 1) {
+        if (ctr_count(ctr[hsb].load()) >= limit) break;
+      }
+      for (u32 l = 0; l < WAVE; ++l) if (probing >> l & 1) ich[l] = probe_next(ich0[l], itry, s);
+      if (++itry % MAX_NTRIES >= MAX_NTRIES - 1) break;
+    }
+    return reserved;
+  }
+
+  // record blocks that passed the counter check
+  void commit_blocks(u32 size_id, u32 hsb, u64 reserved, const u32* ich, u64* out, Rec* rec) {
+    const SizeInfo& s = si[size_id];
+    std::atomic<u64>* nw = &nibs[(size_t)hsb * nib_words];
+    for (u32 l = 0; l < WAVE; ++l) if (reserved >> l & 1) {
+      out[l] = (u64)hsb * sb_sz + chunk_mul(ich[l], s.chunk_sz) + 1;
+      nw[ich[l] / 16].fetch_or((u64)s.ncib << (4 * (ich[l] % 16)));
+      if (rec) rec[l] = Rec{size_id, hsb, ich[l], 0};
+    }
+  }
+  void release_bits(const SizeInfo& s, u32 hsb, u64 reserved, const u32* ich) {
+    const u64 mask1 = (1ull << s.ncib) - 1;
+    std::atomic<u64>* bw = &bits[(size_t)hsb * bit_words];
+    for (u32 l = 0; l < WAVE; ++l) if (reserved >> l & 1) bw[ich[l] / 64].fetch_and(~(mask1 << (ich[l] % 64)));
+  }
+
   // one size-class group of one wave (hamalloc_small, src/halloc.cu:125-182)
   void malloc_group(u32 ihead, u32 size_id, u64 g, u64* out, Rec* rec) {
     const SizeInfo& s = si[size_id];
@@ -561,61 +610,27 @@ class Heap {
     }
     u32 itry = 0;
     u64 want = g;
-    const u64 mask1 = (1ull << s.ncib) - 1;
     while (want) {
       bool need = false;
-      u64 reserved = 0;
       const u32 hsb = hword == SB_NONE ? SB_NONE : (hword & ~HEAD_EXT);
       const u32 limit = head_limit(hword, s);
-      // sb_alloc_in (src/slab.cuh:453-535)
       if (hsb == SB_NONE) {
         need = true;
       } else {
-        std::atomic<u64>* bw = &bits[(size_t)hsb * bit_words];
-        u64 probing = want;
-        for (u32 l = 0; l < WAVE; ++l) if (probing >> l & 1) ich[l] = ich0[l];
-        for (;;) {
-          u64 oldw[WAVE];
-          for (u32 l = 0; l < WAVE; ++l) if (probing >> l & 1) {
-            u64 m = mask1 << (ich[l] % 64);
-            oldw[l] = bw[ich[l] / 64].fetch_or(m);
-          }
-          u64 now = 0;
-          for (u32 l = 0; l < WAVE; ++l) if (probing >> l & 1) {
-            u64 m = mask1 << (ich[l] % 64);
-            if (!(oldw[l] & m)) now |= 1ull << l;
-            else if (~oldw[l] & m) bw[ich[l] / 64].fetch_and(~(m & ~oldw[l]));
-          }
-          reserved |= now;
-          probing &= ~now;
-          if (!probing) break;
-          if (itry % CHECK_NTRIES == CHECK_NTRIES - 1) {
-            if (ctr_count(ctr[hsb].load()) >= limit) break;
-          }
-          for (u32 l = 0; l < WAVE; ++l) if (probing >> l & 1) ich[l] = probe_next(ich0[l], itry, s);
-          if (++itry % MAX_NTRIES >= MAX_NTRIES - 1) break;
-        }
-        if (probing) need = true;
+        u64 reserved = probe_in(s, hsb, limit, want, ich0, ich, itry);
+        if (reserved != want) need = true;
         if (reserved) {
           // sb_ctr_inc (src/slab.cuh:122-155), one group per (slab, ncib)
-          u32 n = (u32)__builtin_popcountll(reserved), change = s.ncib * n;
+          u32 change = s.ncib * (u32)__builtin_popcountll(reserved);
           u64 old = ctr[hsb].fetch_add((u64)change << CTR_COUNT_SH);
           if (ctr_chunk(old) != s.chunk_id) {
-            for (u32 l = 0; l < WAVE; ++l) if (reserved >> l & 1)
-              bw[ich[l] / 64].fetch_and(~(mask1 << (ich[l] % 64)));
+            release_bits(s, hsb, reserved, ich);
             ctr_dec(hsb, change);
-            reserved = 0;
             need = true;
           } else {
             u32 oc = ctr_count(old);
             if (oc < limit && oc + change >= limit) need = true;
-            std::atomic<u64>* nw = &nibs[(size_t)hsb * nib_words];
-            for (u32 l = 0; l < WAVE; ++l) if (reserved >> l & 1) {
-              u64 off = (u64)hsb * sb_sz + chunk_mul(ich[l], s.chunk_sz);
-              out[l] = off + 1;
-              nw[ich[l] / 16].fetch_or((u64)s.ncib << (4 * (ich[l] % 16)));
-              if (rec) rec[l] = Rec{size_id, hsb, ich[l], 0};
-            }
+            commit_blocks(size_id, hsb, reserved, ich, out, rec);
             want &= ~reserved;
           }
         }
@@ -631,6 +646,91 @@ class Heap {
           want = 0;
         }
       }
+    }
+  }
+
+  // hamalloc_batch for one size-class group: lane l wants cnt[l] (<= MAXB)
+  // blocks.  The group takes ONE ticket for all R rounds (round j is based at
+  // old + j*(highest lane + 1), where R back-to-back calls would be), probes every
+  // round in the current head, and adds all its chunks to the slab counter in
+  // ONE atomic.  Lanes a round leaves unserved, and everything after a rollback,
+  // go through plain per-call rounds.  out/rec: [round][lane].
+  void malloc_batch_group(u32 ihead, u32 size_id, u64 g, const u32* cnt, u64* out, Rec* rec) {
+    const SizeInfo& s = si[size_id];
+    size_t hi = (size_t)ihead * NSIZES + size_id;
+    u32 hword = head[hi].load();
+    if (hword == SB_NONE) hword = new_sb_for_size(size_id, ihead);  // no head yet: get one first
+    u32 R = 0;
+    for (u32 l = 0; l < WAVE; ++l) if ((g >> l & 1) && cnt[l] > R) R = cnt[l];
+    const u32 L = 64 - (u32)__builtin_clzll(g);
+    const u32 base = ticket[hi].fetch_add(R * L);
+    u64 pend[MAXB] = {0}, resv[MAXB] = {0};
+    static thread_local u32 ich_r[MAXB][WAVE];
+    for (u32 j = 0; j < R; ++j)
+      for (u32 l = 0; l < WAVE; ++l) if ((g >> l & 1) && cnt[l] > j) pend[j] |= 1ull << l;
+    bool need = false;
+    const u32 hsb = hword == SB_NONE ? SB_NONE : (hword & ~HEAD_EXT);
+    const u32 limit = head_limit(hword, s);
+    if (hsb == SB_NONE) {
+      need = true;
+    } else {
+      u32 total = 0;
+      for (u32 j = 0; j < R; ++j) {
+        u32 ich0[WAVE] = {0};
+        for (u32 l = 0; l < WAVE; ++l) if (pend[j] >> l & 1) ich0[l] = ichunk_init(base + j * L + l, s);
+        u32 itry = 0;
+        resv[j] = probe_in(s, hsb, limit, pend[j], ich0, ich_r[j], itry);
+        if (resv[j] != pend[j]) need = true;
+        total += (u32)__builtin_popcountll(resv[j]);
+      }
+      if (total) {
+        u32 change = s.ncib * total;
+        u64 old = ctr[hsb].fetch_add((u64)change << CTR_COUNT_SH);
+        if (ctr_chunk(old) != s.chunk_id) {
+          for (u32 j = 0; j < R; ++j) release_bits(s, hsb, resv[j], ich_r[j]);
+          ctr_dec(hsb, change);
+          need = true;
+        } else {
+          u32 oc = ctr_count(old);
+          if (oc < limit && oc + change >= limit) need = true;
+          for (u32 j = 0; j < R; ++j) {
+            commit_blocks(size_id, hsb, resv[j], ich_r[j], out + j * WAVE, rec ? rec + j * WAVE : nullptr);
+            pend[j] &= ~resv[j];
+          }
+        }
+      }
+    }
+    if (need) new_sb_for_size(size_id, ihead);
+    for (u32 j = 0; j < R; ++j)
+      if (pend[j]) malloc_group(ihead, size_id, pend[j], out + j * WAVE, rec ? rec + j * WAVE : nullptr);
+  }
+
+  // hamalloc_batch for the active lanes of one wave: lane l asks for cnt[l]
+  // blocks of nbytes[l] (classes above 3072 B go to the device heap one by one)
+  void wave_malloc_batch(u32 ihead, u64 active, const u32* nbytes, const u32* cnt, u64* out, Rec* rec) {
+    u32 sid[WAVE] = {0};
+    u64 small = 0;
+    for (u32 l = 0; l < WAVE; ++l) if (active >> l & 1) {
+      u32 c = cnt[l] > MAXB ? MAXB : cnt[l];
+      if (!c) continue;
+      if (nbytes[l] > MAX_BLOCK_SZ) {
+        for (u32 j = 0; j < c; ++j) {
+          out[j * WAVE + l] = (1ull << 63) | (large_ctr.fetch_add(1) + 1);
+          if (rec) rec[j * WAVE + l] = Rec{REC_LARGE, 0, 0, 0};
+        }
+      } else {
+        sid[l] = size_id_from_nbytes(nbytes[l]);
+        small |= 1ull << l;
+      }
+    }
+    u32 c2[WAVE];
+    for (u32 l = 0; l < WAVE; ++l) c2[l] = cnt[l] > MAXB ? MAXB : cnt[l];
+    while (small) {
+      u32 l0 = (u32)__builtin_ctzll(small);
+      u64 g = 0;
+      for (u32 l = 0; l < WAVE; ++l) if ((small >> l & 1) && sid[l] == sid[l0]) g |= 1ull << l;
+      small &= ~g;
+      malloc_batch_group(ihead % nheads, sid[l0], g, c2, out, rec);
     }
   }
 

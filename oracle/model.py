@@ -24,6 +24,7 @@ SZ_NONE = 0xFFFFFFFF
 REC_LARGE = 16
 REC_NULL = 0xFFFFFFFF
 NSETS = 1 + 2 * NSIZES + 4
+MAXB = 16
 SIZE_TABLE_FIELDS = ("nchunks", "chunk_sz", "chunk_id", "ncib", "sparse", "hash_step",
                      "roomy", "busy", "block_sz", "nblocks", "tmod", "chunk_bytes")
 
@@ -76,6 +77,7 @@ def lib() -> ctypes.CDLL:
             ("hm_destroy", None, [vp]),
             ("hm_nsbs", u32, [vp]),
             ("hm_replay", ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp]),
+            ("hm_replay_batch", ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp]),
             ("hm_dump_words", u64, [vp]),
             ("hm_dump", ctypes.c_int, [vp, vp]),
             ("hm_check_dump", i64, [P(HmOpts), vp, vp]),
@@ -128,6 +130,19 @@ class ModelHeap:
         if rc != 0:
             raise ValueError(f"hm_replay failed: {rc}")
         return recs, ptrs
+
+    def replay_batch(self, ops: np.ndarray, iheads: np.ndarray | None = None):
+        """ops: (nsteps, lanes, 4) uint32 [kind, arg, count, 0]; returns recs (nsteps, lanes, MAXB, 4)."""
+        ops = np.ascontiguousarray(ops, dtype=np.uint32)
+        nsteps, lanes = ops.shape[0], ops.shape[1]
+        recs = np.zeros((nsteps, lanes, MAXB, 4), dtype=np.uint32)
+        ptrs = np.zeros((nsteps, lanes, MAXB), dtype=np.uint64)
+        ih = None if iheads is None else np.ascontiguousarray(iheads, dtype=np.uint32)
+        rc = lib().hm_replay_batch(self.h, ops.ctypes.data, None if ih is None else ih.ctypes.data,
+                                   nsteps, lanes, recs.ctypes.data, ptrs.ctypes.data)
+        if rc != 0:
+            raise ValueError(f"hm_replay_batch failed: {rc}")
+        return recs
 
     def dump(self) -> np.ndarray:
         n = lib().hm_dump_words(self.h)

@@ -103,6 +103,64 @@ int hm_replay(void* hv, const hm_op_t* ops, const uint32_t* iheads, int nsteps, 
   return 0;
 }
 
+// Batched replay: ops are 4 u32 per (step, lane): kind (0 none, 1 hamalloc_batch,
+// 2 hafree_batch), arg (bytes, or the index of an earlier kind-1 op whose
+// blocks are freed), count (blocks, kind 1), pad.  recs: 4 u32 per (op, j)
+// for j < MAXB; ptrs: arena offset + 1 per (op, j).
+int hm_replay_batch(void* hv, const uint32_t* ops, const uint32_t* iheads, int nsteps, int lanes,
+                    uint32_t* recs, uint64_t* ptrs) {
+  Heap* h = (Heap*)hv;
+  if (lanes < 1 || lanes > 64) return -1;
+  for (int st = 0; st < nsteps; ++st) {
+    const uint32_t* row = ops + (size_t)st * lanes * 4;
+    u64 am = 0;
+    u32 nbytes[WAVE] = {0}, cnt[WAVE] = {0}, fref[WAVE] = {0};
+    u64 fm = 0;
+    for (int l = 0; l < lanes; ++l) {
+      if (row[4 * l] == 1) { am |= 1ull << l; nbytes[l] = row[4 * l + 1]; cnt[l] = row[4 * l + 2]; }
+      else if (row[4 * l] == 2) {
+        if ((size_t)row[4 * l + 1] >= (size_t)st * lanes) return -2;
+        fm |= 1ull << l; fref[l] = row[4 * l + 1];
+      }
+    }
+    if (am) {
+      static thread_local u64 outp[MAXB * WAVE];
+      static thread_local Rec mrec[MAXB * WAVE];
+      std::memset(outp, 0, sizeof outp);
+      std::memset(mrec, 0, sizeof mrec);
+      h->wave_malloc_batch(iheads ? iheads[st] : 0, am, nbytes, cnt, outp, mrec);
+      for (int l = 0; l < lanes; ++l) if (am >> l & 1) {
+        size_t i = (size_t)st * lanes + l;
+        u32 c = cnt[l] > MAXB ? MAXB : cnt[l];
+        for (u32 j = 0; j < c; ++j) {
+          ptrs[i * MAXB + j] = outp[j * WAVE + l];
+          std::memcpy(recs + 4 * (i * MAXB + j), &mrec[j * WAVE + l], 16);
+        }
+      }
+    }
+    if (fm) {
+      for (u32 j = 0; j < MAXB; ++j) {
+        u64 m = 0;
+        u64 fp[WAVE] = {0};
+        for (int l = 0; l < lanes; ++l) if (fm >> l & 1) {
+          u64 p = ptrs[(size_t)fref[l] * MAXB + j];
+          if (p) { m |= 1ull << l; fp[l] = p; }
+        }
+        if (!m) continue;
+        Rec frec[WAVE];
+        std::memset(frec, 0, sizeof frec);
+        h->wave_free(m, fp, frec);
+        for (int l = 0; l < lanes; ++l) if (m >> l & 1) {
+          size_t i = (size_t)st * lanes + l;
+          ptrs[(size_t)fref[l] * MAXB + j] = 0;
+          std::memcpy(recs + 4 * (i * MAXB + j), &frec[l], 16);
+        }
+      }
+    }
+  }
+  return 0;
+}
+
 // Canonical state dump (u64 words), shared with the GPU's ha_debug_dump:
 //   [nsbs]            slab counters
 //   [2*nsbs]          descriptors: size | chunk<<32, chunk_sz | is_head<<32

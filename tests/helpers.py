@@ -83,3 +83,20 @@ def sweep_sizes(rng, n, per_max=8, large_frac=0.01, lo=16, hi=3072, large_hi=655
     sizes[large] = rng.integers(hi + 1, large_hi + 1, int(large.sum()))
     sizes[np.arange(per_max)[:, None] >= counts[None, :]] = 0
     return sizes.astype(np.uint32)
+
+
+def random_batch_ops(rng, nsteps, lanes, sizes, max_count=16, p_free=0.4):
+    """Random batched replay program: [kind, arg, count, 0] per (step, lane)."""
+    ops = np.zeros((nsteps, lanes, 4), dtype=np.uint32)
+    live = []
+    for st in range(nsteps):
+        new = []
+        for l in range(lanes):
+            u = rng.random()
+            if live and u < p_free:
+                ops[st, l] = (2, live.pop(int(rng.integers(len(live)))), 0, 0)
+            elif u < 0.9:
+                ops[st, l] = (1, int(rng.choice(sizes)), int(rng.integers(1, max_count + 1)), 0)
+                new.append(st * lanes + l)
+        live.extend(new)
+    return ops

@@ -12,7 +12,7 @@ import pytest
 import halloc_amd as H
 from oracle import model as M
 from tests.helpers import (BLOCK_SIZES, REC_NULL, assert_disjoint, live_intervals, pressure_program,
-                           random_ops, sweep_sizes)
+                           random_batch_ops, random_ops, sweep_sizes)
 
 pytestmark = pytest.mark.gpu
 
@@ -256,3 +256,17 @@ def test_phase_workload_invariants(halloc):
     d = H.Halloc.dump()
     assert ((d[:g["nsbs"]] >> np.uint64(32)) == 0).all()
     assert M.check_dump(model_opts(opts), d)[0] == 0
+
+
+@pytest.mark.parametrize("nheads", [1, 8])
+def test_replay_batch(halloc, nheads):
+    # hamalloc_batch / hafree_batch on one wave, bit-exact against the model
+    say(f"replay batch nheads={nheads}")
+    opts = halloc(H.HallocOpts(memory=256 << 20, sb_sz_sh=21, nheads=nheads))
+    rng = np.random.default_rng(40 + nheads)
+    ops = random_batch_ops(rng, 120, 64, BLOCK_SIZES + [20, 100, 700])
+    recs_g, xcc = H.Halloc.replay_batch(ops)
+    heap = M.ModelHeap(model_opts(opts))
+    recs_m = heap.replay_batch(ops, xcc % nheads)
+    assert (recs_g == recs_m).all()
+    assert_same_state(heap, opts)

@@ -9,7 +9,8 @@ import numpy as np
 import pytest
 
 from oracle import model as M
-from tests.helpers import BLOCK_SIZES, assert_disjoint, live_intervals, pressure_program, random_ops
+from tests.helpers import (BLOCK_SIZES, assert_disjoint, live_intervals, pressure_program, random_batch_ops,
+                           random_ops)
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -333,3 +334,37 @@ def test_model_mixed_sizes_over_eight_heads_share_instead_of_failing():
     assert M.check_dump(opts, d)[0] == 0
     refs = (d[:h.nsbs] >> np.uint64(16)) & np.uint64(0xFF)
     assert refs.max() > 1                      # some heads are shared
+
+
+def test_batch_equals_back_to_back_calls():
+    # on a fresh heap, one batched call of R blocks per lane yields exactly the
+    # blocks of R back-to-back plain calls (same tickets, same runs)
+    for size, R in ((16, 4), (48, 7), (256, 16), (3000, 3)):
+        opts = M.make_opts(nheads=1)
+        a = M.ModelHeap(opts)
+        bops = np.zeros((1, 64, 4), dtype=np.uint32)
+        bops[0, :, :3] = (1, size, R)
+        brecs = a.replay_batch(bops)
+        b = M.ModelHeap(opts)
+        pops = np.zeros((R, 64, 2), dtype=np.uint32)
+        pops[:, :, 0] = 1
+        pops[:, :, 1] = size
+        precs, _ = b.replay(pops)
+        assert (brecs[0, :, :R, :3].transpose(1, 0, 2) == precs[:, :, :3]).all(), (size, R)
+        # the batch touched the ticket once, the counter once: same end state
+        assert M.check_dump(opts, a.dump())[0] == 0
+
+
+@pytest.mark.parametrize("nheads", [1, 8])
+def test_model_batch_programs_keep_invariants(nheads):
+    opts = M.make_opts(memory=128 << 20, sb_sz_sh=21, nheads=nheads)
+    h = M.ModelHeap(opts)
+    rng = np.random.default_rng(9)
+    ops = random_batch_ops(rng, 150, 64, BLOCK_SIZES + [20, 100, 700])
+    recs = h.replay_batch(ops, np.arange(150, dtype=np.uint32) % 8)
+    assert M.check_dump(opts, h.dump())[0] == 0
+    m = ops[..., 0] == 1
+    cnt = ops[..., 2]
+    got = recs[..., 0]
+    for st, l in zip(*np.nonzero(m)):
+        assert (got[st, l, :cnt[st, l]] != M.REC_NULL).all()
