@@ -28,7 +28,7 @@ SIZE_TABLE_FIELDS = ("nchunks", "chunk_sz", "chunk_id", "ncib", "sparse", "hash_
 EXPORTS = ("ha_default_opts", "ha_init_c", "ha_shutdown_c", "ha_extfrag_c", "ha_free_bytes_c",
            "ha_last_error", "ha_geometry_c", "ha_replay", "ha_dump_words", "ha_dump",
            "ha_run_plain", "ha_malloc_sizes", "ha_free_ptrs", "ha_read_words", "ha_probe_atomics",
-           "ha_run_lists", "ha_run_phase", "ha_replay_batch")
+           "ha_run_lists", "ha_run_phase", "ha_replay_batch", "ha_run_graphs")
 
 
 class HallocError(RuntimeError):
@@ -74,6 +74,19 @@ class _PhaseResult(ctypes.Structure):
     _fields_ = [("mallocs", ctypes.c_uint64), ("frees", ctypes.c_uint64), ("nulls", ctypes.c_uint64),
                 ("errors", ctypes.c_uint64), ("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
                 ("steps", ctypes.c_int32), ("trace_len", ctypes.c_int32), ("phase_steps", ctypes.c_int32 * 8)]
+
+
+class _GraphArgs(ctypes.Structure):
+    _fields_ = [("njobs", ctypes.c_int32), ("bs", ctypes.c_int32), ("batch", ctypes.c_int32),
+                ("waves", ctypes.c_int32), ("plain_chunk_nodes", ctypes.c_int32), ("repeats", ctypes.c_int32),
+                ("run_plain", ctypes.c_int32), ("run_batch", ctypes.c_int32), ("seed", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32)]
+
+
+class _GraphResult(ctypes.Structure):
+    _fields_ = [("nodes", ctypes.c_uint64), ("nulls", ctypes.c_uint64), ("errors", ctypes.c_uint64),
+                ("plain_ms", ctypes.c_double), ("batch_ms", ctypes.c_double),
+                ("plain_launches", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 @dataclass
@@ -124,6 +137,7 @@ def lib() -> ctypes.CDLL:
         "ha_run_lists": (i32, [vp, i32, i32, vp, i32, vp, P(_ListResult)]),
         "ha_run_phase": (i32, [P(_PhaseArgs), P(_PhaseResult), vp, i32]),
         "ha_replay_batch": (i32, [vp, i32, i32, vp, vp]),
+        "ha_run_graphs": (i32, [P(_GraphArgs), vp, P(_GraphResult)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -267,3 +281,15 @@ class Halloc:
         return {"mallocs": int(r.mallocs), "frees": int(r.frees), "nulls": int(r.nulls), "errors": int(r.errors),
                 "kernel_ms": r.kernel_ms, "wall_ms": r.wall_ms, "steps": int(r.steps),
                 "phase_steps": list(r.phase_steps)[:nphases], "trace": trace[:r.trace_len]}
+
+    @staticmethod
+    def run_graphs(nnodes, batch=8, waves=1024, bs=256, plain_chunk_nodes=1 << 23, repeats=1,
+                   run_plain=True, run_batch=True, seed=0):
+        """Random-graph builds, plain per-call form and batched persistent form."""
+        nn = np.ascontiguousarray(nnodes, dtype=np.uint32)
+        a = _GraphArgs(len(nn), bs, batch, waves, plain_chunk_nodes, repeats, int(run_plain), int(run_batch),
+                       seed, 0)
+        r = _GraphResult()
+        _check(lib().ha_run_graphs(ctypes.byref(a), nn.ctypes.data, ctypes.byref(r)))
+        return {"nodes": int(r.nodes), "nulls": int(r.nulls), "errors": int(r.errors),
+                "plain_ms": r.plain_ms, "batch_ms": r.batch_ms, "plain_launches": int(r.plain_launches)}

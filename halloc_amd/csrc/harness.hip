@@ -371,6 +371,171 @@ __global__ void phase_rest_k(void** ptrs, u32* sizes, int n, int slots, unsigned
   if (ne) atomicAdd(&stats[3], (unsigned long long)ne);
 }
 
+// ---------------------------------------------------------------- graphs
+// Random-graph builds (the shape of samples/random-graph/random-graph.cu:79-84,
+// 167-181, with frees): node v of job j has a 16 B header {v, degree, adjacency}
+// and an int32 adjacency array of Poisson(8) targets in [0, n_j); degree and
+// targets come from a counter-based hash, so every form builds the same graphs.
+struct NodeHdr {
+  u32 node, deg;
+  u32* adj;
+};
+__device__ __forceinline__ u32 gh(u32 seed, u32 job, u32 node, u32 e) {
+  return hash3(seed ^ (job * 0x9E3779B9u), node, e);
+}
+__device__ __forceinline__ u32 poisson8(u32 seed, u32 job, u32 node) {
+  // Knuth's product-of-uniforms method, lambda = 8
+  const float L = 3.3546262790251185e-4f;  // exp(-8)
+  float p = 1.0f;
+  u32 k = 0;
+  for (;;) {
+    p *= ((float)(gh(seed, job, node, 1000u + k) >> 8) + 0.5f) * (1.0f / 16777216.0f);
+    if (p <= L || k >= 63) break;
+    ++k;
+  }
+  return k;
+}
+__device__ __forceinline__ u32 adj_bytes(u32 deg) { return deg ? 4u * deg : 0u; }
+__device__ __forceinline__ void fill_node(NodeHdr* h, u32* a, u32 seed, u32 job, u32 node, u32 deg, u32 n) {
+  for (u32 e = 0; e < deg; ++e) a[e] = gh(seed, job, node, e) % n;
+  h->node = node;
+  h->deg = deg;
+  h->adj = a;
+}
+__device__ __forceinline__ u32 check_node(const NodeHdr* h, u32 seed, u32 job, u32 node, u32 n) {
+  const u32 deg = poisson8(seed, job, node);
+  if (h->node != node || h->deg != deg || (deg && !h->adj)) return 1;
+  u32 bad = 0;
+  for (u32 e = 0; e < deg; ++e) bad += h->adj[e] != gh(seed, job, node, e) % n;
+  return bad ? 1 : 0;
+}
+__device__ __forceinline__ u32 job_of(const u64* offs, u32 njobs, u64 gi) {
+  u32 lo = 0, hi = njobs;  // offs[lo] <= gi < offs[hi]
+  while (hi - lo > 1) {
+    u32 mid = (lo + hi) / 2;
+    if (offs[mid] <= gi) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// plain per-call form: one thread per node of the job range [j0, j1)
+__global__ void graph_plain_build_k(const u64* offs, const u32* nnodes, u32 njobs, u32 seed, u64 g0, u64 g1,
+                                    NodeHdr** table, unsigned long long* stats) {
+  const u64 gi = g0 + blockIdx.x * (u64)blockDim.x + threadIdx.x;
+  if (gi >= g1) return;
+  const u32 job = job_of(offs, njobs, gi), node = (u32)(gi - offs[job]), n = nnodes[job];
+  const u32 deg = poisson8(seed, job, node);
+  NodeHdr* h = (NodeHdr*)hamalloc(sizeof(NodeHdr));
+  u32* a = (u32*)hamalloc(adj_bytes(deg));
+  if (h && a) {
+    fill_node(h, a, seed, job, node, deg, n);
+  } else {
+    atomicAdd(&stats[0], 1ull);
+    if (a) hafree(a);
+    if (h) hafree(h);
+    h = nullptr;
+  }
+  table[gi] = h;
+}
+__global__ void graph_plain_free_k(const u64* offs, const u32* nnodes, u32 njobs, u32 seed, u64 g0, u64 g1,
+                                   NodeHdr** table, unsigned long long* stats) {
+  const u64 gi = g0 + blockIdx.x * (u64)blockDim.x + threadIdx.x;
+  if (gi >= g1) return;
+  NodeHdr* h = table[gi];
+  if (!h) return;
+  const u32 job = job_of(offs, njobs, gi), node = (u32)(gi - offs[job]);
+  if (check_node(h, seed, job, node, nnodes[job])) atomicAdd(&stats[1], 1ull);
+  hafree(h->adj);
+  hafree(h);
+}
+
+__global__ void reset_counter_k(u32* c) { *c = 0; }
+
+// batched persistent form: waves take jobs from a queue; lane l of a wave
+// handles nodes chunk + l*B + j (j < B).  Headers: one hamalloc_batch per chunk.
+// Adjacency arrays: one hamalloc_batch per size class present in the wave,
+// frees in the same class order.
+template <int B>
+__global__ void __launch_bounds__(256) graph_batch_k(const u64* offs, const u32* nnodes, u32 njobs, u32 seed,
+                                                     NodeHdr** table, u32* job_ctr,
+                                                     unsigned long long* stats) {
+  const u32 me = threadIdx.x & 63;
+  u32 nulls = 0, bad = 0;
+  for (;;) {
+    u32 job = 0;
+    if (me == 0) job = atomicAdd(job_ctr, 1u);
+    job = (u32)__builtin_amdgcn_readfirstlane((int)job);
+    if (job >= njobs) break;
+    const u32 n = nnodes[job];
+    NodeHdr** tab = table + offs[job];
+    for (int pass = 0; pass < 2; ++pass) {  // 0: build, 1: check and free
+      for (u32 chunk = 0; chunk < n; chunk += 64u * B) {
+        const u32 first = chunk + me * B;
+        const u32 cnt = first >= n ? 0u : (n - first < (u32)B ? n - first : (u32)B);
+        u32 deg[B], cls[B];
+        u64 classes = 0;
+        for (int j = 0; j < B; ++j) {
+          deg[j] = (u32)j < cnt ? poisson8(seed, job, first + j) : 0u;
+          cls[j] = ha::size_id_from_nbytes(adj_bytes(deg[j]));
+          if ((u32)j < cnt) classes |= 1ull << cls[j];
+        }
+        u64 all = 0;  // classes present in the wave (bits 0..15)
+        for (u32 c = 0; c < ha::NSIZES; ++c)
+          if (__ballot((classes >> c) & 1)) all |= 1ull << c;
+        if (pass == 0) {
+          void* hdr[B];
+          void* adj[B];
+          void* tmp[B];
+          for (int j = 0; j < B; ++j) { hdr[j] = nullptr; adj[j] = nullptr; }
+          if (cnt) hamalloc_batch(sizeof(NodeHdr), (int)cnt, hdr);
+          for (u64 cs = all; cs; cs &= cs - 1) {
+            const u32 c = (u32)__builtin_ctzll(cs);
+            u32 nc = 0;
+            for (int j = 0; j < B; ++j) nc += ((u32)j < cnt && cls[j] == c);
+            if (nc) {
+              hamalloc_batch(ha::ha_p.si[c].block_sz, (int)nc, tmp);
+              u32 t = 0;
+              for (int j = 0; j < B; ++j)
+                if ((u32)j < cnt && cls[j] == c) adj[j] = tmp[t++];
+            }
+          }
+          for (int j = 0; j < B; ++j) {
+            if ((u32)j >= cnt) continue;
+            NodeHdr* h = (NodeHdr*)hdr[j];
+            u32* a = (u32*)adj[j];
+            if (h && a) {
+              fill_node(h, a, seed, job, first + j, deg[j], n);
+            } else {
+              nulls++;
+              if (a) hafree(a);
+              if (h) hafree(h);
+              h = nullptr;
+            }
+            tab[first + j] = h;
+          }
+        } else {
+          void* hdr[B];
+          void* tmp[B];
+          for (int j = 0; j < B; ++j) {
+            hdr[j] = (u32)j < cnt ? (void*)tab[first + j] : nullptr;
+            if (hdr[j]) bad += check_node((NodeHdr*)hdr[j], seed, job, first + j, n);
+          }
+          for (u64 cs = all; cs; cs &= cs - 1) {
+            const u32 c = (u32)__builtin_ctzll(cs);
+            u32 nc = 0;
+            for (int j = 0; j < B; ++j)
+              if ((u32)j < cnt && cls[j] == c) tmp[nc++] = hdr[j] ? (void*)((NodeHdr*)hdr[j])->adj : nullptr;
+            if (nc) hafree_batch(tmp, (int)nc);
+          }
+          if (cnt) hafree_batch(hdr, (int)cnt);
+        }
+      }
+    }
+  }
+  if (nulls) atomicAdd(&stats[0], (unsigned long long)nulls);
+  if (bad) atomicAdd(&stats[1], (unsigned long long)bad);
+}
+
 }  // namespace
 
 extern "C" {
@@ -721,6 +886,110 @@ int ha_replay_batch(const uint32_t* ops, int nsteps, int lanes, uint32_t* recs, 
   HH(hipDeviceSynchronize());
   HH(hipMemcpy(recs, drecs.p, nops * ha::MAXB * 16, hipMemcpyDeviceToHost));
   HH(hipMemcpy(xcc, dxcc.p, (size_t)nsteps * 4, hipMemcpyDeviceToHost));
+  return HA_OK;
+}
+
+int ha_run_graphs(const ha_graph_args* a, const uint32_t* nnodes, ha_graph_result* res) {
+  if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
+  if (!a || !nnodes || !res || a->njobs < 1 || a->bs < 64 || a->bs > 1024 || a->bs % 64 ||
+      (a->batch != 4 && a->batch != 8 && a->batch != 16) || a->waves < 1 || a->plain_chunk_nodes < 1 ||
+      a->repeats < 1)
+    return hfail(HA_E_INVALID, "bad graph arguments");
+  const u32 njobs = (u32)a->njobs;
+  std::vector<u64> offs(njobs + 1, 0);
+  for (u32 j = 0; j < njobs; ++j) {
+    if (nnodes[j] < 1) return hfail(HA_E_INVALID, "a job needs at least one node");
+    offs[j + 1] = offs[j] + nnodes[j];
+  }
+  const u64 total = offs[njobs];
+  DBuf doffs, dn, dtab, dstats, dctr;
+  HH(doffs.alloc((njobs + 1) * 8));
+  HH(dn.alloc(njobs * 4));
+  HH(dtab.alloc(total * 8));
+  HH(dstats.alloc(16));
+  HH(dctr.alloc(4));
+  HH(hipMemcpy(doffs.p, offs.data(), (njobs + 1) * 8, hipMemcpyHostToDevice));
+  HH(hipMemcpy(dn.p, nnodes, njobs * 4, hipMemcpyHostToDevice));
+  HH(hipMemset(dstats.p, 0, 16));
+  Stream s;
+  HH(hipStreamCreate(&s.s));
+  hipEvent_t e0, e1;
+  HH(hipEventCreate(&e0));
+  HH(hipEventCreate(&e1));
+  unsigned long long* st = (unsigned long long*)dstats.p;
+  std::memset(res, 0, sizeof *res);
+  res->nodes = total;
+  // plain per-call form: chunks of jobs whose nodes fit plain_chunk_nodes
+  std::vector<std::pair<u64, u64>> chunks;
+  for (u32 j = 0; j < njobs;) {
+    u32 k = j;
+    while (k < njobs && (k == j || offs[k + 1] - offs[j] <= (u64)a->plain_chunk_nodes)) ++k;
+    chunks.push_back({offs[j], offs[k]});
+    j = k;
+  }
+  auto run_plain = [&]() -> int {
+    for (auto& c : chunks) {
+      const u64 nn = c.second - c.first;
+      const unsigned grid = (unsigned)((nn + a->bs - 1) / a->bs);
+      graph_plain_build_k<<<grid, a->bs, 0, s.s>>>((const u64*)doffs.p, (const u32*)dn.p, njobs, a->seed, c.first,
+                                                    c.second, (NodeHdr**)dtab.p, st);
+      graph_plain_free_k<<<grid, a->bs, 0, s.s>>>((const u64*)doffs.p, (const u32*)dn.p, njobs, a->seed, c.first,
+                                                   c.second, (NodeHdr**)dtab.p, st);
+    }
+    return (int)hipGetLastError();
+  };
+  // batched persistent form, captured in a graph with the queue reset
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const unsigned bgrid = (unsigned)((a->waves * 64 + a->bs - 1) / a->bs);
+  HH(hipStreamBeginCapture(s.s, hipStreamCaptureModeThreadLocal));
+  reset_counter_k<<<1, 1, 0, s.s>>>((u32*)dctr.p);
+  if (a->batch == 4)
+    graph_batch_k<4><<<bgrid, a->bs, 0, s.s>>>((const u64*)doffs.p, (const u32*)dn.p, njobs, a->seed,
+                                                (NodeHdr**)dtab.p, (u32*)dctr.p, st);
+  else if (a->batch == 8)
+    graph_batch_k<8><<<bgrid, a->bs, 0, s.s>>>((const u64*)doffs.p, (const u32*)dn.p, njobs, a->seed,
+                                                (NodeHdr**)dtab.p, (u32*)dctr.p, st);
+  else
+    graph_batch_k<16><<<bgrid, a->bs, 0, s.s>>>((const u64*)doffs.p, (const u32*)dn.p, njobs, a->seed,
+                                                 (NodeHdr**)dtab.p, (u32*)dctr.p, st);
+  HH(hipStreamEndCapture(s.s, &graph));
+  HH(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  double best_plain = 1e30, best_batch = 1e30;
+  for (int r = 0; r < a->repeats; ++r) {
+    if (a->run_plain) {
+      HH(hipStreamSynchronize(s.s));
+      HH(hipEventRecord(e0, s.s));
+      if (run_plain()) return hfail(HA_E_HIP, "plain graph launch failed");
+      HH(hipEventRecord(e1, s.s));
+      HH(hipStreamSynchronize(s.s));
+      float t = 0;
+      HH(hipEventElapsedTime(&t, e0, e1));
+      best_plain = t < best_plain ? t : best_plain;
+    }
+    if (a->run_batch) {
+      HH(hipStreamSynchronize(s.s));
+      HH(hipEventRecord(e0, s.s));
+      HH(hipGraphLaunch(exec, s.s));
+      HH(hipEventRecord(e1, s.s));
+      HH(hipStreamSynchronize(s.s));
+      float t = 0;
+      HH(hipEventElapsedTime(&t, e0, e1));
+      best_batch = t < best_batch ? t : best_batch;
+    }
+  }
+  HH(hipGetLastError());
+  unsigned long long h[2];
+  HH(hipMemcpy(h, dstats.p, 16, hipMemcpyDeviceToHost));
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  res->nulls = h[0];
+  res->errors = h[1];
+  res->plain_ms = a->run_plain ? best_plain : 0;
+  res->batch_ms = a->run_batch ? best_batch : 0;
+  res->plain_launches = (int32_t)(2 * chunks.size());
   return HA_OK;
 }
 

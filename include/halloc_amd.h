@@ -146,6 +146,25 @@ typedef struct ha_phase_result {
 } ha_phase_result;
 int ha_run_phase(const ha_phase_args *a, ha_phase_result *res, double *trace, int trace_cap);
 
+/* Random-graph builds (batching workload): job j builds a graph of nnodes[j]
+   nodes, each with a 16 B header and an int32 adjacency array of Poisson(8)
+   random targets, checks it and frees it.  Plain per-call form: one thread per
+   node calling hamalloc / hafree, jobs launched in chunks of at most
+   plain_chunk_nodes nodes.  Batched form: one launch of `waves` wavefronts
+   taking jobs from a queue, lane l handling `batch` nodes (4, 8 or 16) through
+   hamalloc_batch / hafree_batch, captured in a HIP graph.  Each form runs
+   `repeats` times and reports its best time; errors counts wrong graphs. */
+typedef struct ha_graph_args {
+  int32_t njobs, bs, batch, waves, plain_chunk_nodes, repeats, run_plain, run_batch;
+  uint32_t seed, pad;
+} ha_graph_args;
+typedef struct ha_graph_result {
+  uint64_t nodes, nulls, errors;
+  double plain_ms, batch_ms;
+  int32_t plain_launches, pad;
+} ha_graph_result;
+int ha_run_graphs(const ha_graph_args *a, const uint32_t *nnodes, ha_graph_result *res);
+
 /* Atomic-throughput probe: 64-bit atomic OR over a 1 GiB buffer from 2^20
    lanes with three access shapes; rates in atomics/s:
    out[0] lane-random (one cache line per lane), out[1] wave-run (a wave

@@ -100,3 +100,8 @@ def random_batch_ops(rng, nsteps, lanes, sizes, max_count=16, p_free=0.4):
                 new.append(st * lanes + l)
         live.extend(new)
     return ops
+
+
+def graph_sizes(rng, njobs, lo=1024, hi=65536):
+    """Node counts of the random-graph jobs: log-uniform on [lo, hi]."""
+    return np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), njobs))).astype(np.uint32)

@@ -12,7 +12,7 @@ import pytest
 import halloc_amd as H
 from oracle import model as M
 from tests.helpers import (BLOCK_SIZES, REC_NULL, assert_disjoint, live_intervals, pressure_program,
-                           random_batch_ops, random_ops, sweep_sizes)
+                           graph_sizes, random_batch_ops, random_ops, sweep_sizes)
 
 pytestmark = pytest.mark.gpu
 
@@ -270,3 +270,18 @@ def test_replay_batch(halloc, nheads):
     recs_m = heap.replay_batch(ops, xcc % nheads)
     assert (recs_g == recs_m).all()
     assert_same_state(heap, opts)
+
+
+@pytest.mark.parametrize("batch", [4, 16])
+def test_graph_builds_plain_and_batched(halloc, batch):
+    # both forms build, check and free the same graphs; the heap ends empty
+    say(f"graphs batch={batch}")
+    opts = halloc(H.HallocOpts(memory=1 << 30))
+    sizes = graph_sizes(np.random.default_rng(0), 64)
+    r = H.Halloc.run_graphs(sizes, batch=batch, waves=256, plain_chunk_nodes=1 << 20)
+    say(f"graphs nodes={r['nodes']} plain={r['plain_ms']:.2f} ms batch={r['batch_ms']:.2f} ms")
+    assert r["nulls"] == 0 and r["errors"] == 0
+    g = H.Halloc.geometry()
+    d = H.Halloc.dump()
+    assert ((d[:g["nsbs"]] >> np.uint64(32)) == 0).all()
+    assert M.check_dump(model_opts(opts), d)[0] == 0
