@@ -28,7 +28,7 @@ SIZE_TABLE_FIELDS = ("nchunks", "chunk_sz", "chunk_id", "ncib", "sparse", "hash_
 EXPORTS = ("ha_default_opts", "ha_init_c", "ha_shutdown_c", "ha_extfrag_c", "ha_free_bytes_c",
            "ha_last_error", "ha_geometry_c", "ha_replay", "ha_dump_words", "ha_dump",
            "ha_run_plain", "ha_malloc_sizes", "ha_free_ptrs", "ha_read_words", "ha_probe_atomics",
-           "ha_run_lists", "ha_run_phase", "ha_replay_batch", "ha_run_graphs")
+           "ha_run_lists", "ha_run_phase", "ha_replay_batch", "ha_run_graphs", "ha_run_serve")
 
 
 class HallocError(RuntimeError):
@@ -89,6 +89,20 @@ class _GraphResult(ctypes.Structure):
                 ("plain_launches", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
+class _ServeIO(ctypes.Structure):
+    _fields_ = [("arrival_ns", ctypes.c_void_p), ("length", ctypes.c_void_p), ("tag", ctypes.c_void_p),
+                ("checksum", ctypes.c_void_p), ("produced", ctypes.c_void_p), ("first_ns", ctypes.c_void_p),
+                ("done_ns", ctypes.c_void_p), ("lat_ns", ctypes.c_void_p), ("nreq", ctypes.c_int32),
+                ("slots", ctypes.c_int32), ("sample_every", ctypes.c_int32), ("lat_cap", ctypes.c_int32),
+                ("watermark", ctypes.c_uint64)]
+
+
+class _ServeResult(ctypes.Structure):
+    _fields_ = [("kernel_ms", ctypes.c_double), ("pages", ctypes.c_uint64), ("nulls", ctypes.c_uint64),
+                ("stalls", ctypes.c_uint64), ("samples", ctypes.c_uint64), ("completed", ctypes.c_uint32),
+                ("admitted", ctypes.c_uint32), ("clock_khz", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+
+
 @dataclass
 class HallocOpts:
     """Fields and defaults of halloc_opts_t (src/halloc.h:15-34), plus nheads."""
@@ -138,6 +152,7 @@ def lib() -> ctypes.CDLL:
         "ha_run_phase": (i32, [P(_PhaseArgs), P(_PhaseResult), vp, i32]),
         "ha_replay_batch": (i32, [vp, i32, i32, vp, vp]),
         "ha_run_graphs": (i32, [P(_GraphArgs), vp, P(_GraphResult)]),
+        "ha_run_serve": (i32, [P(_ServeIO), P(_ServeResult)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -293,3 +308,30 @@ class Halloc:
         _check(lib().ha_run_graphs(ctypes.byref(a), nn.ctypes.data, ctypes.byref(r)))
         return {"nodes": int(r.nodes), "nulls": int(r.nulls), "errors": int(r.errors),
                 "plain_ms": r.plain_ms, "batch_ms": r.batch_ms, "plain_launches": int(r.plain_launches)}
+
+    @staticmethod
+    def run_serve(arrival_ns, length, tag, slots=65536, watermark=64 << 20, sample_every=8, lat_cap=1 << 22):
+        """Serving loop with continuous batching (one persistent launch).
+        Returns per-request checksum, produced tokens, first-token and
+        completion times (ns after the start), and sampled page-malloc
+        latencies (ns)."""
+        arr = np.ascontiguousarray(arrival_ns, dtype=np.uint64)
+        ln = np.ascontiguousarray(length, dtype=np.uint32)
+        tg = np.ascontiguousarray(tag, dtype=np.uint32)
+        n = len(arr)
+        if len(ln) != n or len(tg) != n:
+            raise ValueError("arrival_ns, length and tag must have one entry per request")
+        cs = np.zeros(n, dtype=np.uint32)
+        prod = np.zeros(n, dtype=np.uint32)
+        first = np.zeros(n, dtype=np.uint64)
+        done = np.zeros(n, dtype=np.uint64)
+        lat = np.zeros(max(lat_cap, 1), dtype=np.uint32)
+        io = _ServeIO(arr.ctypes.data, ln.ctypes.data, tg.ctypes.data, cs.ctypes.data, prod.ctypes.data,
+                      first.ctypes.data, done.ctypes.data, lat.ctypes.data, n, slots, sample_every, lat_cap,
+                      watermark)
+        r = _ServeResult()
+        _check(lib().ha_run_serve(ctypes.byref(io), ctypes.byref(r)))
+        return {"kernel_ms": r.kernel_ms, "pages": int(r.pages), "nulls": int(r.nulls), "stalls": int(r.stalls),
+                "completed": int(r.completed), "admitted": int(r.admitted), "clock_khz": int(r.clock_khz),
+                "checksum": cs, "produced": prod, "first_ns": first, "done_ns": done,
+                "lat_ns": lat[:int(r.samples)].copy()}

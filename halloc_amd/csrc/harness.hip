@@ -536,6 +536,205 @@ __global__ void __launch_bounds__(256) graph_batch_k(const u64* offs, const u32*
   if (bad) atomicAdd(&stats[1], (unsigned long long)bad);
 }
 
+// ---------------------------------------------------------------- serving
+// Device-resident request loop with continuous batching.  Each lane is one
+// sequence slot.  Request r arrives arrival[r] wall-clock ticks after the
+// start (sorted).  Idle lanes of a wave admit the next arrived requests, in
+// arrival order, while the heap's free bytes stay >= the watermark; wave 0
+// refreshes that figure from the slab counters every 8 iterations (the
+// per-slab sum of find_free_mem_k, src/statistics.cuh:34-48).  Per iteration a
+// live sequence either decodes one token -- 16 bytes {tag, t, h, ~h},
+// h = tok_hash(tag, t), 16 tokens to a 256 B page from hamalloc -- or, once
+// it has its length, drains one page: reads its tokens back, adds the valid
+// hashes to the checksum and frees the page.  A page malloc that returns NULL
+// ends the sequence early; `produced` says how many tokens it holds.
+// The loop ends when no request is left to admit and every lane of the wave
+// is idle, or at the deadline.
+struct SrvState {
+  u64 t0;        // wall clock at the start
+  u64 free_est;  // heap free bytes from the slab counters
+  u32 next_req, done, pad0, pad1;
+  unsigned long long nulls, samples, pages, stalls;
+};
+
+struct SrvArgs {
+  const u64* arrival;  // ticks after the start, sorted
+  const u32* length;   // tokens, 1..32768
+  const u32* tag;
+  u32* checksum;
+  u32* produced;
+  u64* first_tk;       // tick of the first token
+  u64* done_tk;        // tick of completion
+  u32* lat_tk;         // sampled page-malloc latencies
+  void** page_tab;     // max_pages page pointers per slot
+  SrvState* st;
+  u64 watermark, deadline_tk;
+  u32 nreq, max_pages, sample_every, lat_cap;
+};
+
+constexpr u32 SRV_NONE = 0xffffffffu;
+constexpr u32 SRV_TOK_BYTES = 16, SRV_PAGE_BYTES = 256, SRV_TOK_PER_PAGE = SRV_PAGE_BYTES / SRV_TOK_BYTES;
+
+__device__ __forceinline__ u32 tok_hash(u32 tag, u32 t) { return hash3(tag, t, 0x5E7A11C3u); }
+
+__global__ void srv_start_k(SrvState* st) { st->t0 = wall_clock64(); }
+
+__device__ __forceinline__ u32 wave_sum(u32 v) {
+  for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+__global__ void __launch_bounds__(256) srv_loop_k(SrvArgs a) {
+  const ha::Params& P = ha::ha_p;
+  SrvState* st = a.st;
+  const u32 me = threadIdx.x & 63;
+  const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool refresher = slot < 64;
+  const u64 lt = me ? (~0ull >> (64 - me)) : 0ull;  // lanes below me
+  void** pages = a.page_tab + (u64)slot * a.max_pages;
+  const u64 t0 = __hip_atomic_load(&st->t0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  u32 req = SRV_NONE, tag = 0, len = 0, tok = 0, cs = 0, np = 0, drain = 0;
+  bool draining = false;
+  char* cur = nullptr;
+  u32 my_pages = 0, my_nulls = 0, my_stalls = 0;
+  for (u32 iter = 0;; ++iter) {
+    const u64 now = wall_clock64() - t0;
+    if (refresher && (iter & 7) == 0) {
+      unsigned long long used = 0;
+      for (u32 b = me; b < P.nsbs; b += 64) {
+        const u32 csz = __hip_atomic_load(&P.desc[b].chunk_sz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const u32 c = __hip_atomic_load(P.ctr + (u64)b * ha::CTR_STRIDE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        used += (unsigned long long)ha::ctr_count(c) * csz;
+      }
+      for (int o = 32; o; o >>= 1) used += __shfl_xor(used, o);
+      if (me == 0) {
+        const u64 fe = used < P.arena_bytes ? P.arena_bytes - used : 0;
+        __hip_atomic_store(&st->free_est, fe, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    // ---- admission, in arrival order
+    const u64 I = __ballot(req == SRV_NONE);
+    const u32 nr = (u32)__builtin_amdgcn_readfirstlane(
+        (int)__hip_atomic_load(&st->next_req, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (I == ~0ull && nr >= a.nreq) break;
+    if (now > a.deadline_tk) break;
+    bool admitted = false;
+    if (I && nr < a.nreq) {
+      const u64 fe0 = __hip_atomic_load(&st->free_est, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const u64 fe = ((u64)(u32)__builtin_amdgcn_readfirstlane((int)(u32)(fe0 >> 32)) << 32) |
+                     (u32)__builtin_amdgcn_readfirstlane((int)(u32)fe0);
+      if (fe >= a.watermark) {
+        const u32 cand = nr + me;
+        const bool arrived = cand < a.nreq && a.arrival[cand] <= now;
+        u32 k = (u32)__builtin_popcountll(__ballot(arrived));  // a prefix: arrivals are sorted
+        const u32 nidle = (u32)__builtin_popcountll(I);
+        k = k < nidle ? k : nidle;
+        if (k) {
+          u32 got = 0;
+          if (me == 0) got = atomicCAS(&st->next_req, nr, nr + k) == nr;
+          got = (u32)__builtin_amdgcn_readfirstlane((int)got);
+          if (got) {
+            const u32 rank = (u32)__builtin_popcountll(I & lt);
+            if (req == SRV_NONE && rank < k) {
+              req = nr + rank;
+              tag = a.tag[req];
+              len = a.length[req];
+              tok = cs = np = drain = 0;
+              draining = false;
+              a.first_tk[req] = now;
+            }
+            admitted = true;
+          }
+        }
+      } else {
+        ++my_stalls;
+      }
+    }
+    // ---- one step per live sequence
+    bool sampled = false, finished = false;
+    u32 lat = 0;
+    if (req != SRV_NONE && !draining) {
+      if (tok % SRV_TOK_PER_PAGE == 0) {
+        const u32 pi = tok / SRV_TOK_PER_PAGE;
+        const bool timed = a.sample_every && (req + pi) % a.sample_every == 0;
+        const u64 c0 = wall_clock64();
+        char* pg = (char*)hamalloc(SRV_PAGE_BYTES);
+        __asm__ volatile("" : : "v"(pg));  // the page pointer is ready before the second clock read
+        const u64 c1 = wall_clock64();
+        if (pg) {
+          pages[pi] = pg;
+          cur = pg;
+          ++my_pages;
+          if (timed) {
+            sampled = true;
+            lat = (u32)(c1 - c0);
+          }
+        } else {
+          ++my_nulls;
+          len = tok;  // end the sequence with what it has
+        }
+      }
+      if (tok < len) {
+        const u32 h = tok_hash(tag, tok);
+        *(uint4*)(cur + (tok % SRV_TOK_PER_PAGE) * SRV_TOK_BYTES) = make_uint4(tag, tok, h, ~h);
+        ++tok;
+      }
+      if (tok >= len) {
+        draining = true;
+        np = (tok + SRV_TOK_PER_PAGE - 1) / SRV_TOK_PER_PAGE;
+        drain = 0;
+      }
+    } else if (req != SRV_NONE) {
+      if (drain < np) {
+        const uint4* pg = (const uint4*)pages[drain];
+        const u32 t_lo = drain * SRV_TOK_PER_PAGE;
+        const u32 n = tok - t_lo < SRV_TOK_PER_PAGE ? tok - t_lo : SRV_TOK_PER_PAGE;
+        uint4 v[SRV_TOK_PER_PAGE];
+#pragma unroll
+        for (u32 j = 0; j < SRV_TOK_PER_PAGE; ++j)
+          if (j < n) v[j] = pg[j];
+#pragma unroll
+        for (u32 j = 0; j < SRV_TOK_PER_PAGE; ++j) {
+          if (j < n) {
+            const u32 t = t_lo + j, h = tok_hash(tag, t);
+            if (v[j].x == tag && v[j].y == t && v[j].z == h && v[j].w == ~h) cs += h;
+          }
+        }
+        hafree((void*)pg);
+        ++drain;
+      }
+      if (drain >= np) {
+        a.checksum[req] = cs;
+        a.produced[req] = tok;
+        a.done_tk[req] = now;
+        req = SRV_NONE;
+        finished = true;
+      }
+    }
+    // ---- wave-aggregated counters
+    const u64 F = __ballot(finished);
+    if (F && me == 0) atomicAdd(&st->done, (u32)__builtin_popcountll(F));
+    const u64 S = __ballot(sampled);
+    if (S) {
+      unsigned long long base = 0;
+      if (me == 0) base = atomicAdd(&st->samples, (unsigned long long)__builtin_popcountll(S));
+      base = ((unsigned long long)(u32)__builtin_amdgcn_readfirstlane((int)(u32)(base >> 32)) << 32) |
+             (u32)__builtin_amdgcn_readfirstlane((int)(u32)base);
+      if (sampled) {
+        const unsigned long long i = base + (u64)__builtin_popcountll(S & lt);
+        if (i < a.lat_cap) a.lat_tk[i] = lat;
+      }
+    }
+    if (!admitted && I == ~0ull) __builtin_amdgcn_s_sleep(4);
+  }
+  const u32 wp = wave_sum(my_pages), wn = wave_sum(my_nulls);
+  if (me == 0) {
+    if (wp) atomicAdd(&st->pages, (unsigned long long)wp);
+    if (wn) atomicAdd(&st->nulls, (unsigned long long)wn);
+    if (my_stalls) atomicAdd(&st->stalls, (unsigned long long)my_stalls);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -990,6 +1189,118 @@ int ha_run_graphs(const ha_graph_args* a, const uint32_t* nnodes, ha_graph_resul
   res->plain_ms = a->run_plain ? best_plain : 0;
   res->batch_ms = a->run_batch ? best_batch : 0;
   res->plain_launches = (int32_t)(2 * chunks.size());
+  return HA_OK;
+}
+
+int ha_run_serve(const ha_serve_io* io, ha_serve_result* res) {
+  const ha::Params& P = ha::host_params();
+  if (!P.initialized) return hfail(HA_E_STATE, "allocator not initialized");
+  if (!io || !res) return hfail(HA_E_INVALID, "null argument");
+  const int nreq = io->nreq;
+  if (!io->arrival_ns || !io->length || !io->tag || !io->checksum || !io->produced || nreq < 1 || nreq > (1 << 26) ||
+      io->slots < 256 || io->slots > (1 << 20) || io->slots % 256 || io->lat_cap < 0 || io->sample_every < 0 ||
+      (io->lat_cap && !io->lat_ns))
+    return hfail(HA_E_INVALID, "bad serving arguments");
+  if (io->watermark > P.arena_bytes) return hfail(HA_E_INVALID, "watermark above the heap size");
+  uint32_t maxlen = 0;
+  for (int i = 0; i < nreq; ++i) {
+    if (io->length[i] < 1 || io->length[i] > 32768) return hfail(HA_E_INVALID, "request length must be in 1..32768");
+    if (i && io->arrival_ns[i] < io->arrival_ns[i - 1]) return hfail(HA_E_INVALID, "arrivals must be sorted");
+    maxlen = io->length[i] > maxlen ? io->length[i] : maxlen;
+  }
+  if (io->arrival_ns[nreq - 1] > 60000000000ull) return hfail(HA_E_INVALID, "arrivals beyond 60 s");
+  int dev = 0, khz = 0;
+  HH(hipGetDevice(&dev));
+  HH(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  if (khz <= 0) return hfail(HA_E_HIP, "wall clock rate unavailable");
+  const double tk_per_ns = khz * 1e-6;
+  std::vector<u64> arr_tk(nreq);
+  for (int i = 0; i < nreq; ++i) arr_tk[i] = (u64)((double)io->arrival_ns[i] * tk_per_ns);
+  const u32 max_pages = (maxlen + SRV_TOK_PER_PAGE - 1) / SRV_TOK_PER_PAGE;
+  const size_t n = (size_t)nreq;
+  DBuf darr, dlen, dtag, dcs, dprod, dfirst, ddone, dlat, dtab, dst;
+  HH(darr.alloc(n * 8));
+  HH(dlen.alloc(n * 4));
+  HH(dtag.alloc(n * 4));
+  HH(dcs.alloc(n * 4));
+  HH(dprod.alloc(n * 4));
+  HH(dfirst.alloc(n * 8));
+  HH(ddone.alloc(n * 8));
+  HH(dlat.alloc((size_t)(io->lat_cap ? io->lat_cap : 1) * 4));
+  HH(dtab.alloc((size_t)io->slots * max_pages * 8));
+  HH(dst.alloc(sizeof(SrvState)));
+  HH(hipMemcpy(darr.p, arr_tk.data(), n * 8, hipMemcpyHostToDevice));
+  HH(hipMemcpy(dlen.p, io->length, n * 4, hipMemcpyHostToDevice));
+  HH(hipMemcpy(dtag.p, io->tag, n * 4, hipMemcpyHostToDevice));
+  HH(hipMemset(dcs.p, 0, n * 4));
+  HH(hipMemset(dprod.p, 0, n * 4));
+  HH(hipMemset(dfirst.p, 0, n * 8));
+  HH(hipMemset(ddone.p, 0, n * 8));
+  SrvState init;
+  std::memset(&init, 0, sizeof init);
+  init.free_est = P.arena_bytes;
+  HH(hipMemcpy(dst.p, &init, sizeof init, hipMemcpyHostToDevice));
+  SrvArgs a;
+  a.arrival = (const u64*)darr.p;
+  a.length = (const u32*)dlen.p;
+  a.tag = (const u32*)dtag.p;
+  a.checksum = (u32*)dcs.p;
+  a.produced = (u32*)dprod.p;
+  a.first_tk = (u64*)dfirst.p;
+  a.done_tk = (u64*)ddone.p;
+  a.lat_tk = (u32*)dlat.p;
+  a.page_tab = (void**)dtab.p;
+  a.st = (SrvState*)dst.p;
+  a.watermark = io->watermark;
+  a.deadline_tk = arr_tk[nreq - 1] + (u64)(30e9 * tk_per_ns);
+  a.nreq = (u32)nreq;
+  a.max_pages = max_pages;
+  a.sample_every = (u32)io->sample_every;
+  a.lat_cap = (u32)io->lat_cap;
+  Stream s;
+  HH(hipStreamCreate(&s.s));
+  hipEvent_t e0, e1;
+  HH(hipEventCreate(&e0));
+  HH(hipEventCreate(&e1));
+  HH(hipEventRecord(e0, s.s));
+  srv_start_k<<<1, 1, 0, s.s>>>(a.st);
+  srv_loop_k<<<io->slots / 256, 256, 0, s.s>>>(a);
+  HH(hipEventRecord(e1, s.s));
+  HH(hipStreamSynchronize(s.s));
+  HH(hipGetLastError());
+  float t = 0;
+  HH(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  SrvState fin;
+  HH(hipMemcpy(&fin, dst.p, sizeof fin, hipMemcpyDeviceToHost));
+  HH(hipMemcpy(io->checksum, dcs.p, n * 4, hipMemcpyDeviceToHost));
+  HH(hipMemcpy(io->produced, dprod.p, n * 4, hipMemcpyDeviceToHost));
+  const double ns_per_tk = 1.0 / tk_per_ns;
+  std::vector<u64> tk(n);
+  if (io->first_ns) {
+    HH(hipMemcpy(tk.data(), dfirst.p, n * 8, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) io->first_ns[i] = (uint64_t)((double)tk[i] * ns_per_tk);
+  }
+  if (io->done_ns) {
+    HH(hipMemcpy(tk.data(), ddone.p, n * 8, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) io->done_ns[i] = (uint64_t)((double)tk[i] * ns_per_tk);
+  }
+  const uint64_t ns = fin.samples < (unsigned long long)io->lat_cap ? fin.samples : (uint64_t)io->lat_cap;
+  if (ns) {
+    std::vector<u32> l(ns);
+    HH(hipMemcpy(l.data(), dlat.p, ns * 4, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < ns; ++i) io->lat_ns[i] = (uint32_t)((double)l[i] * ns_per_tk);
+  }
+  res->kernel_ms = t;
+  res->pages = fin.pages;
+  res->nulls = fin.nulls;
+  res->stalls = fin.stalls;
+  res->samples = ns;
+  res->completed = fin.done;
+  res->admitted = fin.next_req;
+  res->clock_khz = (uint32_t)khz;
+  res->pad = 0;
   return HA_OK;
 }
 

@@ -165,6 +165,39 @@ typedef struct ha_graph_result {
 } ha_graph_result;
 int ha_run_graphs(const ha_graph_args *a, const uint32_t *nnodes, ha_graph_result *res);
 
+/* Serving loop (continuous batching): one persistent launch of `slots` lanes
+   (a multiple of 256), one sequence slot each.  Request r arrives
+   arrival_ns[r] after the start (sorted, at most 60 s) and decodes length[r]
+   tokens (1..32768) of 16 B, one per iteration, in 256 B pages from hamalloc.
+   Idle slots admit arrived requests in arrival order while the heap's free
+   bytes, refreshed from the slab counters, stay >= watermark.  Token t of a
+   request holds {tag, t, h, ~h}; at completion the request's pages are read
+   back, the hashes h of the intact tokens summed into checksum[r], and the
+   pages freed.  A page malloc that returns NULL ends the request early:
+   produced[r] is the number of tokens it holds.  first_ns / done_ns (may be
+   NULL) get the first-token and completion times; every page malloc whose
+   (request + page index) is a multiple of sample_every is timed into lat_ns
+   (at most lat_cap samples, ns).  No counterpart in the reference: it is the
+   paged-growth workload built on hamalloc/hafree. */
+typedef struct ha_serve_io {
+  const uint64_t *arrival_ns;
+  const uint32_t *length;
+  const uint32_t *tag;
+  uint32_t *checksum;
+  uint32_t *produced;
+  uint64_t *first_ns;
+  uint64_t *done_ns;
+  uint32_t *lat_ns;
+  int nreq, slots, sample_every, lat_cap;
+  uint64_t watermark;
+} ha_serve_io;
+typedef struct ha_serve_result {
+  double kernel_ms;
+  uint64_t pages, nulls, stalls, samples;
+  uint32_t completed, admitted, clock_khz, pad;
+} ha_serve_result;
+int ha_run_serve(const ha_serve_io *io, ha_serve_result *res);
+
 /* Atomic-throughput probe: 64-bit atomic OR over a 1 GiB buffer from 2^20
    lanes with three access shapes; rates in atomics/s:
    out[0] lane-random (one cache line per lane), out[1] wave-run (a wave

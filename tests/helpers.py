@@ -105,3 +105,33 @@ def random_batch_ops(rng, nsteps, lanes, sizes, max_count=16, p_free=0.4):
 def graph_sizes(rng, njobs, lo=1024, hi=65536):
     """Node counts of the random-graph jobs: log-uniform on [lo, hi]."""
     return np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), njobs))).astype(np.uint32)
+
+
+def _hash3(a, b, c):
+    """The drivers' 32-bit mixing hash (halloc_amd/csrc/harness.hip hash3), on uint32 arrays."""
+    u = np.uint32
+    a, b, c = (np.atleast_1d(np.asarray(v, dtype=u)) for v in (a, b, c))
+    x = (a * u(0x9E3779B1)) ^ ((b + u(0x7F4A7C15)) * u(0x85EBCA77)) ^ ((c + u(0x165667B1)) * u(0xC2B2AE3D))
+    x ^= x >> u(16)
+    x *= u(0x7FEB352D)
+    x ^= x >> u(15)
+    x *= u(0x846CA68B)
+    x ^= x >> u(16)
+    return x
+
+
+def serve_checksums(tag, produced):
+    """Expected per-request checksum of the serving loop: the sum (mod 2^32)
+    of the hashes of tokens 0..produced-1."""
+    tag = np.asarray(tag, dtype=np.uint32)
+    n = np.asarray(produced, dtype=np.int64)
+    out = np.zeros(len(n), dtype=np.uint32)
+    nz = np.nonzero(n)[0]
+    if len(nz) == 0:
+        return out
+    cnt = n[nz]
+    starts = np.concatenate(([0], np.cumsum(cnt)[:-1]))
+    t = np.arange(cnt.sum(), dtype=np.int64) - np.repeat(starts, cnt)
+    h = _hash3(np.repeat(tag[nz], cnt), t.astype(np.uint32), np.uint32(0x5E7A11C3)).astype(np.uint64)
+    out[nz] = (np.add.reduceat(h, starts) & 0xFFFFFFFF).astype(np.uint32)
+    return out

@@ -368,3 +368,27 @@ def test_model_batch_programs_keep_invariants(nheads):
     got = recs[..., 0]
     for st, l in zip(*np.nonzero(m)):
         assert (got[st, l, :cnt[st, l]] != M.REC_NULL).all()
+
+
+def test_serving_generator_shape():
+    from halloc_amd.workloads import pages_for, serving_requests
+    arr, ln, tag = serving_requests(200000, seed=0)
+    gaps = np.diff(arr.astype(np.float64))
+    assert (gaps >= 0).all()
+    assert abs(gaps.mean() - 500.0) < 5.0          # 2000 requests per ms
+    assert abs(np.median(ln) - 512) < 10
+    assert ln.min() >= 1 and ln.max() <= 32768
+    assert (pages_for([1, 16, 17, 32768]) == [1, 1, 2, 2048]).all()
+
+
+def test_serve_checksum_helper_by_loop():
+    from tests.helpers import _hash3, serve_checksums
+    tags = np.array([7, 0xFFFFFFFF, 123456], dtype=np.uint32)
+    prod = np.array([5, 0, 40])
+    want = []
+    for tg, n in zip(tags, prod):
+        s = 0
+        for t in range(n):
+            s += int(_hash3(tg, t, 0x5E7A11C3)[0])
+        want.append(s & 0xFFFFFFFF)
+    assert list(serve_checksums(tags, prod)) == want
