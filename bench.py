@@ -86,9 +86,10 @@ def main():
     import halloc_amd as H
     import numpy as np
 
-    probe = None
+    probe = copy_gbps = None
     if not args.no_probe:
         probe = H.Halloc.probe_atomics()
+        copy_gbps = H.Halloc.probe_copy(1 << 30, 5)
     H.Halloc.init(H.HallocOpts(memory=512 << 20, nheads=args.nheads))
     r = H.Halloc.run_plain(nthreads=NTHREADS, bs=BS, size=16, iters=1, warmup=args.warmup,
                            launches=args.steps)
@@ -127,6 +128,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
     }
+    if copy_gbps is not None:
+        out["roofline"]["copy_probe_gbps"] = copy_gbps
     if probe is not None:
         ach = NTHREADS * PAIR_ALGO_ATOMICS / avg_launch_s
         out["atomic_roofline"] = {"achieved": ach, "peak": float(probe[0]), "unit": "atomics/s",

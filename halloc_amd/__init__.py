@@ -28,7 +28,8 @@ SIZE_TABLE_FIELDS = ("nchunks", "chunk_sz", "chunk_id", "ncib", "sparse", "hash_
 EXPORTS = ("ha_default_opts", "ha_init_c", "ha_shutdown_c", "ha_extfrag_c", "ha_free_bytes_c",
            "ha_last_error", "ha_geometry_c", "ha_replay", "ha_dump_words", "ha_dump",
            "ha_run_plain", "ha_malloc_sizes", "ha_free_ptrs", "ha_read_words", "ha_probe_atomics",
-           "ha_run_lists", "ha_run_phase", "ha_replay_batch", "ha_run_graphs", "ha_run_serve")
+           "ha_run_lists", "ha_run_phase", "ha_replay_batch", "ha_run_graphs", "ha_run_serve",
+           "ha_probe_copy")
 
 
 class HallocError(RuntimeError):
@@ -42,7 +43,10 @@ class _Opts(ctypes.Structure):
                 ("roomy_fraction", ctypes.c_double),
                 ("sparse_fraction", ctypes.c_double),
                 ("sb_sz_sh", ctypes.c_int32),
-                ("nheads", ctypes.c_int32)]
+                ("nheads", ctypes.c_int32),
+                ("hash_step_frac", ctypes.c_double),
+                ("cached_arena", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
 
 
 class _Geometry(ctypes.Structure):
@@ -92,7 +96,8 @@ class _GraphResult(ctypes.Structure):
 class _ServeIO(ctypes.Structure):
     _fields_ = [("arrival_ns", ctypes.c_void_p), ("length", ctypes.c_void_p), ("tag", ctypes.c_void_p),
                 ("checksum", ctypes.c_void_p), ("produced", ctypes.c_void_p), ("first_ns", ctypes.c_void_p),
-                ("done_ns", ctypes.c_void_p), ("lat_ns", ctypes.c_void_p), ("nreq", ctypes.c_int32),
+                ("done_ns", ctypes.c_void_p), ("lat_ns", ctypes.c_void_p), ("bad", ctypes.c_void_p),
+                ("bad_first", ctypes.c_void_p), ("nreq", ctypes.c_int32),
                 ("slots", ctypes.c_int32), ("sample_every", ctypes.c_int32), ("lat_cap", ctypes.c_int32),
                 ("watermark", ctypes.c_uint64)]
 
@@ -105,7 +110,10 @@ class _ServeResult(ctypes.Structure):
 
 @dataclass
 class HallocOpts:
-    """Fields and defaults of halloc_opts_t (src/halloc.h:15-34), plus nheads."""
+    """Fields and defaults of halloc_opts_t (src/halloc.h:15-34), plus nheads,
+    hash_step_frac (0 keeps the reference's probe stride) and cached_arena
+    (0: uncached arena, coherent across XCDs; 1: L2-cached arena with a
+    release in every hafree)."""
     memory: int = 512 << 20
     halloc_fraction: float = 0.75
     busy_fraction: float = 0.835
@@ -113,10 +121,13 @@ class HallocOpts:
     sparse_fraction: float = 0.012
     sb_sz_sh: int = 22
     nheads: int = 8
+    hash_step_frac: float = 0.0
+    cached_arena: int = 0
 
     def _c(self) -> _Opts:
         return _Opts(self.memory, self.halloc_fraction, self.busy_fraction, self.roomy_fraction,
-                     self.sparse_fraction, self.sb_sz_sh, self.nheads)
+                     self.sparse_fraction, self.sb_sz_sh, self.nheads, self.hash_step_frac,
+                     self.cached_arena, 0)
 
 
 _lib = None
@@ -153,6 +164,7 @@ def lib() -> ctypes.CDLL:
         "ha_replay_batch": (i32, [vp, i32, i32, vp, vp]),
         "ha_run_graphs": (i32, [P(_GraphArgs), vp, P(_GraphResult)]),
         "ha_run_serve": (i32, [P(_ServeIO), P(_ServeResult)]),
+        "ha_probe_copy": (i32, [u64, i32, P(ctypes.c_double)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -267,6 +279,13 @@ class Halloc:
         return out
 
     @staticmethod
+    def probe_copy(nbytes=1 << 30, reps=5) -> float:
+        """GB/s (read + write) of a streaming device-to-device copy of nbytes."""
+        v = ctypes.c_double(0)
+        _check(lib().ha_probe_copy(nbytes, reps, ctypes.byref(v)))
+        return v.value
+
+    @staticmethod
     def run_lists(sizes: np.ndarray, order=None, bs=256, want_ptrs=False):
         """sizes: (per, n) uint32, 0 = no request; order: optional permutation of
         the per*n slots for the frees.  Returns timings, counts and (optionally)
@@ -326,12 +345,15 @@ class Halloc:
         first = np.zeros(n, dtype=np.uint64)
         done = np.zeros(n, dtype=np.uint64)
         lat = np.zeros(max(lat_cap, 1), dtype=np.uint32)
+        bad = np.zeros(n, dtype=np.uint32)
+        badf = np.zeros(n, dtype=np.uint64)
         io = _ServeIO(arr.ctypes.data, ln.ctypes.data, tg.ctypes.data, cs.ctypes.data, prod.ctypes.data,
-                      first.ctypes.data, done.ctypes.data, lat.ctypes.data, n, slots, sample_every, lat_cap,
-                      watermark)
+                      first.ctypes.data, done.ctypes.data, lat.ctypes.data, bad.ctypes.data, badf.ctypes.data,
+                      n, slots, sample_every, lat_cap, watermark)
         r = _ServeResult()
         _check(lib().ha_run_serve(ctypes.byref(io), ctypes.byref(r)))
         return {"kernel_ms": r.kernel_ms, "pages": int(r.pages), "nulls": int(r.nulls), "stalls": int(r.stalls),
                 "completed": int(r.completed), "admitted": int(r.admitted), "clock_khz": int(r.clock_khz),
-                "checksum": cs, "produced": prod, "first_ns": first, "done_ns": done,
+                "checksum": cs, "produced": prod, "first_ns": first, "done_ns": done, "bad": bad,
+                "bad_first": badf,
                 "lat_ns": lat[:int(r.samples)].copy()}

@@ -79,6 +79,7 @@ struct Params {
   u32* ticket;
   u32 nsbs, sb_sz, sb_sz_sh, nheads;
   u32 nsw, bit_words, nib_words, initialized;
+  u32 cached_arena, pad;  // cached_arena: hafree releases before publishing
   u64 cuda_memory;
   SizeInfo si[NSIZES];
 };

@@ -246,6 +246,9 @@ __device__ __noinline__ bool steal_idle_head(u32 ihead, u32 size_id) {
   return false;
 }
 
+// A head search frees at most this many idle heads before it gives up.
+constexpr u32 MAX_STEALS = 4;
+
 // With several head sets, free slabs are scarce below nheads * 16 / 2.
 HA_DEV bool free_scarce() {
   if (ha_p.nheads <= 1) return false;
@@ -262,7 +265,8 @@ HA_DEV bool free_scarce() {
 __device__ __noinline__ u32 find_sb_for_size(u32 size_id, u32 chunk_id, u32 ihead, bool now) {
   const SizeInfo& s = ha_p.si[size_id];
   u32 sb;
-  bool tried_share = false, stole = false;
+  bool tried_share = false;
+  u32 nsteal = 0;
 again:
   while ((sb = set_pop(set_roomy(size_id))) != SB_NONE) {
     Desc* d = ha_p.desc + sb;
@@ -353,48 +357,55 @@ again:
     u32 shared = share_head(ihead, size_id, chunk_id);
     if (shared != SB_NONE) return shared;
   }
-  if (!stole && steal_idle_head(ihead, size_id)) {
-    stole = true;
+  if (nsteal < MAX_STEALS && steal_idle_head(ihead, size_id)) {
+    ++nsteal;
     goto again;
   }
   return SB_NONE;
 }
 
-// new_sb_for_size (src/slab.cuh:378-445)
-__device__ __noinline__ u32 new_sb_for_size(u32 size_id, u32 ihead) {
+// new_sb_for_size (src/slab.cuh:378-445).  `seen` is the head word the caller
+// last used.  A head replaced since then is returned as it is; otherwise the
+// wave that takes the head lock replaces a missing or full head, and a wave
+// that finds the lock taken waits for the head to change or the lock to be
+// released and looks again.  SB_NONE only comes back from a wave that held
+// the lock itself and found no slab.
+__device__ __noinline__ u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen) {
   const SizeInfo& s = ha_p.si[size_id];
   u32* hp = hot(ha_p.head, ihead, size_id);
   u32* lp = hot(ha_p.head_lock, ihead, size_id);
   u32* cp = hot(ha_p.cached, ihead, size_id);
-  u32 cur = ld32(hp);
-  if (try_lock(lp)) {
-    cur = ld32(hp);
-    u32 nh;
-    if (cur == SB_NONE || ctr_count(ld64(ctr_of(cur & ~HEAD_EXT))) >= head_limit(cur, s)) {
-      nh = ld32(cp);
-      st32(cp, SB_NONE);
-      if (nh == SB_NONE) nh = find_sb_for_size(size_id, s.chunk_id, ihead, true);
-      if (nh != SB_NONE) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        st32(hp, nh);
-        if (cur != SB_NONE) {
-          u32 csb = cur & ~HEAD_EXT;
-          u32* l = ha_p.sb_lock + csb;
-          lock(l);
-          detach_head(csb);
-          unlock(l);
+  for (;;) {
+    u32 cur = ld32(hp);
+    if (cur != seen && cur != SB_NONE) return cur;
+    if (try_lock(lp)) {
+      cur = ld32(hp);
+      u32 nh;
+      if (cur == SB_NONE || ctr_count(ld64(ctr_of(cur & ~HEAD_EXT))) >= head_limit(cur, s)) {
+        nh = ld32(cp);
+        st32(cp, SB_NONE);
+        if (nh == SB_NONE) nh = find_sb_for_size(size_id, s.chunk_id, ihead, true);
+        if (nh != SB_NONE) {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+          st32(hp, nh);
+          if (cur != SB_NONE) {
+            u32 csb = cur & ~HEAD_EXT;
+            u32* l = ha_p.sb_lock + csb;
+            lock(l);
+            detach_head(csb);
+            unlock(l);
+          }
+          st32(cp, find_sb_for_size(size_id, s.chunk_id, ihead, false));
         }
-        st32(cp, find_sb_for_size(size_id, s.chunk_id, ihead, false));
+      } else {
+        nh = cur;
       }
-    } else {
-      nh = ld32(hp);
+      unlock(lp);
+      return nh;
     }
-    unlock(lp);
-    return nh;
+    while (ld32(hp) == cur && ld32(lp) != 0) __builtin_amdgcn_s_sleep(2);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
-  while (ld32(hp) == cur && ld32(lp) != 0) __builtin_amdgcn_s_sleep(2);
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return ld32(hp);
 }
 
 // ------------------------------------------------------------- malloc path
@@ -526,7 +537,7 @@ HA_DEV void* malloc_group(u32 sid, u64 g, u32 ihead, u32 me) {
     }
     if (need) {
       u32 nh = 0;
-      if (me == lead) nh = new_sb_for_size(sid, ihead);
+      if (me == lead) nh = new_sb_for_size(sid, ihead, hword);
       hword = rdl(nh, lead);
       if (hword == SB_NONE) want = 0;
     }
@@ -549,7 +560,7 @@ HA_DEV void malloc_batch_group(u32 sid, u64 g, u32 ihead, u32 me, u32 cnt, void*
   u32 hword = rfl(ld32(hot(ha_p.head, ihead, sid)));
   if (hword == SB_NONE) {  // no head yet: get one first
     u32 nh = 0;
-    if (me == lead) nh = new_sb_for_size(sid, ihead);
+    if (me == lead) nh = new_sb_for_size(sid, ihead, SB_NONE);
     hword = rdl(nh, lead);
   }
   u32 base = 0;
@@ -593,7 +604,7 @@ HA_DEV void malloc_batch_group(u32 sid, u64 g, u32 ihead, u32 me, u32 cnt, void*
       }
     }
   }
-  if (need && me == lead) new_sb_for_size(sid, ihead);
+  if (need && me == lead) new_sb_for_size(sid, ihead, hword);
   for (u32 j = 0; j < R; ++j) {
     const bool pend = cnt > j && out[j] == nullptr;
     const u64 Pj = ballot(pend);
@@ -619,6 +630,12 @@ HA_DEV void* malloc_small(u32 nbytes, u32 ihead) {
 // --------------------------------------------------------------- free path
 // hafree_small (src/halloc.cu:211-230) for the active lanes; off = p - arena.
 HA_DEV void free_small(u64 off) {
+  // Each XCD has its own write-back L2.  With a cached arena the caller's
+  // stores to the blocks must reach memory before another XCD can be handed
+  // the blocks, or a later eviction of this XCD's dirty lines would overwrite
+  // the new owner's data: one agent-scope release per wave call.  The default
+  // uncached arena needs none.
+  if (ha_p.cached_arena) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   const u32 me = lane_id();
   const u32 sb = (u32)(off >> ha_p.sb_sz_sh);
   Desc* d = ha_p.desc + sb;
@@ -770,6 +787,7 @@ namespace ha {
 static std::mutex g_mu;
 static Params g_host;          // host copy of ha_p
 static void* g_meta = nullptr;  // one allocation for all metadata
+
 static thread_local std::string g_err;
 
 int fail(int code, const std::string& msg) {
@@ -781,6 +799,50 @@ int fail(int code, const std::string& msg) {
     hipError_t e_ = (call);                                                                       \
     if (e_ != hipSuccess) return ::ha::fail(HA_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+// Uncached arenas are never freed once a kernel has touched them (freeing
+// such memory has corrupted later allocations on this platform); shutdown
+// hands the arena back to this pool and a later init on the same device
+// reuses the smallest one that is large enough.
+struct UcArena {
+  int dev;
+  void* p;
+  u64 bytes;
+  bool in_use;
+};
+static std::vector<UcArena> g_uc;
+
+static int arena_get(int dev, u64 bytes, bool cached, void** out) {
+  *out = nullptr;
+  if (cached) {
+    if (hipMalloc(out, bytes) != hipSuccess) return fail(HA_E_NOMEM, "hipMalloc of the arena failed");
+    return HA_OK;
+  }
+  UcArena* best = nullptr;
+  for (auto& a : g_uc)
+    if (a.dev == dev && !a.in_use && a.bytes >= bytes && (!best || a.bytes < best->bytes)) best = &a;
+  if (!best) {
+    void* p = nullptr;
+    if (hipExtMallocWithFlags(&p, bytes, hipDeviceMallocUncached) != hipSuccess)
+      return fail(HA_E_NOMEM, "allocation of the uncached arena failed");
+    g_uc.push_back(UcArena{dev, p, bytes, false});
+    best = &g_uc.back();
+  }
+  best->in_use = true;
+  *out = best->p;
+  return HA_OK;
+}
+
+static int arena_put(void* p, bool cached) {
+  if (!p) return HA_OK;
+  if (cached) {
+    HA_HIP(hipFree(p));
+    return HA_OK;
+  }
+  for (auto& a : g_uc)
+    if (a.p == p) a.in_use = false;
+  return HA_OK;
+}
 
 // largest prime p in [3, n) with n % p != 0 and nb % p != 0 (src/utils.cu:8-24)
 static u32 max_prime_below(u32 n, u32 nb) {
@@ -814,7 +876,8 @@ static bool build_size_infos(u32 sb_sz, const ha_opts_c& o, SizeInfo* out) {
     s.chunk_sz = chunk_val(s.chunk_bytes);
     s.ncib = s.block_sz / s.chunk_bytes;
     s.nchunks = s.nblocks * s.ncib;
-    u32 p = max_prime_below(s.nblocks / 256 + s.nblocks / 64, s.nblocks);
+    const u32 bound = o.hash_step_frac > 0 ? (u32)(o.hash_step_frac * s.nblocks) : s.nblocks / 256 + s.nblocks / 64;
+    u32 p = max_prime_below(bound, s.nblocks);
     if (!p || !s.nblocks) return false;
     s.hash_step = s.ncib * p;
     s.roomy_thr = (u32)(o.roomy_fraction * s.nchunks);
@@ -853,7 +916,8 @@ static int shutdown_locked() {
   std::memset(&zero, 0, sizeof zero);
   HA_HIP(hipDeviceSynchronize());
   HA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(ha_p), &zero, sizeof zero));
-  if (g_host.arena) HA_HIP(hipFree(g_host.arena));
+  int rc = arena_put(g_host.arena, g_host.cached_arena != 0);
+  if (rc) return rc;
   if (g_meta) HA_HIP(hipFree(g_meta));
   g_meta = nullptr;
   std::memset(&g_host, 0, sizeof g_host);
@@ -872,6 +936,8 @@ static int init_locked(const ha_opts_c& o) {
       !(o.sparse_fraction <= o.roomy_fraction) || !(o.roomy_fraction <= o.busy_fraction) ||
       !(o.busy_fraction <= 1))
     return fail(HA_E_INVALID, "fractions must satisfy 0 <= sparse <= roomy <= busy <= 1, 0 < halloc <= 1");
+  if (!(o.hash_step_frac >= 0 && o.hash_step_frac < 1)) return fail(HA_E_INVALID, "hash_step_frac must be in [0, 1)");
+  if (o.cached_arena != 0 && o.cached_arena != 1) return fail(HA_E_INVALID, "cached_arena must be 0 or 1");
   int dev = 0;
   HA_HIP(hipGetDevice(&dev));
   hipDeviceProp_t prop;
@@ -886,6 +952,7 @@ static int init_locked(const ha_opts_c& o) {
   P.nsbs = (u32)(halloc_memory / P.sb_sz);
   if (P.nsbs == 0 || P.nsbs > MAX_NSBS) return fail(HA_E_INVALID, "memory gives no slab or too many slabs");
   P.nheads = (u32)o.nheads;
+  P.cached_arena = (u32)o.cached_arena;
   P.nsw = (P.nsbs + 63) / 64;
   P.bit_words = P.sb_sz / BLOCK_STEP / 64;
   P.nib_words = P.sb_sz / BLOCK_STEP / 16;
@@ -910,10 +977,11 @@ static int init_locked(const ha_opts_c& o) {
   u64 o_hlock = off; off = align_up(off + hot_bytes, 256);
   u64 o_ticket = off; off = align_up(off + hot_bytes, 256);
   void* arena = nullptr;
-  if (hipMalloc(&arena, P.arena_bytes) != hipSuccess) return fail(HA_E_NOMEM, "hipMalloc of the arena failed");
+  int rc = arena_get(dev, P.arena_bytes, o.cached_arena != 0, &arena);
+  if (rc) return rc;
   void* meta = nullptr;
   if (hipMalloc(&meta, off) != hipSuccess) {
-    (void)hipFree(arena);
+    (void)arena_put(arena, o.cached_arena != 0);
     return fail(HA_E_NOMEM, "hipMalloc of the metadata failed");
   }
   char* m = (char*)meta;
@@ -993,6 +1061,9 @@ void ha_default_opts(ha_opts_c* o) {
   o->sparse_fraction = d.sparse_fraction;
   o->sb_sz_sh = d.sb_sz_sh;
   o->nheads = 8;
+  o->hash_step_frac = 0.0;
+  o->cached_arena = 0;
+  o->pad = 0;
 }
 
 int ha_init_c(const ha_opts_c* o) {

@@ -262,6 +262,13 @@ __global__ void probe_k(u64* buf, u64 mask, int iters, int shape, u64* sink) {
   sink[tid] = acc;
 }
 
+// Streaming copy of n16 16-byte words, grid-stride: the known-size kernel
+// against which measured bandwidth and HBM byte counters are calibrated.
+__global__ void __launch_bounds__(256) copy_k(const uint4* __restrict__ src, uint4* __restrict__ dst, u64 n16) {
+  const u64 stride = (u64)gridDim.x * blockDim.x;
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 // --------------------------------------------------------------- lists
 // Lane i allocates sizes[j*n + i] for j < per (0 = no request) and writes the
 // size into the block's first word (tst/common/common.h:375-388).
@@ -566,6 +573,8 @@ struct SrvArgs {
   u64* first_tk;       // tick of the first token
   u64* done_tk;        // tick of completion
   u32* lat_tk;         // sampled page-malloc latencies
+  u32* bad;            // tokens that did not read back intact
+  u64* bad_first;      // words 0 and 1 of the first such token
   void** page_tab;     // max_pages page pointers per slot
   SrvState* st;
   u64 watermark, deadline_tk;
@@ -593,7 +602,8 @@ __global__ void __launch_bounds__(256) srv_loop_k(SrvArgs a) {
   const u64 lt = me ? (~0ull >> (64 - me)) : 0ull;  // lanes below me
   void** pages = a.page_tab + (u64)slot * a.max_pages;
   const u64 t0 = __hip_atomic_load(&st->t0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  u32 req = SRV_NONE, tag = 0, len = 0, tok = 0, cs = 0, np = 0, drain = 0;
+  u32 req = SRV_NONE, tag = 0, len = 0, tok = 0, cs = 0, np = 0, drain = 0, nbad = 0;
+  u64 first_bad = 0;
   bool draining = false;
   char* cur = nullptr;
   u32 my_pages = 0, my_nulls = 0, my_stalls = 0;
@@ -603,7 +613,7 @@ __global__ void __launch_bounds__(256) srv_loop_k(SrvArgs a) {
       unsigned long long used = 0;
       for (u32 b = me; b < P.nsbs; b += 64) {
         const u32 csz = __hip_atomic_load(&P.desc[b].chunk_sz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const u32 c = __hip_atomic_load(P.ctr + (u64)b * ha::CTR_STRIDE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const u64 c = __hip_atomic_load(P.ctr + (u64)b * ha::CTR_STRIDE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         used += (unsigned long long)ha::ctr_count(c) * csz;
       }
       for (int o = 32; o; o >>= 1) used += __shfl_xor(used, o);
@@ -639,7 +649,8 @@ __global__ void __launch_bounds__(256) srv_loop_k(SrvArgs a) {
               req = nr + rank;
               tag = a.tag[req];
               len = a.length[req];
-              tok = cs = np = drain = 0;
+              tok = cs = np = drain = nbad = 0;
+              first_bad = 0;
               draining = false;
               a.first_tk[req] = now;
             }
@@ -697,7 +708,12 @@ __global__ void __launch_bounds__(256) srv_loop_k(SrvArgs a) {
         for (u32 j = 0; j < SRV_TOK_PER_PAGE; ++j) {
           if (j < n) {
             const u32 t = t_lo + j, h = tok_hash(tag, t);
-            if (v[j].x == tag && v[j].y == t && v[j].z == h && v[j].w == ~h) cs += h;
+            if (v[j].x == tag && v[j].y == t && v[j].z == h && v[j].w == ~h) {
+              cs += h;
+            } else {
+              if (!nbad) first_bad = (u64)v[j].x | ((u64)v[j].y << 32);
+              ++nbad;
+            }
           }
         }
         hafree((void*)pg);
@@ -706,6 +722,8 @@ __global__ void __launch_bounds__(256) srv_loop_k(SrvArgs a) {
       if (drain >= np) {
         a.checksum[req] = cs;
         a.produced[req] = tok;
+        a.bad[req] = nbad;
+        a.bad_first[req] = first_bad;
         a.done_tk[req] = now;
         req = SRV_NONE;
         finished = true;
@@ -909,6 +927,41 @@ int ha_probe_atomics(double* out) {
   HH(hipGetLastError());
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  return HA_OK;
+}
+
+int ha_probe_copy(uint64_t bytes, int reps, double* gbps) {
+  if (!gbps || reps < 1 || bytes < (1u << 20) || bytes % 16 || bytes > (64ull << 30))
+    return hfail(HA_E_INVALID, "bad copy probe arguments");
+  DBuf a, b;
+  HH(a.alloc(bytes));
+  HH(b.alloc(bytes));
+  HH(hipMemset(a.p, 1, bytes));
+  HH(hipMemset(b.p, 0, bytes));
+  int dev = 0, ncu = 0;
+  HH(hipGetDevice(&dev));
+  HH(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const u64 n16 = bytes / 16;
+  const int grid = ncu * 8;
+  Stream s;
+  HH(hipStreamCreate(&s.s));
+  hipEvent_t e0, e1;
+  HH(hipEventCreate(&e0));
+  HH(hipEventCreate(&e1));
+  float best = 1e30f;
+  for (int r = 0; r < reps; ++r) {
+    HH(hipEventRecord(e0, s.s));
+    copy_k<<<grid, 256, 0, s.s>>>((const uint4*)a.p, (uint4*)b.p, n16);
+    HH(hipEventRecord(e1, s.s));
+    HH(hipEventSynchronize(e1));
+    float t = 0;
+    HH(hipEventElapsedTime(&t, e0, e1));
+    best = t < best ? t : best;
+  }
+  HH(hipGetLastError());
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *gbps = 2.0 * (double)bytes / (best * 1e-3) / 1e9;
   return HA_OK;
 }
 
@@ -1218,7 +1271,11 @@ int ha_run_serve(const ha_serve_io* io, ha_serve_result* res) {
   for (int i = 0; i < nreq; ++i) arr_tk[i] = (u64)((double)io->arrival_ns[i] * tk_per_ns);
   const u32 max_pages = (maxlen + SRV_TOK_PER_PAGE - 1) / SRV_TOK_PER_PAGE;
   const size_t n = (size_t)nreq;
-  DBuf darr, dlen, dtag, dcs, dprod, dfirst, ddone, dlat, dtab, dst;
+  DBuf darr, dlen, dtag, dcs, dprod, dfirst, ddone, dlat, dtab, dst, dbad, dbadf;
+  HH(dbad.alloc(n * 4));
+  HH(dbadf.alloc(n * 8));
+  HH(hipMemset(dbad.p, 0, n * 4));
+  HH(hipMemset(dbadf.p, 0, n * 8));
   HH(darr.alloc(n * 8));
   HH(dlen.alloc(n * 4));
   HH(dtag.alloc(n * 4));
@@ -1249,6 +1306,8 @@ int ha_run_serve(const ha_serve_io* io, ha_serve_result* res) {
   a.first_tk = (u64*)dfirst.p;
   a.done_tk = (u64*)ddone.p;
   a.lat_tk = (u32*)dlat.p;
+  a.bad = (u32*)dbad.p;
+  a.bad_first = (u64*)dbadf.p;
   a.page_tab = (void**)dtab.p;
   a.st = (SrvState*)dst.p;
   a.watermark = io->watermark;
@@ -1286,6 +1345,8 @@ int ha_run_serve(const ha_serve_io* io, ha_serve_result* res) {
     HH(hipMemcpy(tk.data(), ddone.p, n * 8, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n; ++i) io->done_ns[i] = (uint64_t)((double)tk[i] * ns_per_tk);
   }
+  if (io->bad) HH(hipMemcpy(io->bad, dbad.p, n * 4, hipMemcpyDeviceToHost));
+  if (io->bad_first) HH(hipMemcpy(io->bad_first, dbadf.p, n * 8, hipMemcpyDeviceToHost));
   const uint64_t ns = fin.samples < (unsigned long long)io->lat_cap ? fin.samples : (uint64_t)io->lat_cap;
   if (ns) {
     std::vector<u32> l(ns);

@@ -35,6 +35,14 @@ typedef struct ha_opts_c {
   double sparse_fraction;    /* default 0.012 */
   int32_t sb_sz_sh;          /* default 22 (4 MiB slabs), 21..26 */
   int32_t nheads;            /* head slabs per size class; lanes pick head xcc_id % nheads; default 8 */
+  double hash_step_frac;     /* 0 (default): the probe stride's prime is the largest below
+                                nblocks/256 + nblocks/64 (src/halloc.cu:378-381); else below
+                                hash_step_frac * nblocks, 0 < hash_step_frac < 1 */
+  int32_t cached_arena;      /* 0 (default): the arena is uncached device memory, coherent
+                                across the XCDs' L2s, and kept for reuse after ha_shutdown;
+                                1: L2-cached (hipMalloc) arena, and every hafree performs an
+                                agent-scope release before the blocks can move to another XCD */
+  int32_t pad;
 } ha_opts_c;
 
 /* Fills *o with the defaults of src/halloc.h:30-32 and nheads = 8. */
@@ -177,7 +185,9 @@ int ha_run_graphs(const ha_graph_args *a, const uint32_t *nnodes, ha_graph_resul
    produced[r] is the number of tokens it holds.  first_ns / done_ns (may be
    NULL) get the first-token and completion times; every page malloc whose
    (request + page index) is a multiple of sample_every is timed into lat_ns
-   (at most lat_cap samples, ns).  No counterpart in the reference: it is the
+   (at most lat_cap samples, ns).  bad / bad_first (may be NULL) get the
+   number of tokens that did not read back intact and words 0-1 of the first
+   one.  No counterpart in the reference: it is the
    paged-growth workload built on hamalloc/hafree. */
 typedef struct ha_serve_io {
   const uint64_t *arrival_ns;
@@ -188,6 +198,8 @@ typedef struct ha_serve_io {
   uint64_t *first_ns;
   uint64_t *done_ns;
   uint32_t *lat_ns;
+  uint32_t *bad;
+  uint64_t *bad_first;
   int nreq, slots, sample_every, lat_cap;
   uint64_t watermark;
 } ha_serve_io;
@@ -197,6 +209,11 @@ typedef struct ha_serve_result {
   uint32_t completed, admitted, clock_khz, pad;
 } ha_serve_result;
 int ha_run_serve(const ha_serve_io *io, ha_serve_result *res);
+
+/* Copy probe: `reps` launches of a streaming copy of `bytes` (a multiple of 16,
+   1 MiB..64 GiB) between two device buffers; *gbps = 2 * bytes / best launch
+   time.  The known-size kernel for calibrating HBM byte counters. */
+int ha_probe_copy(uint64_t bytes, int reps, double *gbps);
 
 /* Atomic-throughput probe: 64-bit atomic OR over a 1 GiB buffer from 2^20
    lanes with three access shapes; rates in atomics/s:

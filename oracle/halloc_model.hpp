@@ -90,6 +90,7 @@ constexpr u32 MAXB = 16;            // most blocks one lane takes in one batched
 // a head word is a slab id, plus HEAD_EXT when the slab came from the busy set:
 // such a head is filled up to (busy + nchunks) / 2 instead of busy
 constexpr u32 HEAD_EXT = 0x80000000u;
+constexpr u32 MAX_STEALS = 4;       // idle heads one head search may free before it gives up
 
 // slab counter layout (64-bit)
 constexpr u32 CTR_CHUNK_SH = 8;
@@ -145,6 +146,7 @@ struct Opts {
   int sb_sz_sh = 22;
   int nheads = 1;
   int hash_mode = 1;
+  double hash_step_frac = 0.0;   // 0: the reference's prime bound
   u64 device_memory = 0;         // 0: no 3/4-of-device cap
 };
 
@@ -168,7 +170,8 @@ inline bool build_size_infos(u32 sb_sz, const Opts& o, SizeInfo out[NSIZES]) {
     si.chunk_sz = chunk_val(si.chunk_bytes);
     si.ncib = si.block_sz / si.chunk_bytes;
     si.nchunks = nblocks * si.ncib;
-    u32 p = max_prime_below(nblocks / 256 + nblocks / 64, nblocks);
+    const u32 bound = o.hash_step_frac > 0 ? (u32)(o.hash_step_frac * nblocks) : nblocks / 256 + nblocks / 64;
+    u32 p = max_prime_below(bound, nblocks);
     if (p == 0 || nblocks == 0) return false;
     si.hash_step = si.ncib * p;
     si.roomy_thr = (u32)(o.roomy_fraction * si.nchunks);
@@ -431,7 +434,8 @@ class Heap {
   u32 find_sb_for_size(u32 size_id, u32 chunk_id, u32 ihead = 0, bool now = false) {
     const SizeInfo& s = si[size_id];
     u32 sb;
-    bool tried_share = false, stole = false;
+    bool tried_share = false;
+    u32 nsteal = 0;
   again:
     while ((sb = set_pop(set_roomy(size_id))) != SB_NONE) {
       lock(sb_lock[sb]);
@@ -508,38 +512,43 @@ class Heap {
       u32 shared = share_head(ihead, size_id, chunk_id);
       if (shared != SB_NONE) return shared;
     }
-    if (!stole && steal_idle_head(ihead, size_id)) { stole = true; goto again; }
+    if (nsteal < MAX_STEALS && steal_idle_head(ihead, size_id)) { ++nsteal; goto again; }
     return SB_NONE;
   }
 
-  // new_sb_for_size (src/slab.cuh:378-445)
-  u32 new_sb_for_size(u32 size_id, u32 ihead) {
+  // new_sb_for_size (src/slab.cuh:378-445).  `seen` is the head word the
+  // caller last used: a head replaced since then is returned as it is; a
+  // thread that finds the head lock taken waits for a change and looks again.
+  // SB_NONE only comes back from the lock holder that found no slab.
+  u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen) {
     const SizeInfo& s = si[size_id];
     size_t hi = (size_t)ihead * NSIZES + size_id;
-    u32 cur = head[hi].load();
-    if (try_lock(head_lock[hi])) {
-      cur = head[hi].load();
-      u32 nh;
-      if (cur == SB_NONE || ctr_count(ctr[cur & ~HEAD_EXT].load()) >= head_limit(cur, s)) {
-        nh = cached[hi].load();
-        cached[hi].store(SB_NONE);
-        if (nh == SB_NONE) nh = find_sb_for_size(size_id, s.chunk_id, ihead, true);
-        if (nh != SB_NONE) {
-          head[hi].store(nh);
-          if (cur != SB_NONE) {
-            u32 csb = cur & ~HEAD_EXT;
-            lock(sb_lock[csb]); detach_head(csb); unlock(sb_lock[csb]);
+    for (;;) {
+      u32 cur = head[hi].load();
+      if (cur != seen && cur != SB_NONE) return cur;
+      if (try_lock(head_lock[hi])) {
+        cur = head[hi].load();
+        u32 nh;
+        if (cur == SB_NONE || ctr_count(ctr[cur & ~HEAD_EXT].load()) >= head_limit(cur, s)) {
+          nh = cached[hi].load();
+          cached[hi].store(SB_NONE);
+          if (nh == SB_NONE) nh = find_sb_for_size(size_id, s.chunk_id, ihead, true);
+          if (nh != SB_NONE) {
+            head[hi].store(nh);
+            if (cur != SB_NONE) {
+              u32 csb = cur & ~HEAD_EXT;
+              lock(sb_lock[csb]); detach_head(csb); unlock(sb_lock[csb]);
+            }
+            cached[hi].store(find_sb_for_size(size_id, s.chunk_id));
           }
-          cached[hi].store(find_sb_for_size(size_id, s.chunk_id));
+        } else {
+          nh = cur;
         }
-      } else {
-        nh = head[hi].load();
+        unlock(head_lock[hi]);
+        return nh;
       }
-      unlock(head_lock[hi]);
-      return nh;
+      while (head[hi].load() == cur && head_lock[hi].load() != 0) std::this_thread::yield();
     }
-    while (head[hi].load() == cur && head_lock[hi].load() != 0) std::this_thread::yield();
-    return head[hi].load();
   }
 
   // ---------------------------------------------------------- malloc (wave)
@@ -636,7 +645,7 @@ class Heap {
         }
       }
       if (need) {
-        u32 nh = new_sb_for_size(size_id, ihead);
+        u32 nh = new_sb_for_size(size_id, ihead, hword);
         hword = nh;
         if (nh == SB_NONE) {
           for (u32 l = 0; l < WAVE; ++l) if (want >> l & 1) {
@@ -659,7 +668,7 @@ class Heap {
     const SizeInfo& s = si[size_id];
     size_t hi = (size_t)ihead * NSIZES + size_id;
     u32 hword = head[hi].load();
-    if (hword == SB_NONE) hword = new_sb_for_size(size_id, ihead);  // no head yet: get one first
+    if (hword == SB_NONE) hword = new_sb_for_size(size_id, ihead, SB_NONE);  // no head yet: get one first
     u32 R = 0;
     for (u32 l = 0; l < WAVE; ++l) if ((g >> l & 1) && cnt[l] > R) R = cnt[l];
     const u32 L = 64 - (u32)__builtin_clzll(g);
@@ -700,7 +709,7 @@ class Heap {
         }
       }
     }
-    if (need) new_sb_for_size(size_id, ihead);
+    if (need) new_sb_for_size(size_id, ihead, hword);
     for (u32 j = 0; j < R; ++j)
       if (pend[j]) malloc_group(ihead, size_id, pend[j], out + j * WAVE, rec ? rec + j * WAVE : nullptr);
   }

@@ -38,14 +38,15 @@ class HmOpts(ctypes.Structure):
                 ("sb_sz_sh", ctypes.c_int32),
                 ("nheads", ctypes.c_int32),
                 ("hash_mode", ctypes.c_int32),
-                ("pad", ctypes.c_int32)]
+                ("pad", ctypes.c_int32),
+                ("hash_step_frac", ctypes.c_double)]
 
 
 def make_opts(memory=512 << 20, halloc_fraction=0.75, busy_fraction=0.835, roomy_fraction=0.6,
-              sparse_fraction=0.012, sb_sz_sh=22, nheads=1, hash_mode=1) -> HmOpts:
+              sparse_fraction=0.012, sb_sz_sh=22, nheads=1, hash_mode=1, hash_step_frac=0.0) -> HmOpts:
     """Defaults follow src/halloc.h:30-32."""
     return HmOpts(memory, halloc_fraction, busy_fraction, roomy_fraction, sparse_fraction,
-                  sb_sz_sh, nheads, hash_mode, 0)
+                  sb_sz_sh, nheads, hash_mode, 0, hash_step_frac)
 
 
 _lib = None
@@ -81,6 +82,8 @@ def lib() -> ctypes.CDLL:
             ("hm_dump_words", u64, [vp]),
             ("hm_dump", ctypes.c_int, [vp, vp]),
             ("hm_check_dump", i64, [P(HmOpts), vp, vp]),
+            ("hm_malloc_threads", ctypes.c_int, [vp, ctypes.c_int, vp, ctypes.c_int, vp]),
+            ("hm_free_threads", ctypes.c_int, [vp, ctypes.c_int, vp, ctypes.c_int]),
             ("hm_bench_plain", ctypes.c_double, [P(HmOpts), ctypes.c_int, ctypes.c_int, ctypes.c_int, P(u64)]),
             ("hm_bench_glibc", ctypes.c_double, [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(u64)]),
         ]:
@@ -143,6 +146,20 @@ class ModelHeap:
         if rc != 0:
             raise ValueError(f"hm_replay_batch failed: {rc}")
         return recs
+
+    def malloc_threads(self, sizes, nthreads: int = 8) -> np.ndarray:
+        """Mallocs of `sizes` in 64-lane waves raced over nthreads threads;
+        returns arena offset + 1 per request (0 = NULL)."""
+        sz = np.ascontiguousarray(sizes, dtype=np.uint32)
+        ptrs = np.zeros(len(sz), dtype=np.uint64)
+        if lib().hm_malloc_threads(self.h, nthreads, sz.ctypes.data, len(sz), ptrs.ctypes.data) != 0:
+            raise ValueError("hm_malloc_threads failed")
+        return ptrs
+
+    def free_threads(self, ptrs, nthreads: int = 8) -> None:
+        p = np.ascontiguousarray(ptrs, dtype=np.uint64)
+        if lib().hm_free_threads(self.h, nthreads, p.ctypes.data, len(p)) != 0:
+            raise ValueError("hm_free_threads failed")
 
     def dump(self) -> np.ndarray:
         n = lib().hm_dump_words(self.h)

@@ -19,6 +19,7 @@ struct hm_opts_t {
   uint64_t memory;
   double halloc_fraction, busy_fraction, roomy_fraction, sparse_fraction;
   int32_t sb_sz_sh, nheads, hash_mode, pad;
+  double hash_step_frac;
 };
 struct hm_op_t { uint32_t kind, arg; };  // kind 0 none, 1 malloc(arg = bytes), 2 free(arg = op index)
 
@@ -27,6 +28,7 @@ static Opts to_opts(const hm_opts_t* o) {
   r.memory = o->memory; r.halloc_fraction = o->halloc_fraction; r.busy_fraction = o->busy_fraction;
   r.roomy_fraction = o->roomy_fraction; r.sparse_fraction = o->sparse_fraction;
   r.sb_sz_sh = o->sb_sz_sh; r.nheads = o->nheads; r.hash_mode = o->hash_mode;
+  r.hash_step_frac = o->hash_step_frac;
   return r;
 }
 
@@ -100,6 +102,55 @@ int hm_replay(void* hv, const hm_op_t* ops, const uint32_t* iheads, int nsteps, 
       std::memcpy(recs + 4 * i, &frec[l], 16);
     }
   }
+  return 0;
+}
+
+// Concurrent mallocs: nthreads threads take the waves of `sizes` (64 lanes
+// each) in turn, wave w on thread w % nthreads and head set
+// (w / 4) % nheads, as blocks of 256 lanes map to the XCDs.  ptrs receives
+// arena offset + 1 (0 = NULL).  The threads race on the heap as waves do on
+// the GPU.
+int hm_malloc_threads(void* hv, int nthreads, const uint32_t* sizes, int n, uint64_t* ptrs) {
+  Heap* h = (Heap*)hv;
+  if (nthreads < 1 || n < 0) return -1;
+  const int nwaves = (n + (int)WAVE - 1) / (int)WAVE;
+  auto body = [&](int t) {
+    u32 nb[WAVE];
+    u64 out[WAVE];
+    for (int w = t; w < nwaves; w += nthreads) {
+      const int base = w * (int)WAVE, m = std::min<int>((int)WAVE, n - base);
+      const u64 act = m == 64 ? ~0ull : ((1ull << m) - 1);
+      for (int l = 0; l < m; ++l) nb[l] = sizes[base + l];
+      h->wave_malloc((u32)(w / 4) % h->nheads, act, nb, out, nullptr);
+      for (int l = 0; l < m; ++l) ptrs[base + l] = out[l];
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 0; t < nthreads; ++t) th.emplace_back(body, t);
+  for (auto& x : th) x.join();
+  return 0;
+}
+
+// Concurrent frees of ptrs (arena offset + 1, 0 skipped), split as above.
+int hm_free_threads(void* hv, int nthreads, const uint64_t* ptrs, int n) {
+  Heap* h = (Heap*)hv;
+  if (nthreads < 1 || n < 0) return -1;
+  const int nwaves = (n + (int)WAVE - 1) / (int)WAVE;
+  auto body = [&](int t) {
+    u64 p[WAVE];
+    for (int w = t; w < nwaves; w += nthreads) {
+      const int base = w * (int)WAVE, m = std::min<int>((int)WAVE, n - base);
+      u64 act = 0;
+      for (int l = 0; l < m; ++l) {
+        p[l] = ptrs[base + l];
+        if (p[l]) act |= 1ull << l;
+      }
+      if (act) h->wave_free(act, p, nullptr);
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 0; t < nthreads; ++t) th.emplace_back(body, t);
+  for (auto& x : th) x.join();
   return 0;
 }
 

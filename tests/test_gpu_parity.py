@@ -24,7 +24,7 @@ def say(msg):
 def model_opts(o: H.HallocOpts):
     return M.make_opts(memory=o.memory, halloc_fraction=o.halloc_fraction, busy_fraction=o.busy_fraction,
                        roomy_fraction=o.roomy_fraction, sparse_fraction=o.sparse_fraction,
-                       sb_sz_sh=o.sb_sz_sh, nheads=o.nheads, hash_mode=1)
+                       sb_sz_sh=o.sb_sz_sh, nheads=o.nheads, hash_mode=1, hash_step_frac=o.hash_step_frac)
 
 
 @pytest.fixture
@@ -59,13 +59,26 @@ def assert_same_state(heap, opts):
     assert tot == 0, det
 
 
-def test_geometry_matches_model(halloc):
-    opts = halloc(H.HallocOpts())
+@pytest.mark.parametrize("frac", [0.0, 0.05])
+def test_geometry_matches_model(halloc, frac):
+    with pytest.raises(H.HallocError):
+        H.Halloc.init(H.HallocOpts(cached_arena=2))
+    opts = halloc(H.HallocOpts(hash_step_frac=frac))
     g = H.Halloc.geometry()
-    say(f"geometry nsbs={g['nsbs']} nheads={g['nheads']}")
+    say(f"geometry nsbs={g['nsbs']} nheads={g['nheads']} hash_step_frac={frac}")
     t = M.size_table(model_opts(opts))
     assert (g["size_table"] == t).all()
     assert g["nsbs"] == int(0.75 * (512 << 20)) // (4 << 20)
+
+
+def test_replay_wave_tuned_stride(halloc):
+    say("replay one wave, hash_step_frac 0.11")
+    opts = halloc(H.HallocOpts(memory=512 << 20, sb_sz_sh=22, nheads=8, hash_step_frac=0.11))
+    rng = np.random.default_rng(17)
+    ops = random_ops(rng, 200, 64, BLOCK_SIZES + [20, 40, 200, 700, 2500])
+    recs_g, recs_m, heap = replay_both(opts, ops)
+    assert (recs_g == recs_m).all()
+    assert_same_state(heap, opts)
 
 
 @pytest.mark.parametrize("seed", [0, 1])
@@ -157,9 +170,10 @@ def test_replay_large_requests(halloc):
     assert_same_state(heap, opts)
 
 
-def test_plain_workload_invariants(halloc):
-    say("plain 2^18 lanes x 2 iters")
-    opts = halloc(H.HallocOpts())
+@pytest.mark.parametrize("cached", [0, 1])
+def test_plain_workload_invariants(halloc, cached):
+    say(f"plain 2^18 lanes x 2 iters cached_arena={cached}")
+    opts = halloc(H.HallocOpts(cached_arena=cached))
     r = H.Halloc.run_plain(nthreads=1 << 18, bs=256, size=16, iters=2, warmup=1, launches=3)
     assert r["errors"] == 0 and r["nulls"] == 0
     d = H.Halloc.dump()

@@ -24,9 +24,26 @@ def say(msg):
 def check_empty_heap(opts):
     g = H.Halloc.geometry()
     assert H.Halloc.free_bytes() == g["arena_bytes"]
-    mo = M.make_opts(memory=opts.memory, sb_sz_sh=opts.sb_sz_sh, nheads=opts.nheads)
+    mo = M.make_opts(memory=opts.memory, sb_sz_sh=opts.sb_sz_sh, nheads=opts.nheads,
+                     hash_step_frac=opts.hash_step_frac)
     tot, detail = M.check_dump(mo, H.Halloc.dump())
     assert tot == 0, detail
+
+
+def report_bad(r, tag, ln, arr):
+    """Prints what the damaged requests held: whose token sat where theirs should."""
+    idx = np.nonzero(r["bad"])[0]
+    if len(idx) == 0:
+        return
+    by_tag = {int(t): i for i, t in enumerate(tag)}
+    say(f"{len(idx)} requests with damaged tokens, {int(r['bad'].sum())} tokens")
+    for i in idx[:12]:
+        x = int(r["bad_first"][i] & 0xFFFFFFFF)
+        y = int(r["bad_first"][i] >> 32)
+        o = by_tag.get(x, -1)
+        say(f"  req {i} len {ln[i]} bad {r['bad'][i]} first ({r['first_ns'][i]}..{r['done_ns'][i]} ns) "
+            f"holds tag {x:#x} (req {o}, len {ln[o] if o >= 0 else '-'}) token {y}"
+            + (f" ({r['first_ns'][o]}..{r['done_ns'][o]} ns)" if o >= 0 else ""))
 
 
 @pytest.fixture
@@ -51,6 +68,8 @@ def test_serve_mixed_and_alone(heap):
     assert r["completed"] == len(arr) and r["admitted"] == len(arr)
     assert r["nulls"] == 0
     assert (r["produced"] == ln).all()
+    report_bad(r, tag, ln, arr)
+    assert (r["bad"] == 0).all()
     assert (r["checksum"] == serve_checksums(tag, ln)).all()
     assert r["pages"] == int(pages_for(ln).sum())
     assert (r["first_ns"] + 1000 >= arr).all()  # one 100 MHz tick of rounding
@@ -68,15 +87,20 @@ def test_serve_mixed_and_alone(heap):
     check_empty_heap(opts)
 
 
-def test_serve_pool_runs_dry(heap):
-    opts = heap(H.HallocOpts(memory=64 << 20, sb_sz_sh=21, nheads=8))
+@pytest.mark.parametrize("nheads,cached", [(1, 0), (8, 0), (1, 1), (8, 1)])
+def test_serve_pool_runs_dry(heap, nheads, cached):
+    """With no watermark the heap runs dry; slabs then move between XCDs all
+    the time, and no token may come back damaged in either arena mode."""
+    opts = heap(H.HallocOpts(memory=64 << 20, sb_sz_sh=21, nheads=nheads, cached_arena=cached))
     arr, ln, tag = serving_requests(4000, rate_per_ms=4000, median=2048, sigma=1.0, seed=2)
-    say(f"serve dry nreq={len(arr)} total_pages={int(pages_for(ln).sum())}")
-    r = H.Halloc.run_serve(arr, ln, tag, slots=16384, watermark=1 << 20)
+    say(f"serve dry nheads={nheads} cached_arena={cached} nreq={len(arr)} total_pages={int(pages_for(ln).sum())}")
+    r = H.Halloc.run_serve(arr, ln, tag, slots=16384, watermark=0)
     assert r["completed"] == len(arr)
     assert r["nulls"] > 0, "the workload should exhaust a 48 MiB heap"
     assert (r["produced"] <= ln).all()
     assert ((r["produced"] < ln).sum()) == r["nulls"]
+    report_bad(r, tag, ln, arr)
+    assert (r["bad"] == 0).all()
     assert (r["checksum"] == serve_checksums(tag, r["produced"])).all()
     assert r["pages"] == int(pages_for(r["produced"]).sum())
     check_empty_heap(opts)

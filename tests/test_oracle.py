@@ -392,3 +392,38 @@ def test_serve_checksum_helper_by_loop():
             s += int(_hash3(tg, t, 0x5E7A11C3)[0])
         want.append(s & 0xFFFFFFFF)
     assert list(serve_checksums(tags, prod)) == want
+
+
+def test_model_threads_race_mixed_sizes():
+    """The model under real threads (the GPU's concurrent mode): 2^16 mixed
+    sizes over 8 head sets in 96 slabs; no NULL, invariants hold before and
+    after the frees, and every count returns to zero."""
+    rng = np.random.default_rng(21)
+    sizes = rng.integers(1, 3073, 1 << 16).astype(np.uint32)
+    o = M.make_opts(nheads=8)
+    h = M.ModelHeap(o)
+    p = h.malloc_threads(sizes, nthreads=32)
+    assert (p != 0).all()
+    assert M.check_dump(o, h.dump())[0] == 0
+    h.free_threads(p, nthreads=32)
+    d = h.dump()
+    assert M.check_dump(o, d)[0] == 0
+    assert ((d[:h.nsbs] >> np.uint64(32)) == 0).all()
+
+
+@pytest.mark.parametrize("frac", [0.01, 0.05, 0.11, 0.3])
+def test_hash_step_frac_prime_bound(frac):
+    """A tuned stride keeps the table's shape: the prime is the largest below
+    frac * nblocks that divides neither the bound nor nblocks."""
+    t0 = M.size_table(M.make_opts())
+    t = M.size_table(M.make_opts(hash_step_frac=frac))
+    f = {n: i for i, n in enumerate(M.SIZE_TABLE_FIELDS)}
+    same = [n for n in M.SIZE_TABLE_FIELDS if n != "hash_step"]
+    assert (t[:, [f[n] for n in same]] == t0[:, [f[n] for n in same]]).all()
+    for row in t:
+        nb, k, hs = int(row[f["nblocks"]]), int(row[f["ncib"]]), int(row[f["hash_step"]])
+        bound = int(frac * nb)
+        p = hs // k
+        P = set(primes_below(bound))
+        assert hs % k == 0 and p in P and bound % p and nb % p
+        assert not any(q in P and bound % q and nb % q for q in range(p + 1, bound))
