@@ -8,7 +8,22 @@ back and calls hafree (tst/perf/priv-throughput/priv-throughput.cu:14-27 with
 the write/verify of tst/corr/test/test.cu:84-95).  W untimed steps, then K
 timed steps; the device is synchronized before and after the timed region.
 
-Also reported:
+Also reported (each on its own freshly initialized heap):
+  batching         4096 random-graph builds (n_i log-uniform on [1k, 64k]
+                   nodes; a 16 B header plus a Poisson(8) int32 adjacency
+                   array per node), 4 GiB pool: plain per-call kernels vs
+                   one persistent hipGraph launch using hamalloc_batch, with
+                   the autotuned options (halloc_amd/tuned) when present
+  serving          continuous-batching request loop: Poisson arrivals at
+                   2000 req/ms, log-normal lengths (median 512, sigma 1, cap
+                   32768 tokens of 16 B) growing in 256 B pages, <= 65536
+                   live sequences, 16 GiB pool, watermark admission;
+                   p50/p99 page-malloc latency and request latency
+  phase            2^20 lanes x 4 slots, sizes 8*U{2..32} B, 5 phases
+                   between fill 0.91 and 0.09 of a 512 MiB pool
+  sweep            2^19 lanes, U{1..8} requests log-uniform on [16 B, 3 KiB]
+                   plus 1% on (3 KiB, 64 KiB], random-order frees, 1 GiB
+                   pool; per size class; the large path timed alone
   roofline         dominant kernel (the plain workload kernel) against HBM:
                    algorithmic bytes per pair = 4.05 reference-scheme atomics
                    x 8 B + 32 B payload = 64.4 B; traffic from the committed
@@ -58,6 +73,122 @@ def cpu_baseline(budget_s: float):
     }
 
 
+def pct(a, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(a, dtype=np.float64), q)) if len(a) else None
+
+
+def bench_batching(H, njobs=4096, memory=4 << 30, seed=0):
+    import numpy as np
+    from halloc_amd import autotune
+    from halloc_amd.workloads import graph_sizes
+    cfg = autotune.load()
+    nn = graph_sizes(np.random.default_rng(seed), njobs)
+    H.Halloc.init(autotune.opts_of(cfg, memory))
+    r = H.Halloc.run_graphs(nn, batch=cfg["batch"], waves=cfg["waves"], repeats=3)
+    H.Halloc.shutdown()
+    H.Halloc.init(H.HallocOpts(memory=memory))
+    d = H.Halloc.run_graphs(nn, batch=8, waves=2048, repeats=3)
+    H.Halloc.shutdown()
+    nodes = r["nodes"]
+    pairs = 2 * nodes  # a header and an adjacency array per node
+    return {
+        "workload": f"{njobs} random-graph builds, n_i log-uniform on [1k, 64k], seed {seed}, "
+                    f"{memory >> 30} GiB pool",
+        "nodes": nodes, "pairs": pairs,
+        "batched_ms": r["batch_ms"], "plain_ms": r["plain_ms"], "plain_launches": r["plain_launches"],
+        "batched_pairs_per_s": pairs / (r["batch_ms"] / 1e3),
+        "plain_pairs_per_s": pairs / (r["plain_ms"] / 1e3),
+        "batched_vs_plain": r["plain_ms"] / r["batch_ms"],
+        "tuned": cfg, "tuned_from_file": os.path.exists(autotune.TUNED),
+        "default_opts": {"batched_ms": d["batch_ms"], "plain_ms": d["plain_ms"],
+                         "batched_vs_plain": d["plain_ms"] / d["batch_ms"]},
+        "violations": r["nulls"] + r["errors"] + d["nulls"] + d["errors"],
+    }
+
+
+def bench_serving(H, nreq=100000, memory=16 << 30, seed=0):
+    import numpy as np
+    from halloc_amd.workloads import pages_for, serving_requests
+    arr, ln, tag = serving_requests(nreq, seed=seed)
+    H.Halloc.init(H.HallocOpts(memory=memory))
+    g = H.Halloc.geometry()
+    wm = g["arena_bytes"] // 16
+    r = H.Halloc.run_serve(arr, ln, tag, slots=65536, watermark=wm, sample_every=4)
+    H.Halloc.shutdown()
+    lat = r["lat_ns"]
+    req_ms = (r["done_ns"].astype(np.float64) - arr.astype(np.float64)) / 1e6
+    queue_us = (r["first_ns"].astype(np.float64) - arr.astype(np.float64)) / 1e3
+    pages = r["pages"]
+    return {
+        "workload": f"{nreq} requests, Poisson 2000 req/ms, log-normal length (median 512, sigma 1, cap 32768 "
+                    f"tokens x 16 B), 256 B pages, 65536 slots, {memory >> 30} GiB pool, watermark {wm >> 20} MiB",
+        "completed": r["completed"], "pages": pages, "kernel_ms": r["kernel_ms"],
+        "pairs_per_s": pages / (r["kernel_ms"] / 1e3),
+        "p50_alloc_ns": pct(lat, 50), "p99_alloc_ns": pct(lat, 99), "alloc_samples": int(len(lat)),
+        "p50_request_ms": pct(req_ms, 50), "p99_request_ms": pct(req_ms, 99),
+        "p99_admission_wait_us": pct(queue_us, 99),
+        "stalls": r["stalls"],
+        "violations": int(r["nulls"]) + int(r["bad"].sum()) + int((r["produced"] != ln).sum())
+                      + int(r["completed"] != nreq) + int(pages != int(pages_for(ln).sum())),
+    }
+
+
+def bench_phase(H, memory=512 << 20):
+    H.Halloc.init(H.HallocOpts(memory=memory))
+    r = H.Halloc.run_phase(nthreads=1 << 20, slots=4, nphases=5, max_steps=400)
+    H.Halloc.shutdown()
+    pairs = (r["mallocs"] + r["frees"]) / 2
+    return {"workload": "2^20 lanes x 4 slots, sizes 8*U{2..32} B, p 0.95/0.05, fill 0.91/0.09, 5 phases, "
+                        f"{memory >> 20} MiB pool",
+            "pairs_per_s": pairs / (r["kernel_ms"] / 1e3), "mallocs": r["mallocs"], "frees": r["frees"],
+            "nulls": r["nulls"], "phase_steps": r["phase_steps"], "kernel_ms": r["kernel_ms"],
+            "peak_fill": float(r["trace"][:, 1].max()) if len(r["trace"]) else None,
+            "violations": r["errors"] + (r["mallocs"] != r["frees"])}
+
+
+def bench_sweep(H, memory=1 << 30, seed=0, rounds=8):
+    """2^19 lanes in `rounds` rounds, so that each round's live set (and its
+    large requests on the device heap) fits the pool; per size class, as many
+    blocks as fill half the slab heap."""
+    import numpy as np
+    from halloc_amd.workloads import sweep_sizes
+    rng = np.random.default_rng(seed)
+    H.Halloc.init(H.HallocOpts(memory=memory))
+    arena = H.Halloc.geometry()["arena_bytes"]
+    nreq, ms, viol, nulls = 0, 0.0, 0, 0
+    nbig, ms_big, nulls_big = 0, 0.0, 0
+    for _ in range(rounds):
+        sizes = sweep_sizes(rng, (1 << 19) // rounds)
+        small = np.where(sizes <= 3072, sizes, 0).astype(np.uint32)
+        big = np.where(sizes > 3072, sizes, 0).astype(np.uint32)
+        order = rng.permutation(sizes.size).astype(np.uint32)
+        r = H.Halloc.run_lists(small, order)
+        nreq += int((small != 0).sum())
+        ms += r["malloc_ms"] + r["free_ms"]
+        viol += r["errors"]
+        nulls += r["nulls"]
+        b = H.Halloc.run_lists(big, order)
+        nbig += int((big != 0).sum())
+        ms_big += b["malloc_ms"] + b["free_ms"]
+        viol += b["errors"]
+        nulls_big += b["nulls"]
+    out = {"workload": f"2^19 lanes in {rounds} rounds, U{{1..8}} requests log-uniform on [16 B, 3 KiB] + 1% "
+                       f"on (3 KiB, 64 KiB] (device heap, timed apart), random-order frees, "
+                       f"{memory >> 30} GiB pool",
+           "requests": nreq, "pairs_per_s": nreq / (ms / 1e3), "nulls": nulls,
+           "large_requests": nbig, "large_path_pairs_per_s": nbig / (ms_big / 1e3), "large_path_nulls": nulls_big,
+           "violations": viol + nulls, "per_class_pairs_per_s": {}}
+    for bsz in (16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072):
+        n = min(1 << 19, arena // 2 // bsz // 4)
+        sz = np.full((4, n), bsz, dtype=np.uint32)
+        c = H.Halloc.run_lists(sz, None)
+        out["per_class_pairs_per_s"][str(bsz)] = sz.size / ((c["malloc_ms"] + c["free_ms"]) / 1e3)
+        out["violations"] += c["nulls"] + c["errors"]
+    H.Halloc.shutdown()
+    return out
+
+
 def traffic_from_profiles():
     p = os.path.join(ROOT, "profiles", "plain_16B_counters.json")
     if not os.path.exists(p):
@@ -77,6 +208,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds for the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-probe", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="only the plain 16 B workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the allocator keeps per-device "
@@ -138,6 +270,15 @@ def main():
                                           "rate of 64-bit atomics on distinct cache lines",
                                   "probe": {"lane_random": float(probe[0]), "wave_run": float(probe[1]),
                                             "one_word": float(probe[2])}}
+    if not args.no_extras:
+        bt = bench_batching(H)
+        sv = bench_serving(H)
+        out["batched_vs_plain"] = bt["batched_vs_plain"]
+        out["p99_serve_alloc_ns"] = sv["p99_alloc_ns"]
+        out["batching"] = bt
+        out["serving"] = sv
+        out["phase"] = bench_phase(H)
+        out["sweep"] = bench_sweep(H)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_budget)
     print(json.dumps(out))

@@ -77,6 +77,8 @@ struct Params {
   u32* cached;
   u32* head_lock;
   u32* ticket;
+  u32* class_lock;  // one per size class, serializes head searches of its pairs
+  u32* search_ep;   // per pair: searches done (bits 31..1) and whether the last failed (bit 0)
   u32 nsbs, sb_sz, sb_sz_sh, nheads;
   u32 nsw, bit_words, nib_words, initialized;
   u32 cached_arena, pad;  // cached_arena: hafree releases before publishing

@@ -368,23 +368,34 @@ again:
 // last used.  A head replaced since then is returned as it is; otherwise the
 // wave that takes the head lock replaces a missing or full head, and a wave
 // that finds the lock taken waits for the head to change or the lock to be
-// released and looks again.  SB_NONE only comes back from a wave that held
-// the lock itself and found no slab.
+// released.  A waiter whose wait saw a slab search fail gets SB_NONE too
+// rather than repeating the search; otherwise it looks again.
 __device__ __noinline__ u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen) {
   const SizeInfo& s = ha_p.si[size_id];
   u32* hp = hot(ha_p.head, ihead, size_id);
   u32* lp = hot(ha_p.head_lock, ihead, size_id);
   u32* cp = hot(ha_p.cached, ihead, size_id);
+  u32* ep = hot(ha_p.search_ep, ihead, size_id);
   for (;;) {
     u32 cur = ld32(hp);
     if (cur != seen && cur != SB_NONE) return cur;
+    const u32 e0 = ld32(ep);
     if (try_lock(lp)) {
       cur = ld32(hp);
       u32 nh;
       if (cur == SB_NONE || ctr_count(ld64(ctr_of(cur & ~HEAD_EXT))) >= head_limit(cur, s)) {
         nh = ld32(cp);
         st32(cp, SB_NONE);
-        if (nh == SB_NONE) nh = find_sb_for_size(size_id, s.chunk_id, ihead, true);
+        if (nh == SB_NONE) {
+          // one head search per class at a time: a pair that follows sees the
+          // head the one before it took and can share it instead of taking
+          // another slab
+          u32* cl = ha_p.class_lock + size_id * HOT_STRIDE;
+          if (ha_p.nheads > 1) lock(cl);
+          nh = find_sb_for_size(size_id, s.chunk_id, ihead, true);
+          if (ha_p.nheads > 1) unlock(cl);
+          st32(ep, (((ld32(ep) >> 1) + 1) << 1) | (nh == SB_NONE ? 1u : 0u));
+        }
         if (nh != SB_NONE) {
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
           st32(hp, nh);
@@ -405,6 +416,8 @@ __device__ __noinline__ u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen) {
     }
     while (ld32(hp) == cur && ld32(lp) != 0) __builtin_amdgcn_s_sleep(2);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const u32 e1 = ld32(ep);
+    if (e1 != e0 && (e1 & 1u) && ld32(hp) == cur) return SB_NONE;
   }
 }
 
@@ -976,6 +989,8 @@ static int init_locked(const ha_opts_c& o) {
   u64 o_cached = off; off = align_up(off + hot_bytes, 256);
   u64 o_hlock = off; off = align_up(off + hot_bytes, 256);
   u64 o_ticket = off; off = align_up(off + hot_bytes, 256);
+  u64 o_clock = off; off = align_up(off + (u64)NSIZES * HOT_STRIDE * 4, 256);
+  u64 o_sep = off; off = align_up(off + hot_bytes, 256);
   void* arena = nullptr;
   int rc = arena_get(dev, P.arena_bytes, o.cached_arena != 0, &arena);
   if (rc) return rc;
@@ -996,6 +1011,8 @@ static int init_locked(const ha_opts_c& o) {
   P.cached = (u32*)(m + o_cached);
   P.head_lock = (u32*)(m + o_hlock);
   P.ticket = (u32*)(m + o_ticket);
+  P.class_lock = (u32*)(m + o_clock);
+  P.search_ep = (u32*)(m + o_sep);
   P.initialized = 1;
   HA_HIP(hipMemset(meta, 0, off));
   u32 n = std::max(std::max(P.nsbs, P.nsw), P.nheads * NSIZES);

@@ -6,6 +6,7 @@
 // results back.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -315,23 +316,29 @@ __device__ __forceinline__ u32 hash3(u32 a, u32 b, u32 c) {
   return x;
 }
 // One step of the phase workload (tst/perf/phase-throughput/phase-throughput.cu:18-50):
-// each of a lane's slots, if empty, allocates with probability p_alloc a block
-// of 8*U{lo/8..hi/8} bytes and stamps it; if full, frees it with probability
-// p_free after checking the stamp.  stats: mallocs, frees, nulls, errors,
-// bytes allocated, bytes freed.
-__global__ void phase_step_k(void** ptrs, u32* sizes, int n, int slots, u32 seed, u32 step, float p_alloc,
-                             float p_free, u32 lo8, u32 hi8, unsigned long long* stats) {
+// each lane takes one action per step: with probability p_alloc it
+// allocates a block of 8*U{lo8..hi8} bytes into its next empty slot and
+// stamps it, otherwise it checks the stamp of its next live slot and frees
+// it.  stats: mallocs, frees, nulls, errors, bytes allocated, bytes freed,
+// summed per wave before one lane adds them.
+__device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
+  for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__global__ void __launch_bounds__(256) phase_step_k(void** ptrs, u32* sizes, int n, int slots, u32 seed, u32 step,
+                                                    float p_alloc, u32 lo8, u32 hi8, unsigned long long* stats) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
   u32 nm = 0, nf = 0, nn = 0, ne = 0;
   unsigned long long ba = 0, bf = 0;
-  for (int s = 0; s < slots; ++s) {
-    const size_t k = (size_t)s * n + i;
-    void* p = ptrs[k];
-    const u32 r = hash3(seed, (u32)i * 64u + (u32)s, step);
+  if (i < n) {
+    const u32 r = hash3(seed, (u32)i, step);
     const float u = (float)(r >> 8) * (1.0f / 16777216.0f);
-    if (!p) {
-      if (u < p_alloc) {
+    const int s0 = (int)(hash3(r, seed, 11u) % (u32)slots);
+    if (u < p_alloc) {
+      for (int t = 0; t < slots; ++t) {
+        const int s = (s0 + t) % slots;
+        const size_t k = (size_t)s * n + i;
+        if (ptrs[k]) continue;
         const u32 sz = 8u * (lo8 + hash3(r, seed, 7u) % (hi8 - lo8 + 1));
         void* q = hamalloc(sz);
         if (q) {
@@ -343,29 +350,36 @@ __global__ void phase_step_k(void** ptrs, u32* sizes, int n, int slots, u32 seed
         } else {
           nn++;
         }
+        break;
       }
-    } else if (u < p_free) {
-      const u32 sz = sizes[k];
-      if (*(const u32*)p != ((u32)i ^ (sz << 20))) ne++;
-      hafree(p);
-      ptrs[k] = nullptr;
-      nf++;
-      bf += sz;
+    } else {
+      for (int t = 0; t < slots; ++t) {
+        const int s = (s0 + t) % slots;
+        const size_t k = (size_t)s * n + i;
+        void* p = ptrs[k];
+        if (!p) continue;
+        const u32 sz = sizes[k];
+        if (*(const u32*)p != ((u32)i ^ (sz << 20))) ne++;
+        hafree(p);
+        ptrs[k] = nullptr;
+        nf++;
+        bf += sz;
+        break;
+      }
     }
   }
-  if (nm) atomicAdd(&stats[0], (unsigned long long)nm);
-  if (nf) atomicAdd(&stats[1], (unsigned long long)nf);
-  if (nn) atomicAdd(&stats[2], (unsigned long long)nn);
-  if (ne) atomicAdd(&stats[3], (unsigned long long)ne);
-  if (ba) atomicAdd(&stats[4], ba);
-  if (bf) atomicAdd(&stats[5], bf);
+  const unsigned long long v[6] = {wave_sum64(nm), wave_sum64(nf), wave_sum64(nn), wave_sum64(ne), wave_sum64(ba),
+                                   wave_sum64(bf)};
+  if ((threadIdx.x & 63) == 0)
+    for (int c = 0; c < 6; ++c)
+      if (v[c]) atomicAdd(&stats[c], v[c]);
 }
 // frees what is left (free_rest_k, tst/common/common.h:403-412)
-__global__ void phase_rest_k(void** ptrs, u32* sizes, int n, int slots, unsigned long long* stats) {
+__global__ void __launch_bounds__(256) phase_rest_k(void** ptrs, u32* sizes, int n, int slots,
+                                                    unsigned long long* stats) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
   u32 nf = 0, ne = 0;
-  for (int s = 0; s < slots; ++s) {
+  for (int s = 0; i < n && s < slots; ++s) {
     const size_t k = (size_t)s * n + i;
     void* p = ptrs[k];
     if (!p) continue;
@@ -374,8 +388,11 @@ __global__ void phase_rest_k(void** ptrs, u32* sizes, int n, int slots, unsigned
     ptrs[k] = nullptr;
     nf++;
   }
-  if (nf) atomicAdd(&stats[1], (unsigned long long)nf);
-  if (ne) atomicAdd(&stats[3], (unsigned long long)ne);
+  const unsigned long long wf = wave_sum64(nf), we = wave_sum64(ne);
+  if ((threadIdx.x & 63) == 0) {
+    if (wf) atomicAdd(&stats[1], wf);
+    if (we) atomicAdd(&stats[3], we);
+  }
 }
 
 // ---------------------------------------------------------------- graphs
@@ -402,16 +419,15 @@ __device__ __forceinline__ u32 poisson8(u32 seed, u32 job, u32 node) {
   }
   return k;
 }
-__device__ __forceinline__ u32 adj_bytes(u32 deg) { return deg ? 4u * deg : 0u; }
+__device__ __forceinline__ u32 adj_bytes(u32 deg) { return deg ? 4u * deg : 16u; }
 __device__ __forceinline__ void fill_node(NodeHdr* h, u32* a, u32 seed, u32 job, u32 node, u32 deg, u32 n) {
   for (u32 e = 0; e < deg; ++e) a[e] = gh(seed, job, node, e) % n;
   h->node = node;
   h->deg = deg;
   h->adj = a;
 }
-__device__ __forceinline__ u32 check_node(const NodeHdr* h, u32 seed, u32 job, u32 node, u32 n) {
-  const u32 deg = poisson8(seed, job, node);
-  if (h->node != node || h->deg != deg || (deg && !h->adj)) return 1;
+__device__ __forceinline__ u32 check_node(const NodeHdr* h, u32 seed, u32 job, u32 node, u32 n, u32 deg) {
+  if (h->node != node || h->deg != deg || !h->adj) return 1;
   u32 bad = 0;
   for (u32 e = 0; e < deg; ++e) bad += h->adj[e] != gh(seed, job, node, e) % n;
   return bad ? 1 : 0;
@@ -451,116 +467,215 @@ __global__ void graph_plain_free_k(const u64* offs, const u32* nnodes, u32 njobs
   NodeHdr* h = table[gi];
   if (!h) return;
   const u32 job = job_of(offs, njobs, gi), node = (u32)(gi - offs[job]);
-  if (check_node(h, seed, job, node, nnodes[job])) atomicAdd(&stats[1], 1ull);
-  hafree(h->adj);
+  if (check_node(h, seed, job, node, nnodes[job], poisson8(seed, job, node))) {
+    atomicAdd(&stats[1], 1ull);  // a damaged header's adjacency pointer is not trusted
+  } else {
+    hafree(h->adj);
+  }
   hafree(h);
 }
 
-__global__ void reset_counter_k(u32* c) { *c = 0; }
+// Batched persistent form.  Work items are chunks of 64*B nodes of one job
+// (node = chunk*64*B + j*64 + lane, j < B).  The host lays the items out as
+// one sequence: every build item in job order, and each job's free items (check
+// and free one chunk) a fixed lag after the job's last build item.  Waves take
+// GRAPH_GRAB items at a time from one counter; a free item waits until its
+// job's build items, all taken earlier by running waves, are done.  Inside a
+// chunk, allocation is class-major: all of the wave's adjacency arrays of one
+// size class come from ONE hamalloc_batch call in which every lane takes an
+// equal share, handed to the nodes through LDS; the frees use the same layout,
+// so each round frees a contiguous run.
+constexpr u32 GRAPH_GRAB = 2;
 
-// batched persistent form: waves take jobs from a queue; lane l of a wave
-// handles nodes chunk + l*B + j (j < B).  Headers: one hamalloc_batch per chunk.
-// Adjacency arrays: one hamalloc_batch per size class present in the wave,
-// frees in the same class order.
+__global__ void graph_reset_k(u32* next, u32* built, u32 njobs) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *next = 0;
+  for (u32 k = i; k < njobs; k += gridDim.x * blockDim.x) built[k] = 0;
+}
+
+__device__ __forceinline__ u32 rfl32(u32 v) { return (u32)__builtin_amdgcn_readfirstlane((int)v); }
+
+// LDS of one wave is written and read back by other lanes of the same wave
+__device__ __forceinline__ void wave_lds_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// the wave's nodes of class c, ranked class-major; T = how many
 template <int B>
-__global__ void __launch_bounds__(256) graph_batch_k(const u64* offs, const u32* nnodes, u32 njobs, u32 seed,
-                                                     NodeHdr** table, u32* job_ctr,
-                                                     unsigned long long* stats) {
+__device__ __forceinline__ u32 class_ranks(const u32* cls, u32 cnt, u32 c, u64 lt, u32* rank) {
+  u32 T = 0;
+#pragma unroll
+  for (int j = 0; j < B; ++j) {
+    const u64 m = __ballot((u32)j < cnt && cls[j] == c);
+    rank[j] = T + (u32)__builtin_popcountll(m & lt);
+    T += (u32)__builtin_popcountll(m);
+  }
+  return T;
+}
+
+template <int B>
+__device__ __forceinline__ u32 classes_present(const u32* cls, u32 cnt) {
+  u32 mine = 0;
+#pragma unroll
+  for (int j = 0; j < B; ++j)
+    if ((u32)j < cnt) mine |= 1u << cls[j];
+  u32 all = 0;
+  for (u32 c = 0; c < ha::NSIZES; ++c)
+    if (__ballot((mine >> c) & 1u)) all |= 1u << c;
+  return all;
+}
+
+template <int B>
+__device__ void graph_chunk_build(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr** tab, void** lds, u32 me, u64 lt,
+                                  u32& nulls) {
+  const u32 base = chunk * 64u * B;
+  u32 deg[B], cls[B];
+  void* hdr[B];
+  void* adj[B];
+  u32 cnt = 0;  // this lane's nodes: a prefix of j
+#pragma unroll
+  for (int j = 0; j < B; ++j) {
+    const u32 v = base + (u32)j * 64u + me;
+    cnt += v < n;
+    deg[j] = v < n ? poisson8(seed, job, v) : 0u;
+    cls[j] = ha::size_id_from_nbytes(adj_bytes(deg[j]));
+    hdr[j] = nullptr;
+    adj[j] = nullptr;
+  }
+  if (cnt) hamalloc_batch(sizeof(NodeHdr), (int)cnt, hdr);
+  for (u32 cs = classes_present<B>(cls, cnt); cs; cs &= cs - 1) {
+    const u32 c = (u32)__builtin_ctz(cs);
+    u32 rank[B];
+    const u32 T = class_ranks<B>(cls, cnt, c, lt, rank);
+    const u32 q = (T + 63) / 64, lo = me * q;  // lane share: [lo, lo + k)
+    const u32 k = lo >= T ? 0u : (T - lo < q ? T - lo : q);
+    void* tmp[B];
+#pragma unroll
+    for (int t = 0; t < B; ++t) tmp[t] = nullptr;
+    if (k) hamalloc_batch(ha::ha_p.si[c].block_sz, (int)k, tmp);
+#pragma unroll
+    for (int t = 0; t < B; ++t)
+      if ((u32)t < k) lds[lo + t] = tmp[t];
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < B; ++j)
+      if ((u32)j < cnt && cls[j] == c) adj[j] = lds[rank[j]];
+    wave_lds_sync();
+  }
+#pragma unroll
+  for (int j = 0; j < B; ++j) {
+    if ((u32)j >= cnt) continue;
+    const u32 v = base + (u32)j * 64u + me;
+    NodeHdr* h = (NodeHdr*)hdr[j];
+    u32* a = (u32*)adj[j];
+    if (h && a) {
+      fill_node(h, a, seed, job, v, deg[j], n);
+    } else {
+      nulls++;
+      if (a) hafree(a);
+      if (h) hafree(h);
+      h = nullptr;
+    }
+    tab[v] = h;
+  }
+}
+
+template <int B>
+__device__ void graph_chunk_free(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr** tab, void** lds, u32 me, u64 lt,
+                                 u32& bad) {
+  const u32 base = chunk * 64u * B;
+  u32 cls[B];
+  void* hdr[B];
+  void* adj[B];
+  u32 cnt = 0;
+#pragma unroll
+  for (int j = 0; j < B; ++j) {
+    const u32 v = base + (u32)j * 64u + me;
+    cnt += v < n;
+    const u32 deg = v < n ? poisson8(seed, job, v) : 0u;
+    cls[j] = ha::size_id_from_nbytes(adj_bytes(deg));
+    NodeHdr* h = v < n ? tab[v] : nullptr;
+    hdr[j] = h;
+    adj[j] = nullptr;
+    if (h) {
+      if (check_node(h, seed, job, v, n, deg)) bad++;  // a damaged node's adjacency is not trusted
+      else adj[j] = h->adj;
+    }
+  }
+  for (u32 cs = classes_present<B>(cls, cnt); cs; cs &= cs - 1) {
+    const u32 c = (u32)__builtin_ctz(cs);
+    u32 rank[B];
+    const u32 T = class_ranks<B>(cls, cnt, c, lt, rank);
+#pragma unroll
+    for (int j = 0; j < B; ++j)
+      if ((u32)j < cnt && cls[j] == c) lds[rank[j]] = adj[j];
+    wave_lds_sync();
+    const u32 q = (T + 63) / 64, lo = me * q;
+    const u32 k = lo >= T ? 0u : (T - lo < q ? T - lo : q);
+    void* tmp[B];
+#pragma unroll
+    for (int t = 0; t < B; ++t) tmp[t] = (u32)t < k ? lds[lo + t] : nullptr;
+    wave_lds_sync();
+    if (k) hafree_batch(tmp, (int)k);
+  }
+  if (cnt) hafree_batch(hdr, (int)cnt);
+}
+
+template <int B>
+__global__ void __launch_bounds__(256) graph_batch_k(const u64* seq, u32 nseq, const u32* coffs, const u64* offs,
+                                                     const u32* nnodes, u32 seed, NodeHdr** table, u32* next,
+                                                     u32* built, unsigned long long* stats) {
+  __shared__ void* lds_all[4][64 * B];
+  void** lds = lds_all[threadIdx.x >> 6];
   const u32 me = threadIdx.x & 63;
+  const u64 lt = me ? (~0ull >> (64 - me)) : 0ull;
   u32 nulls = 0, bad = 0;
   for (;;) {
-    u32 job = 0;
-    if (me == 0) job = atomicAdd(job_ctr, 1u);
-    job = (u32)__builtin_amdgcn_readfirstlane((int)job);
-    if (job >= njobs) break;
-    const u32 n = nnodes[job];
-    NodeHdr** tab = table + offs[job];
-    for (int pass = 0; pass < 2; ++pass) {  // 0: build, 1: check and free
-      for (u32 chunk = 0; chunk < n; chunk += 64u * B) {
-        const u32 first = chunk + me * B;
-        const u32 cnt = first >= n ? 0u : (n - first < (u32)B ? n - first : (u32)B);
-        u32 deg[B], cls[B];
-        u64 classes = 0;
-        for (int j = 0; j < B; ++j) {
-          deg[j] = (u32)j < cnt ? poisson8(seed, job, first + j) : 0u;
-          cls[j] = ha::size_id_from_nbytes(adj_bytes(deg[j]));
-          if ((u32)j < cnt) classes |= 1ull << cls[j];
-        }
-        u64 all = 0;  // classes present in the wave (bits 0..15)
-        for (u32 c = 0; c < ha::NSIZES; ++c)
-          if (__ballot((classes >> c) & 1)) all |= 1ull << c;
-        if (pass == 0) {
-          void* hdr[B];
-          void* adj[B];
-          void* tmp[B];
-          for (int j = 0; j < B; ++j) { hdr[j] = nullptr; adj[j] = nullptr; }
-          if (cnt) hamalloc_batch(sizeof(NodeHdr), (int)cnt, hdr);
-          for (u64 cs = all; cs; cs &= cs - 1) {
-            const u32 c = (u32)__builtin_ctzll(cs);
-            u32 nc = 0;
-            for (int j = 0; j < B; ++j) nc += ((u32)j < cnt && cls[j] == c);
-            if (nc) {
-              hamalloc_batch(ha::ha_p.si[c].block_sz, (int)nc, tmp);
-              u32 t = 0;
-              for (int j = 0; j < B; ++j)
-                if ((u32)j < cnt && cls[j] == c) adj[j] = tmp[t++];
-            }
-          }
-          for (int j = 0; j < B; ++j) {
-            if ((u32)j >= cnt) continue;
-            NodeHdr* h = (NodeHdr*)hdr[j];
-            u32* a = (u32*)adj[j];
-            if (h && a) {
-              fill_node(h, a, seed, job, first + j, deg[j], n);
-            } else {
-              nulls++;
-              if (a) hafree(a);
-              if (h) hafree(h);
-              h = nullptr;
-            }
-            tab[first + j] = h;
-          }
-        } else {
-          void* hdr[B];
-          void* tmp[B];
-          for (int j = 0; j < B; ++j) {
-            hdr[j] = (u32)j < cnt ? (void*)tab[first + j] : nullptr;
-            if (hdr[j]) bad += check_node((NodeHdr*)hdr[j], seed, job, first + j, n);
-          }
-          for (u64 cs = all; cs; cs &= cs - 1) {
-            const u32 c = (u32)__builtin_ctzll(cs);
-            u32 nc = 0;
-            for (int j = 0; j < B; ++j)
-              if ((u32)j < cnt && cls[j] == c) tmp[nc++] = hdr[j] ? (void*)((NodeHdr*)hdr[j])->adj : nullptr;
-            if (nc) hafree_batch(tmp, (int)nc);
-          }
-          if (cnt) hafree_batch(hdr, (int)cnt);
-        }
+    u32 g = 0;
+    if (me == 0) g = atomicAdd(next, GRAPH_GRAB);
+    g = rfl32(g);
+    if (g >= nseq) break;
+    const u32 gend = g + GRAPH_GRAB < nseq ? g + GRAPH_GRAB : nseq;
+    for (u32 it = g; it < gend; ++it) {
+      const u64 e = seq[it];
+      const u32 job = (u32)(e >> 32) & 0x7fffffffu, chunk = (u32)e;
+      const u32 nch = coffs[job + 1] - coffs[job];
+      if (e >> 63) {
+        while (rfl32(__hip_atomic_load(&built[job], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < nch)
+          __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        graph_chunk_free<B>(job, chunk, nnodes[job], seed, table + offs[job], lds, me, lt, bad);
+      } else {
+        graph_chunk_build<B>(job, chunk, nnodes[job], seed, table + offs[job], lds, me, lt, nulls);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the chunk and its table entries before the count
+        if (me == 0) atomicAdd(&built[job], 1u);
       }
     }
   }
-  if (nulls) atomicAdd(&stats[0], (unsigned long long)nulls);
-  if (bad) atomicAdd(&stats[1], (unsigned long long)bad);
+  const unsigned long long wn = wave_sum64(nulls), wb = wave_sum64(bad);
+  if (me == 0) {
+    if (wn) atomicAdd(&stats[0], wn);
+    if (wb) atomicAdd(&stats[1], wb);
+  }
 }
 
 // ---------------------------------------------------------------- serving
 // Device-resident request loop with continuous batching.  Each lane is one
-// sequence slot.  Request r arrives arrival[r] wall-clock ticks after the
-// start (sorted).  Idle lanes of a wave admit the next arrived requests, in
-// arrival order, while the heap's free bytes stay >= the watermark; wave 0
-// refreshes that figure from the slab counters every 8 iterations (the
-// per-slab sum of find_free_mem_k, src/statistics.cuh:34-48).  Per iteration a
-// live sequence either decodes one token -- 16 bytes {tag, t, h, ~h},
-// h = tok_hash(tag, t), 16 tokens to a 256 B page from hamalloc -- or, once
-// it has its length, drains one page: reads its tokens back, adds the valid
-// hashes to the checksum and frees the page.  A page malloc that returns NULL
-// ends the sequence early; `produced` says how many tokens it holds.
-// The loop ends when no request is left to admit and every lane of the wave
-// is idle, or at the deadline.
+// sequence slot; wave w serves the request stream r = w, w + W, w + 2W, ...
+// (W waves), in arrival order: its idle lanes admit the stream's requests
+// that have arrived (arrival[r] wall-clock ticks after the start) while the
+// heap's free bytes stay >= the watermark.  Wave 0 keeps that figure current:
+// every iteration it adds 64 slabs' live bytes from the slab counters (the
+// per-slab sum of find_free_mem_k, src/statistics.cuh:34-48) and publishes a
+// full pass.  Per iteration a live sequence either decodes one token -- 16
+// bytes {tag, t, h, ~h}, h = tok_hash(tag, t), 16 tokens to a 256 B page from
+// hamalloc -- or, once it has its length, drains one page: reads its tokens
+// back, adds the hashes of the intact ones to the checksum and frees the
+// page.  A page malloc that returns NULL ends the sequence early; `produced`
+// says how many tokens it holds.  A wave ends when its stream is exhausted and
+// its lanes are idle, or at the deadline.
 struct SrvState {
   u64 t0;        // wall clock at the start
   u64 free_est;  // heap free bytes from the slab counters
-  u32 next_req, done, pad0, pad1;
+  u32 admitted, done, pad0, pad1;
   unsigned long long nulls, samples, pages, stalls;
 };
 
@@ -598,7 +713,7 @@ __global__ void __launch_bounds__(256) srv_loop_k(SrvArgs a) {
   SrvState* st = a.st;
   const u32 me = threadIdx.x & 63;
   const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool refresher = slot < 64;
+  const u32 wave = slot >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
   const u64 lt = me ? (~0ull >> (64 - me)) : 0ull;  // lanes below me
   void** pages = a.page_tab + (u64)slot * a.max_pages;
   const u64 t0 = __hip_atomic_load(&st->t0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -606,57 +721,56 @@ __global__ void __launch_bounds__(256) srv_loop_k(SrvArgs a) {
   u64 first_bad = 0;
   bool draining = false;
   char* cur = nullptr;
-  u32 my_pages = 0, my_nulls = 0, my_stalls = 0;
-  for (u32 iter = 0;; ++iter) {
+  u32 my_pages = 0, my_nulls = 0, my_stalls = 0, my_admits = 0;
+  u32 next = wave;                    // the wave's next request
+  unsigned long long used_acc = 0;   // wave 0: live bytes of the slabs scanned so far
+  u32 scan = 0;
+  for (;;) {
     const u64 now = wall_clock64() - t0;
-    if (refresher && (iter & 7) == 0) {
-      unsigned long long used = 0;
-      for (u32 b = me; b < P.nsbs; b += 64) {
+    if (wave == 0) {
+      const u32 b = scan + me;
+      if (b < P.nsbs) {
         const u32 csz = __hip_atomic_load(&P.desc[b].chunk_sz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const u64 c = __hip_atomic_load(P.ctr + (u64)b * ha::CTR_STRIDE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        used += (unsigned long long)ha::ctr_count(c) * csz;
+        if (csz) used_acc += ha::chunk_mul(ha::ctr_count(c), csz);
       }
-      for (int o = 32; o; o >>= 1) used += __shfl_xor(used, o);
-      if (me == 0) {
-        const u64 fe = used < P.arena_bytes ? P.arena_bytes - used : 0;
-        __hip_atomic_store(&st->free_est, fe, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      scan += 64;
+      if (scan >= P.nsbs) {
+        const unsigned long long used = wave_sum64(used_acc);
+        if (me == 0) {
+          const u64 fe = used < P.arena_bytes ? P.arena_bytes - used : 0;
+          __hip_atomic_store(&st->free_est, fe, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        used_acc = 0;
+        scan = 0;
       }
     }
-    // ---- admission, in arrival order
+    // ---- admission from the wave's own stream
     const u64 I = __ballot(req == SRV_NONE);
-    const u32 nr = (u32)__builtin_amdgcn_readfirstlane(
-        (int)__hip_atomic_load(&st->next_req, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    if (I == ~0ull && nr >= a.nreq) break;
+    if (I == ~0ull && next >= a.nreq) break;
     if (now > a.deadline_tk) break;
     bool admitted = false;
-    if (I && nr < a.nreq) {
+    if (I && next < a.nreq && a.arrival[next] <= now) {
       const u64 fe0 = __hip_atomic_load(&st->free_est, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const u64 fe = ((u64)(u32)__builtin_amdgcn_readfirstlane((int)(u32)(fe0 >> 32)) << 32) |
-                     (u32)__builtin_amdgcn_readfirstlane((int)(u32)fe0);
+      const u64 fe = ((u64)rfl32((u32)(fe0 >> 32)) << 32) | rfl32((u32)fe0);
       if (fe >= a.watermark) {
-        const u32 cand = nr + me;
-        const bool arrived = cand < a.nreq && a.arrival[cand] <= now;
-        u32 k = (u32)__builtin_popcountll(__ballot(arrived));  // a prefix: arrivals are sorted
-        const u32 nidle = (u32)__builtin_popcountll(I);
-        k = k < nidle ? k : nidle;
-        if (k) {
-          u32 got = 0;
-          if (me == 0) got = atomicCAS(&st->next_req, nr, nr + k) == nr;
-          got = (u32)__builtin_amdgcn_readfirstlane((int)got);
-          if (got) {
-            const u32 rank = (u32)__builtin_popcountll(I & lt);
-            if (req == SRV_NONE && rank < k) {
-              req = nr + rank;
-              tag = a.tag[req];
-              len = a.length[req];
-              tok = cs = np = drain = nbad = 0;
-              first_bad = 0;
-              draining = false;
-              a.first_tk[req] = now;
-            }
-            admitted = true;
-          }
+        const bool idle = req == SRV_NONE;
+        const u32 rank = (u32)__builtin_popcountll(I & lt);
+        const u32 cand = next + rank * nwaves;
+        const bool take = idle && cand < a.nreq && a.arrival[cand] <= now;  // a prefix of the idle lanes
+        const u32 k = (u32)__builtin_popcountll(__ballot(take));
+        if (take) {
+          req = cand;
+          tag = a.tag[req];
+          len = a.length[req];
+          tok = cs = np = drain = nbad = 0;
+          first_bad = 0;
+          draining = false;
+          a.first_tk[req] = now;
+          ++my_admits;
         }
+        next += k * nwaves;
+        admitted = k != 0;
       } else {
         ++my_stalls;
       }
@@ -736,8 +850,7 @@ __global__ void __launch_bounds__(256) srv_loop_k(SrvArgs a) {
     if (S) {
       unsigned long long base = 0;
       if (me == 0) base = atomicAdd(&st->samples, (unsigned long long)__builtin_popcountll(S));
-      base = ((unsigned long long)(u32)__builtin_amdgcn_readfirstlane((int)(u32)(base >> 32)) << 32) |
-             (u32)__builtin_amdgcn_readfirstlane((int)(u32)base);
+      base = ((unsigned long long)rfl32((u32)(base >> 32)) << 32) | rfl32((u32)base);
       if (sampled) {
         const unsigned long long i = base + (u64)__builtin_popcountll(S & lt);
         if (i < a.lat_cap) a.lat_tk[i] = lat;
@@ -745,10 +858,11 @@ __global__ void __launch_bounds__(256) srv_loop_k(SrvArgs a) {
     }
     if (!admitted && I == ~0ull) __builtin_amdgcn_s_sleep(4);
   }
-  const u32 wp = wave_sum(my_pages), wn = wave_sum(my_nulls);
+  const u32 wp = wave_sum(my_pages), wn = wave_sum(my_nulls), wa = wave_sum(my_admits);
   if (me == 0) {
     if (wp) atomicAdd(&st->pages, (unsigned long long)wp);
     if (wn) atomicAdd(&st->nulls, (unsigned long long)wn);
+    if (wa) atomicAdd(&st->admitted, wa);
     if (my_stalls) atomicAdd(&st->stalls, (unsigned long long)my_stalls);
   }
 }
@@ -1032,7 +1146,7 @@ int ha_run_lists(const uint32_t* sizes, int n, int per, const uint32_t* order, i
 int ha_run_phase(const ha_phase_args* a, ha_phase_result* res, double* trace, int trace_cap) {
   const ha::Params& P = ha::host_params();
   if (!P.initialized) return hfail(HA_E_STATE, "allocator not initialized");
-  if (!a || !res || a->nthreads < 1 || a->bs < 1 || a->bs > 1024 || a->slots < 1 || a->nphases < 1 ||
+  if (!a || !res || a->nthreads < 1 || a->bs < 64 || a->bs > 1024 || a->bs % 64 || a->slots < 1 || a->nphases < 1 ||
       a->min_size < 8 || a->max_size > 3072 || a->min_size > a->max_size || a->max_steps < 1)
     return hfail(HA_E_INVALID, "bad phase arguments");
   const int n = a->nthreads, slots = a->slots;
@@ -1064,8 +1178,7 @@ int ha_run_phase(const ha_phase_args* a, ha_phase_result* res, double* trace, in
       const auto w0 = std::chrono::steady_clock::now();
       HH(hipEventRecord(e0, s.s));
       phase_step_k<<<grid, a->bs, 0, s.s>>>((void**)dp.p, (u32*)ds.p, n, slots, a->seed, step,
-                                              alloc_phase ? a->p_hi : a->p_lo, alloc_phase ? a->p_lo : a->p_hi,
-                                              a->min_size / 8, a->max_size / 8, c);
+                                              alloc_phase ? a->p_hi : a->p_lo, a->min_size / 8, a->max_size / 8, c);
       HH(hipEventRecord(e1, s.s));
       HH(hipStreamSynchronize(s.s));
       const auto w1 = std::chrono::steady_clock::now();
@@ -1143,7 +1256,7 @@ int ha_replay_batch(const uint32_t* ops, int nsteps, int lanes, uint32_t* recs, 
 
 int ha_run_graphs(const ha_graph_args* a, const uint32_t* nnodes, ha_graph_result* res) {
   if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
-  if (!a || !nnodes || !res || a->njobs < 1 || a->bs < 64 || a->bs > 1024 || a->bs % 64 ||
+  if (!a || !nnodes || !res || a->njobs < 1 || a->njobs > (1 << 30) || a->bs < 64 || a->bs > 1024 || a->bs % 64 ||
       (a->batch != 4 && a->batch != 8 && a->batch != 16) || a->waves < 1 || a->plain_chunk_nodes < 1 ||
       a->repeats < 1)
     return hfail(HA_E_INVALID, "bad graph arguments");
@@ -1154,12 +1267,11 @@ int ha_run_graphs(const ha_graph_args* a, const uint32_t* nnodes, ha_graph_resul
     offs[j + 1] = offs[j] + nnodes[j];
   }
   const u64 total = offs[njobs];
-  DBuf doffs, dn, dtab, dstats, dctr;
+  DBuf doffs, dn, dtab, dstats;
   HH(doffs.alloc((njobs + 1) * 8));
   HH(dn.alloc(njobs * 4));
   HH(dtab.alloc(total * 8));
   HH(dstats.alloc(16));
-  HH(dctr.alloc(4));
   HH(hipMemcpy(doffs.p, offs.data(), (njobs + 1) * 8, hipMemcpyHostToDevice));
   HH(hipMemcpy(dn.p, nnodes, njobs * 4, hipMemcpyHostToDevice));
   HH(hipMemset(dstats.p, 0, 16));
@@ -1190,21 +1302,54 @@ int ha_run_graphs(const ha_graph_args* a, const uint32_t* nnodes, ha_graph_resul
     }
     return (int)hipGetLastError();
   };
-  // batched persistent form, captured in a graph with the queue reset
+  // batched persistent form, captured in a graph with its counters reset
+  const u32 per_chunk = 64u * (u32)a->batch;
+  std::vector<u32> coffs(njobs + 1, 0);
+  for (u32 j = 0; j < njobs; ++j) coffs[j + 1] = coffs[j] + (nnodes[j] + per_chunk - 1) / per_chunk;
+  const u32 nitems = coffs[njobs];
+  // the item sequence: builds in job order, each job's frees `lag` items after its last build
+  const u64 lag = (u64)a->waves * GRAPH_GRAB;
+  std::vector<u64> seq;
+  seq.reserve(2 * (size_t)nitems);
+  u32 jf = 0;  // next job whose frees are due
+  for (u64 b = 0; b < (u64)nitems + lag; ++b) {
+    if (b < nitems) {
+      const u32 j = (u32)(std::upper_bound(coffs.begin(), coffs.end(), (u32)b) - coffs.begin()) - 1;
+      seq.push_back(((u64)j << 32) | (b - coffs[j]));
+    }
+    while (jf < njobs && (u64)coffs[jf + 1] - 1 + lag <= b) {
+      for (u32 c = 0; c < coffs[jf + 1] - coffs[jf]; ++c) seq.push_back((1ull << 63) | ((u64)jf << 32) | c);
+      ++jf;
+    }
+  }
+  while (jf < njobs) {
+    for (u32 c = 0; c < coffs[jf + 1] - coffs[jf]; ++c) seq.push_back((1ull << 63) | ((u64)jf << 32) | c);
+    ++jf;
+  }
+  const u32 nseq = (u32)seq.size();
+  DBuf dcoffs, dseq, dnext, dbuilt;
+  HH(dcoffs.alloc((njobs + 1) * 4));
+  HH(dseq.alloc((size_t)nseq * 8));
+  HH(dnext.alloc(4));
+  HH(dbuilt.alloc(njobs * 4));
+  HH(hipMemcpy(dcoffs.p, coffs.data(), (njobs + 1) * 4, hipMemcpyHostToDevice));
+  HH(hipMemcpy(dseq.p, seq.data(), (size_t)nseq * 8, hipMemcpyHostToDevice));
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  const unsigned bgrid = (unsigned)((a->waves * 64 + a->bs - 1) / a->bs);
+  const unsigned bgrid = (unsigned)((a->waves * 64 + 255) / 256);
   HH(hipStreamBeginCapture(s.s, hipStreamCaptureModeThreadLocal));
-  reset_counter_k<<<1, 1, 0, s.s>>>((u32*)dctr.p);
+  graph_reset_k<<<16, 256, 0, s.s>>>((u32*)dnext.p, (u32*)dbuilt.p, njobs);
+#define HA_GRAPH_BATCH(BB)                                                                                  \
+  graph_batch_k<BB><<<bgrid, 256, 0, s.s>>>((const u64*)dseq.p, nseq, (const u32*)dcoffs.p, (const u64*)doffs.p, \
+                                            (const u32*)dn.p, a->seed, (NodeHdr**)dtab.p, (u32*)dnext.p,        \
+                                            (u32*)dbuilt.p, st)
   if (a->batch == 4)
-    graph_batch_k<4><<<bgrid, a->bs, 0, s.s>>>((const u64*)doffs.p, (const u32*)dn.p, njobs, a->seed,
-                                                (NodeHdr**)dtab.p, (u32*)dctr.p, st);
+    HA_GRAPH_BATCH(4);
   else if (a->batch == 8)
-    graph_batch_k<8><<<bgrid, a->bs, 0, s.s>>>((const u64*)doffs.p, (const u32*)dn.p, njobs, a->seed,
-                                                (NodeHdr**)dtab.p, (u32*)dctr.p, st);
+    HA_GRAPH_BATCH(8);
   else
-    graph_batch_k<16><<<bgrid, a->bs, 0, s.s>>>((const u64*)doffs.p, (const u32*)dn.p, njobs, a->seed,
-                                                 (NodeHdr**)dtab.p, (u32*)dctr.p, st);
+    HA_GRAPH_BATCH(16);
+#undef HA_GRAPH_BATCH
   HH(hipStreamEndCapture(s.s, &graph));
   HH(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   double best_plain = 1e30, best_batch = 1e30;
@@ -1359,7 +1504,7 @@ int ha_run_serve(const ha_serve_io* io, ha_serve_result* res) {
   res->stalls = fin.stalls;
   res->samples = ns;
   res->completed = fin.done;
-  res->admitted = fin.next_req;
+  res->admitted = fin.admitted;
   res->clock_khz = (uint32_t)khz;
   res->pad = 0;
   return HA_OK;

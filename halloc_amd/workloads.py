@@ -28,3 +28,21 @@ def pages_for(length) -> np.ndarray:
     """256 B pages a sequence of `length` tokens occupies."""
     ln = np.asarray(length, dtype=np.int64)
     return (ln + PAGE_BYTES // TOKEN_BYTES - 1) // (PAGE_BYTES // TOKEN_BYTES)
+
+
+def sweep_sizes(rng, n, per_max=8, large_frac=0.01, lo=16, hi=3072, large_hi=65536):
+    """(per_max, n) request sizes: lane i makes U{1..per_max} requests, each
+    log-uniform on [lo, hi] bytes, or with probability large_frac uniform on
+    (hi, large_hi]; 0 marks no request."""
+    counts = rng.integers(1, per_max + 1, n)
+    logs = rng.uniform(np.log(lo), np.log(hi), (per_max, n))
+    sizes = np.floor(np.exp(logs)).astype(np.int64).clip(lo, hi)
+    large = rng.random((per_max, n)) < large_frac
+    sizes[large] = rng.integers(hi + 1, large_hi + 1, int(large.sum()))
+    sizes[np.arange(per_max)[:, None] >= counts[None, :]] = 0
+    return sizes.astype(np.uint32)
+
+
+def graph_sizes(rng, njobs, lo=1024, hi=65536):
+    """Node counts of the random-graph jobs: log-uniform on [lo, hi]."""
+    return np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), njobs))).astype(np.uint32)

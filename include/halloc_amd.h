@@ -133,13 +133,13 @@ int ha_run_lists(const uint32_t *sizes, int n, int per, const uint32_t *order, i
                  ha_list_result *res);
 
 /* Phase workload (tst/perf/phase-throughput/phase-throughput.cu:18-50): each
-   of nthreads lanes owns `slots` slots.  Per step an empty slot allocates with
-   probability p_alloc a block of 8*U{min/8..max/8} bytes, a full one frees with
-   probability p_free.  Even phases allocate (p_alloc = p_hi, p_free = p_lo)
-   until the requested bytes reach fill_hi of the slab heap, or more than 1% of
-   a step's requests return NULL; odd phases free (swapped probabilities) until
-   fill_lo.  Then everything left is freed.  trace (3 doubles per step: phase,
-   fill, NULLs) may be NULL. */
+   of nthreads lanes owns `slots` slots and takes one action per step: with
+   probability p it allocates a block of 8*U{min/8..max/8} bytes into an empty
+   slot (stamping its first word), otherwise it checks and frees a live slot.
+   Even phases allocate (p = p_hi) until the live bytes reach fill_hi of the
+   slab heap, or more than 1% of a step's requests return NULL; odd phases free
+   (p = p_lo) until fill_lo.  Then everything left is freed.  bs is a multiple
+   of 64.  trace (3 doubles per step: phase, fill, NULLs) may be NULL. */
 typedef struct ha_phase_args {
   int32_t nthreads, bs, slots, nphases, max_steps, min_size, max_size;
   uint32_t seed;
@@ -158,10 +158,13 @@ int ha_run_phase(const ha_phase_args *a, ha_phase_result *res, double *trace, in
    nodes, each with a 16 B header and an int32 adjacency array of Poisson(8)
    random targets, checks it and frees it.  Plain per-call form: one thread per
    node calling hamalloc / hafree, jobs launched in chunks of at most
-   plain_chunk_nodes nodes.  Batched form: one launch of `waves` wavefronts
-   taking jobs from a queue, lane l handling `batch` nodes (4, 8 or 16) through
-   hamalloc_batch / hafree_batch, captured in a HIP graph.  Each form runs
-   `repeats` times and reports its best time; errors counts wrong graphs. */
+   plain_chunk_nodes nodes.  Batched form: one persistent launch of `waves`
+   wavefronts (blocks of 256), captured in a HIP graph, taking chunks of
+   64 * batch nodes (batch 4, 8 or 16): every build chunk, and each job's check
+   and free chunks after the job is built; a chunk's adjacency arrays of one
+   size class come from one hamalloc_batch call and go back through
+   hafree_batch.  Each form runs `repeats` times and reports its best time;
+   errors counts wrong graph nodes. */
 typedef struct ha_graph_args {
   int32_t njobs, bs, batch, waves, plain_chunk_nodes, repeats, run_plain, run_batch;
   uint32_t seed, pad;

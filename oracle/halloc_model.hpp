@@ -223,7 +223,8 @@ class Heap {
 
   std::vector<std::atomic<u64>> bits, nibs, ctr, sets;
   std::vector<std::atomic<u32>> d_size, d_chunk, d_csz, d_head, sb_lock;
-  std::vector<std::atomic<u32>> head, cached, head_lock, ticket;
+  std::vector<std::atomic<u32>> head, cached, head_lock, ticket, search_ep;
+  std::atomic<u32> class_lock[NSIZES] = {};
   std::vector<uint8_t> arena;  // payload memory (allocated lazily by users that touch data)
   std::atomic<u64> large_ctr{0};
 
@@ -258,6 +259,7 @@ class Heap {
     head = std::vector<std::atomic<u32>>((size_t)nheads * NSIZES);
     cached = std::vector<std::atomic<u32>>((size_t)nheads * NSIZES);
     head_lock = std::vector<std::atomic<u32>>((size_t)nheads * NSIZES);
+    search_ep = std::vector<std::atomic<u32>>((size_t)nheads * NSIZES);
     ticket = std::vector<std::atomic<u32>>((size_t)nheads * NSIZES);
     for (auto& x : bits) x.store(0, std::memory_order_relaxed);
     for (auto& x : nibs) x.store(0, std::memory_order_relaxed);
@@ -270,6 +272,7 @@ class Heap {
     }
     for (size_t i = 0; i < head.size(); ++i) {
       head[i].store(SB_NONE); cached[i].store(SB_NONE); head_lock[i].store(0); ticket[i].store(TICKET_INIT);
+      search_ep[i].store(0);
     }
     if (with_arena) arena.assign((size_t)arena_bytes, 0);
     ok = true;
@@ -518,21 +521,29 @@ class Heap {
 
   // new_sb_for_size (src/slab.cuh:378-445).  `seen` is the head word the
   // caller last used: a head replaced since then is returned as it is; a
-  // thread that finds the head lock taken waits for a change and looks again.
-  // SB_NONE only comes back from the lock holder that found no slab.
+  // thread that finds the head lock taken waits for a change, and gets
+  // SB_NONE if a slab search failed meanwhile, else looks again.
   u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen) {
     const SizeInfo& s = si[size_id];
     size_t hi = (size_t)ihead * NSIZES + size_id;
     for (;;) {
       u32 cur = head[hi].load();
       if (cur != seen && cur != SB_NONE) return cur;
+      const u32 e0 = search_ep[hi].load();
       if (try_lock(head_lock[hi])) {
         cur = head[hi].load();
         u32 nh;
         if (cur == SB_NONE || ctr_count(ctr[cur & ~HEAD_EXT].load()) >= head_limit(cur, s)) {
           nh = cached[hi].load();
           cached[hi].store(SB_NONE);
-          if (nh == SB_NONE) nh = find_sb_for_size(size_id, s.chunk_id, ihead, true);
+          if (nh == SB_NONE) {
+            // one head search per class at a time, so a pair that follows can
+            // share the head the one before it took
+            if (nheads > 1) lock(class_lock[size_id]);
+            nh = find_sb_for_size(size_id, s.chunk_id, ihead, true);
+            if (nheads > 1) unlock(class_lock[size_id]);
+            search_ep[hi].store((((search_ep[hi].load() >> 1) + 1) << 1) | (nh == SB_NONE ? 1u : 0u));
+          }
           if (nh != SB_NONE) {
             head[hi].store(nh);
             if (cur != SB_NONE) {
@@ -548,6 +559,8 @@ class Heap {
         return nh;
       }
       while (head[hi].load() == cur && head_lock[hi].load() != 0) std::this_thread::yield();
+      const u32 e1 = search_ep[hi].load();
+      if (e1 != e0 && (e1 & 1u) && head[hi].load() == cur) return SB_NONE;
     }
   }
 

@@ -1,6 +1,8 @@
 """Shared helpers of the test suite: random replay programs and block checks."""
 import numpy as np
 
+from halloc_amd.workloads import graph_sizes, sweep_sizes  # noqa: F401  (re-exported for the tests)
+
 BLOCK_SIZES = [16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072]
 REC_NULL = 0xFFFFFFFF
 
@@ -72,19 +74,6 @@ def pressure_program(seed=3, nsteps=6500):
     return np.stack(steps)
 
 
-def sweep_sizes(rng, n, per_max=8, large_frac=0.01, lo=16, hi=3072, large_hi=65536):
-    """(per_max, n) request sizes: lane i makes U{1..per_max} requests, each
-    log-uniform on [lo, hi] bytes, or with probability large_frac uniform on
-    (hi, large_hi]; 0 marks no request."""
-    counts = rng.integers(1, per_max + 1, n)
-    logs = rng.uniform(np.log(lo), np.log(hi), (per_max, n))
-    sizes = np.floor(np.exp(logs)).astype(np.int64).clip(lo, hi)
-    large = rng.random((per_max, n)) < large_frac
-    sizes[large] = rng.integers(hi + 1, large_hi + 1, int(large.sum()))
-    sizes[np.arange(per_max)[:, None] >= counts[None, :]] = 0
-    return sizes.astype(np.uint32)
-
-
 def random_batch_ops(rng, nsteps, lanes, sizes, max_count=16, p_free=0.4):
     """Random batched replay program: [kind, arg, count, 0] per (step, lane)."""
     ops = np.zeros((nsteps, lanes, 4), dtype=np.uint32)
@@ -100,11 +89,6 @@ def random_batch_ops(rng, nsteps, lanes, sizes, max_count=16, p_free=0.4):
                 new.append(st * lanes + l)
         live.extend(new)
     return ops
-
-
-def graph_sizes(rng, njobs, lo=1024, hi=65536):
-    """Node counts of the random-graph jobs: log-uniform on [lo, hi]."""
-    return np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), njobs))).astype(np.uint32)
 
 
 def _hash3(a, b, c):

@@ -259,8 +259,9 @@ def test_size_sweep_lists_random_free_order(halloc):
 
 
 def test_phase_workload_invariants(halloc):
+    # 24 slabs of 2 MiB for the 9 classes of 16..256 B
     say("phase 2^16 lanes x 4 slots")
-    opts = halloc(H.HallocOpts(memory=32 << 20))
+    opts = halloc(H.HallocOpts(memory=64 << 20, sb_sz_sh=21))
     r = H.Halloc.run_phase(nthreads=1 << 16, slots=4, nphases=3, max_steps=60)
     say(f"phase steps={r['phase_steps']} nulls={r['nulls']}")
     assert r["errors"] == 0
@@ -299,3 +300,16 @@ def test_graph_builds_plain_and_batched(halloc, batch):
     d = H.Halloc.dump()
     assert ((d[:g["nsbs"]] >> np.uint64(32)) == 0).all()
     assert M.check_dump(model_opts(opts), d)[0] == 0
+
+
+def test_autotune_small_space():
+    # the tuner's candidates run the batched graph form; every accepted one is
+    # violation-free and the best is no slower than the start
+    from halloc_amd import autotune
+    say("autotune small space")
+    nn = graph_sizes(np.random.default_rng(9), 24)
+    best, ms, hist = autotune.tune(nn, memory=1 << 30, sweeps=1, log=say,
+                                   space={"batch": [4, 16], "nheads": [1, 8], "hash_step_frac": [0.0, 0.05]})
+    assert np.isfinite(ms) and ms <= hist[0][1]
+    assert len(hist) == 1 + 4  # batch 4 and 16 besides the start's 8, nheads 1, hash_step_frac 0.05
+    assert best["batch"] in (4, 8, 16)
