@@ -53,7 +53,7 @@ class _Geometry(ctypes.Structure):
     _fields_ = [("nsbs", ctypes.c_uint32), ("sb_sz", ctypes.c_uint32), ("sb_sz_sh", ctypes.c_uint32),
                 ("nheads", ctypes.c_uint32), ("nsw", ctypes.c_uint32), ("bit_words", ctypes.c_uint32),
                 ("nib_words", ctypes.c_uint32), ("pad", ctypes.c_uint32),
-                ("arena_bytes", ctypes.c_uint64), ("cuda_memory", ctypes.c_uint64),
+                ("arena_bytes", ctypes.c_uint64), ("large_bytes", ctypes.c_uint64),
                 ("size_table", ctypes.c_uint32 * (16 * 12))]
 
 
@@ -120,7 +120,7 @@ class HallocOpts:
     roomy_fraction: float = 0.6
     sparse_fraction: float = 0.012
     sb_sz_sh: int = 22
-    nheads: int = 8
+    nheads: int = 16
     hash_step_frac: float = 0.0
     cached_arena: int = 0
 
@@ -211,7 +211,7 @@ class Halloc:
         table = np.frombuffer(bytes(g.size_table), dtype=np.uint32).reshape(NSIZES, 12).copy()
         return {"nsbs": g.nsbs, "sb_sz": g.sb_sz, "sb_sz_sh": g.sb_sz_sh, "nheads": g.nheads,
                 "nsw": g.nsw, "bit_words": g.bit_words, "nib_words": g.nib_words,
-                "arena_bytes": g.arena_bytes, "cuda_memory": g.cuda_memory, "size_table": table}
+                "arena_bytes": g.arena_bytes, "large_bytes": g.large_bytes, "size_table": table}
 
     # ---------------------------------------------------------- workloads
     @staticmethod

@@ -28,13 +28,13 @@ SPACE = {
     "batch": [4, 8, 16],
     "waves": [1024, 2048, 4096, 8192],
     "sb_sz_sh": [21, 22, 23, 24],
-    "nheads": [1, 2, 4, 8],
+    "nheads": [1, 2, 4, 8, 16, 32],
     "busy_fraction": [0.7, 0.835, 0.9],
     "roomy_fraction": [0.4, 0.6],
     "sparse_fraction": [0.005, 0.012, 0.05],
     "hash_step_frac": [0.0, 0.01, 0.05, 0.11, 0.25],
 }
-DEFAULT = {"batch": 8, "waves": 2048, "sb_sz_sh": 22, "nheads": 8, "busy_fraction": 0.835,
+DEFAULT = {"batch": 8, "waves": 2048, "sb_sz_sh": 22, "nheads": 16, "busy_fraction": 0.835,
            "roomy_fraction": 0.6, "sparse_fraction": 0.012, "hash_step_frac": 0.0}
 OPT_KEYS = ("sb_sz_sh", "nheads", "busy_fraction", "roomy_fraction", "sparse_fraction", "hash_step_frac")
 
@@ -51,7 +51,10 @@ def evaluate(cfg: dict, nnodes: np.ndarray, memory: int, repeats: int = 2) -> fl
     """Best batched-form time (ms) of the graph jobs under cfg; inf if rejected."""
     if not valid(cfg):
         return math.inf
-    Halloc.init(opts_of(cfg, memory))
+    try:
+        Halloc.init(opts_of(cfg, memory))  # rejects options with no size table
+    except HallocError:
+        return math.inf
     try:
         r = Halloc.run_graphs(nnodes, batch=cfg["batch"], waves=cfg["waves"], run_plain=False,
                               repeats=repeats)

@@ -48,6 +48,10 @@ constexpr u32 MAX_SB_SZ_SH = 26;     // tst/common/common.cu:169 allows -b 20..2
 // a head word is a slab id, plus HEAD_EXT when the slab came from the busy set:
 // such a head is filled up to (busy + nchunks) / 2 instead of busy
 constexpr u32 HEAD_EXT = 0x80000000u;
+// large blocks (above MAX_BLOCK_SZ): power-of-two multiples of LARGE_UNIT
+constexpr u32 LARGE_UNIT_SH = 12;     // 4 KiB
+constexpr u32 LARGE_NCLASSES = 40;    // classes 4 KiB << c
+constexpr u32 LSTACK_STRIDE = 16;     // u64 words: one 128 B line per class stack head
 
 constexpr u32 CTR_CHUNK_SH = 8;
 constexpr u64 CTR_CHUNK_NONE = 0xF;
@@ -79,6 +83,16 @@ struct Params {
   u32* ticket;
   u32* class_lock;  // one per size class, serializes head searches of its pairs
   u32* search_ep;   // per pair: searches done (bits 31..1) and whether the last failed (bit 0)
+  u32* active;      // pairs that hold a head
+  // large-block region: per class a free stack (tag << 32 | unit + 1), a bump
+  // pointer in units, and per 4 KiB unit the next link and the block's class
+  char* large;
+  u64 large_bytes;
+  u64* lstack;
+  u64* lbump;
+  u32* llink;
+  u32* lclass;
+  u32 lunits, lnclasses;
   u32 nsbs, sb_sz, sb_sz_sh, nheads;
   u32 nsw, bit_words, nib_words, initialized;
   u32 cached_arena, pad;  // cached_arena: hafree releases before publishing

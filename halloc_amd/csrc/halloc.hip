@@ -54,6 +54,15 @@ HA_DEV u64 shfl64(u64 v, u32 src) {
   return ((u64)hi << 32) | lo;
 }
 HA_DEV u32 xcc_id() { return __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)); }
+// HW_ID bits 8..15: the wave's CU, shader array and shader engine
+HA_DEV u32 cu_bits() { return __builtin_amdgcn_s_getreg((4) | (8 << 6) | ((8 - 1) << 11)); }
+// The head set a wave uses: its XCD's, and with more than 8 sets (a multiple
+// of 8), one of the XCD's nheads / 8 sets picked by the wave's CU.
+HA_DEV u32 head_set() {
+  const u32 x = xcc_id(), n = ha_p.nheads;
+  if (n <= 8) return x % n;
+  return (x + 8u * (cu_bits() % (n / 8u))) % n;
+}
 
 // ------------------------------------------------ agent-scope atomic helpers
 #define HA_SCOPE __HIP_MEMORY_SCOPE_AGENT
@@ -90,7 +99,7 @@ HA_DEV void unlock(u32* m) {
 HA_DEV void set_add(u32 set, u32 sb) { a_or(set_words(set) + sb / 64, 1ull << (sb % 64)); }
 HA_DEV void set_remove(u32 set, u32 sb) { a_and(set_words(set) + sb / 64, ~(1ull << (sb % 64))); }
 // first-fit claim over every word holding a slab (src/sbset.cuh:5-35)
-__device__ __noinline__ u32 set_pop(u32 set) {
+HA_DEV u32 set_pop(u32 set) {
   u64* w = set_words(set);
   for (u32 i = 0; i < ha_p.nsw; ++i) {
     u64 word = ld64(w + i);
@@ -105,7 +114,7 @@ __device__ __noinline__ u32 set_pop(u32 set) {
 }
 
 // sb_try_mark_free (src/slab.cuh:89-113); caller holds sb_lock[sb]
-__device__ __noinline__ void try_mark_free(u32 sb, u32 size_id, u32 chunk_id, bool from_head) {
+HA_DEV void try_mark_free(u32 sb, u32 size_id, u32 chunk_id, bool from_head) {
   Desc* d = ha_p.desc + sb;
   if (ld32(&d->is_head) || chunk_id == SZ_NONE) return;
   u64 delta = ctr_chunk_delta(chunk_id), old2;
@@ -134,7 +143,7 @@ __device__ __noinline__ void try_mark_free(u32 sb, u32 size_id, u32 chunk_id, bo
 
 // count transitions after a decrement (src/slab.cuh:176-221): `old` is the
 // counter before the decrement of `change` chunks
-__device__ __noinline__ void ctr_transitions(u32 sb, u64 old, u32 change) {
+HA_DEV void ctr_transitions(u32 sb, u64 old, u32 change) {
   if (ctr_is_head(old)) return;
   Desc* d = ha_p.desc + sb;
   u32 size_id = ld32(&d->size_id), chunk_id = ld32(&d->chunk_id);
@@ -158,20 +167,11 @@ __device__ __noinline__ void ctr_transitions(u32 sb, u64 old, u32 change) {
   }
 }
 
-HA_DEV bool needs_transition(u32 sb, u64 old, u32 change) {
-  if (ctr_is_head(old)) return false;
-  Desc* d = ha_p.desc + sb;
-  u32 size_id = ld32(&d->size_id), chunk_id = ld32(&d->chunk_id);
-  if (size_id >= NSIZES || chunk_id == SZ_NONE) return false;
-  u32 oc = ctr_count(old);
-  const SizeInfo& s = ha_p.si[size_id];
-  return sb_class(oc, s) != sb_class(oc - change, s);
-}
 
 // drop one head reference (detach_head, src/slab.cuh:227-255); the last one
 // files the slab by its count.  Caller holds sb_lock[sb].  True if the slab
 // became free.
-__device__ __noinline__ bool detach_head(u32 sb) {
+HA_DEV bool detach_head(u32 sb) {
   Desc* d = ha_p.desc + sb;
   const u32 refs = ld32(&d->is_head) - 1;
   st32(&d->is_head, refs);
@@ -191,7 +191,7 @@ __device__ __noinline__ bool detach_head(u32 sb) {
 
 // Shares the head of another XCD's pair of the same class if it has room.
 // Caller holds the head lock of (ihead, size_id).
-__device__ __noinline__ u32 share_head(u32 ihead, u32 size_id, u32 chunk_id) {
+HA_DEV u32 share_head(u32 ihead, u32 size_id, u32 chunk_id) {
   const SizeInfo& s = ha_p.si[size_id];
   for (u32 ih = 0; ih < ha_p.nheads; ++ih) {
     if (ih == ihead) continue;
@@ -217,7 +217,7 @@ __device__ __noinline__ u32 share_head(u32 ihead, u32 size_id, u32 chunk_id) {
 // Takes over empty heads and cached heads of other (head, class) pairs until
 // one slab becomes free; false if none does.  Caller holds the head lock of
 // (ihead, size_id); the victims' head locks are only try-locked.
-__device__ __noinline__ bool steal_idle_head(u32 ihead, u32 size_id) {
+HA_DEV bool steal_idle_head(u32 ihead, u32 size_id) {
   for (u32 ih = 0; ih < ha_p.nheads; ++ih) {
     for (u32 c = 0; c < NSIZES; ++c) {
       if (ih == ihead && c == size_id) continue;
@@ -233,6 +233,7 @@ __device__ __noinline__ bool steal_idle_head(u32 ihead, u32 size_id) {
         bool freed = false;
         if (ok) {
           st32(wp, SB_NONE);
+          if (which == 1) a_add32(ha_p.active, ~0u);  // the pair no longer holds a head
           u32* l = ha_p.sb_lock + sb;
           lock(l);
           freed = detach_head(sb);
@@ -246,23 +247,30 @@ __device__ __noinline__ bool steal_idle_head(u32 ihead, u32 size_id) {
   return false;
 }
 
+// hamalloc_batch issues the first probes of this many rounds together
+constexpr u32 BATCH_PIPE = 4;
+
 // A head search frees at most this many idle heads before it gives up.
 constexpr u32 MAX_STEALS = 4;
 
-// With several head sets, free slabs are scarce below nheads * 16 / 2.
+// With several head sets, free slabs are scarce below the largest of nheads,
+// half the (head set, class) pairs that hold a head, and a quarter of the heap.
 HA_DEV bool free_scarce() {
   if (ha_p.nheads <= 1) return false;
   u32 n = 0;
   u64* w = set_words(0);
   for (u32 i = 0; i < ha_p.nsw; ++i) n += (u32)__builtin_popcountll(ld64(w + i));
-  return n < ha_p.nheads * NSIZES / 2;
+  u32 lim = ld32(ha_p.active) / 2;
+  lim = lim > ha_p.nheads ? lim : ha_p.nheads;
+  lim = lim > ha_p.nsbs / 4 ? lim : ha_p.nsbs / 4;
+  return n < lim;
 }
 
 // find_sb_for_size (src/slab.cuh:259-372): roomy -> sparse -> free -> busy;
 // then, for a head needed now, another XCD's head of the class, then an idle
 // head of another class.  A cache refill (now == false) skips the free set
 // when free slabs are scarce; a head needed now then tries sharing first.
-__device__ __noinline__ u32 find_sb_for_size(u32 size_id, u32 chunk_id, u32 ihead, bool now) {
+HA_DEV u32 find_sb_for_size(u32 size_id, u32 chunk_id, u32 ihead, bool now) {
   const SizeInfo& s = ha_p.si[size_id];
   u32 sb;
   bool tried_share = false;
@@ -370,7 +378,7 @@ again:
 // that finds the lock taken waits for the head to change or the lock to be
 // released.  A waiter whose wait saw a slab search fail gets SB_NONE too
 // rather than repeating the search; otherwise it looks again.
-__device__ __noinline__ u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen) {
+HA_DEV u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen) {
   const SizeInfo& s = ha_p.si[size_id];
   u32* hp = hot(ha_p.head, ihead, size_id);
   u32* lp = hot(ha_p.head_lock, ihead, size_id);
@@ -399,6 +407,7 @@ __device__ __noinline__ u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen) {
         if (nh != SB_NONE) {
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
           st32(hp, nh);
+          if (cur == SB_NONE) a_add32(ha_p.active, 1u);
           if (cur != SB_NONE) {
             u32 csb = cur & ~HEAD_EXT;
             u32* l = ha_p.sb_lock + csb;
@@ -433,8 +442,10 @@ __device__ __noinline__ u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen) {
 // hsb for the lanes in P (wave-uniform), from the blocks of ticket base `old`.
 // It returns the lanes that reserved, records their block lengths in the
 // nibbles, and leaves each reserved lane's chunk in my_ich.  itry advances for
-// the whole group.
-HA_DEV u64 probe_run(const SizeInfo& s, u32 hsb, u32 limit, u32 old, u64 P, u32 me, u32& itry, u32& my_ich) {
+// the whole group; off0 is the probe offset of the first attempt (a run that
+// resumes after a first attempt made elsewhere starts at itry 1).
+HA_DEV u64 probe_run(const SizeInfo& s, u32 hsb, u32 limit, u32 old, u64 P, u32 me, u32& itry, u32& my_ich,
+                     u64 off0 = 0) {
   const u32 k = s.ncib, nchunks = s.nchunks, nwb = nchunks >> 6, nwn = nchunks >> 4;
   const u32 slot0 = old & (RUN - 1), split = RUN - slot0;
   const u32 runA = (((old & ~(RUN - 1)) * THREAD_FREQ) * k) % nchunks;
@@ -445,7 +456,7 @@ HA_DEV u64 probe_run(const SizeInfo& s, u32 hsb, u32 limit, u32 old, u64 P, u32 
   u64* bw = ha_p.bits + (u64)hsb * ha_p.bit_words;
   u64* nw = ha_p.nibs + (u64)hsb * ha_p.nib_words;
   u64 reserved = 0;
-  u64 off = 0;
+  u64 off = off0;
   for (;;) {
     const u32 pA = (u32)(((u64)runA + off) % nchunks), pB = (u32)(((u64)runB + off) % nchunks);
     const u32 st = inA ? pA + slot0 * k : pB - split * k;
@@ -588,13 +599,78 @@ HA_DEV void malloc_batch_group(u32 sid, u64 g, u32 ihead, u32 me, u32 cnt, void*
   } else {
     u32 total = 0;
     char* sbase = ha_p.arena + (u64)hsb * ha_p.sb_sz;
-    for (u32 j = 0; j < R; ++j) {
-      const u64 P = ballot(cnt > j);
-      u32 itry = 0, my_ich = 0;
-      const u64 S = probe_run(s, hsb, limit, base + j * L, P, me, itry, my_ich);
-      if (S != P) need = true;
-      total += (u32)__builtin_popcountll(S);
-      if ((S >> me) & 1) out[j] = sbase + chunk_mul(my_ich, s.chunk_sz);
+    const u32 nchunks = s.nchunks, nwb = nchunks >> 6, nwn = nchunks >> 4;
+    u64* bw = ha_p.bits + (u64)hsb * ha_p.bit_words;
+    u64* nw = ha_p.nibs + (u64)hsb * ha_p.nib_words;
+    // rounds in groups of BATCH_PIPE: the first probe of every round in the
+    // group is issued before any result is awaited; rounds with lanes left
+    // over then go on from the second probe, in round order
+    for (u32 j0 = 0; j0 < R; j0 += BATCH_PIPE) {
+      u64 Pj[BATCH_PIPE], oldw[BATCH_PIPE], myseg[BATCH_PIPE];
+      u32 st[BATCH_PIPE], ib[BATCH_PIPE], wreal[BATCH_PIPE];
+#pragma unroll
+      for (u32 t = 0; t < BATCH_PIPE; ++t) {
+        const u32 j = j0 + t;
+        Pj[t] = j < R ? ballot(cnt > j) : 0ull;
+        oldw[t] = 0;
+        st[t] = 0;
+        ib[t] = 64;
+        wreal[t] = 0;
+        myseg[t] = 0;
+        if (!Pj[t]) continue;
+        const u32 old = base + j * L;
+        const u32 slot0 = old & (RUN - 1), split = RUN - slot0;
+        const u32 runA = (((old & ~(RUN - 1)) * THREAD_FREQ) * k) % nchunks;
+        const u32 runB = ((((old & ~(RUN - 1)) + RUN) * THREAD_FREQ) * k) % nchunks;
+        const bool inA = me < split;
+        const u64 amask = split >= 64 ? ~0ull : ((1ull << split) - 1);
+        myseg[t] = inA ? amask : ~amask;
+        st[t] = inA ? runA + slot0 * k : runB - split * k;
+        u32 wb;
+        u64 ub;
+        run_word_bits(Pj[t] & myseg[t], me, st[t], k, wb, ub, ib[t]);
+        wreal[t] = wb >= nwb ? wb - nwb : wb;
+        if (((Pj[t] >> me) & 1) && ib[t] == me) oldw[t] = a_or(bw + wreal[t], ub);
+      }
+#pragma unroll
+      for (u32 t = 0; t < BATCH_PIPE; ++t) {
+        const u32 j = j0 + t;
+        if (!Pj[t]) continue;
+        const u64 P = Pj[t];
+        const u64 ow = shfl64(oldw[t], ib[t] & 63);
+        u32 c = st[t] + me * k;
+        c = c >= nchunks ? c - nchunks : c;
+        const u64 mym = ((1ull << k) - 1) << (c & 63);
+        const bool ok = ((P >> me) & 1) && !(ow & mym);
+        const u64 S = ballot(ok);
+        const u64 F = P & ~S;
+        if (F) {
+          u32 wf, iff;
+          u64 uf;
+          run_word_bits(F & myseg[t], me, st[t], k, wf, uf, iff);
+          const u64 clr = uf & ~ow;
+          if (((F >> me) & 1) && iff == me && clr) a_and(bw + wreal[t], ~clr);
+        }
+        if (S) {
+          u32 wn, inn;
+          u64 un;
+          run_word_nibs(S & myseg[t], me, st[t], k, (u64)k, wn, un, inn);
+          const u32 wnreal = wn >= nwn ? wn - nwn : wn;
+          if (ok && inn == me) a_or(nw + wnreal, un);
+          if (ok) out[j] = sbase + chunk_mul(c, s.chunk_sz);
+        }
+        u64 Sall = S;
+        if (F) {
+          // resume at the second probe (itry 1) for the lanes left over
+          u32 itry = 1, my_ich = 0;
+          const u64 off1 = (u64)((1u * STEP_FREQ * k) % s.hash_step);
+          const u64 S2 = probe_run(s, hsb, limit, base + j * L, F, me, itry, my_ich, off1);
+          if ((S2 >> me) & 1) out[j] = sbase + chunk_mul(my_ich, s.chunk_sz);
+          Sall |= S2;
+        }
+        if (Sall != P) need = true;
+        total += (u32)__builtin_popcountll(Sall);
+      }
     }
     if (total) {
       const u32 change = k * total;
@@ -641,8 +717,14 @@ HA_DEV void* malloc_small(u32 nbytes, u32 ihead) {
 }
 
 // --------------------------------------------------------------- free path
-// hafree_small (src/halloc.cu:211-230) for the active lanes; off = p - arena.
-HA_DEV void free_small(u64 off) {
+// hafree_small (src/halloc.cu:211-230) for NR rounds at once: in round j the
+// lanes with act[j] free the block at arena offset off[j].  The slab
+// descriptors of every round are loaded together, then every round's
+// nibbles are cleared, then its bits; the counter decrements and the set
+// transitions they cause follow round by round, in the order NR one-round
+// calls would make them.
+template <int NR>
+HA_DEV void free_rounds(const u64* off, const bool* act) {
   // Each XCD has its own write-back L2.  With a cached arena the caller's
   // stores to the blocks must reach memory before another XCD can be handed
   // the blocks, or a later eviction of this XCD's dirty lines would overwrite
@@ -650,93 +732,192 @@ HA_DEV void free_small(u64 off) {
   // uncached arena needs none.
   if (ha_p.cached_arena) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   const u32 me = lane_id();
-  const u32 sb = (u32)(off >> ha_p.sb_sz_sh);
-  Desc* d = ha_p.desc + sb;
-  const u32 cs = ld32(&d->chunk_sz);
-  const bool valid = cs != 0;
-  const u32 ich = valid ? chunk_div((u32)(off & (ha_p.sb_sz - 1)), cs) : 0u;
-  u64* bw = ha_p.bits + (u64)sb * ha_p.bit_words;
-  u64* nw = ha_p.nibs + (u64)sb * ha_p.nib_words;
-  const u64 A = ballot(valid);
-  u32 nch = 0;
-  // fast path: the lanes free one slab's blocks laid out as at most two runs
-  // (what one class group of a wave allocates): one atomic per word touched
-  bool fast = false;
-  u32 k = 0, st = 0;
-  u64 myseg = 0;
-  if (A) {
-    const u32 l0 = (u32)__builtin_ctzll(A);
-    const u32 sb0 = rdl(sb, l0);
-    if (ballot(valid && sb == sb0) == A) {
-      const u32 sid0 = rfl(ld32(&ha_p.desc[sb0].size_id));
-      if (sid0 < NSIZES) {
-        k = ha_p.si[sid0].ncib;
-        const u32 sa = rdl(ich, l0) - l0 * k;
-        const bool ina = valid && ich == sa + me * k;
-        const u64 Am = ballot(ina);
-        const u64 R = A & ~Am;
-        u32 sbb = 0;
-        u64 Bm = 0;
-        if (R) {
-          const u32 l1 = (u32)__builtin_ctzll(R);
-          sbb = rdl(ich, l1) - l1 * k;
-          Bm = ballot(valid && !ina && ich == sbb + me * k);
+  u32 sb[NR], cs[NR], sid[NR], cid[NR];
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    sb[j] = act[j] ? (u32)(off[j] >> ha_p.sb_sz_sh) : 0u;
+    cs[j] = sid[j] = cid[j] = 0;
+    if (act[j]) {
+      Desc* d = ha_p.desc + sb[j];
+      cs[j] = ld32(&d->chunk_sz);
+      sid[j] = ld32(&d->size_id);
+      cid[j] = ld32(&d->chunk_id);
+    }
+  }
+  // nibbles: read back the block lengths and clear them
+  u32 ich[NR], nch[NR], k[NR], st[NR];
+  u64 A[NR], myseg[NR], o[NR];
+  bool fast[NR];
+  u32 inn[NR];
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    const bool valid = act[j] && cs[j] != 0;
+    ich[j] = valid ? chunk_div((u32)(off[j] & (ha_p.sb_sz - 1)), cs[j]) : 0u;
+    A[j] = ballot(valid);
+    nch[j] = 0;
+    k[j] = st[j] = 0;
+    myseg[j] = o[j] = 0;
+    inn[j] = 64;
+    fast[j] = false;
+    // fast path: the lanes free one slab's blocks laid out as at most two runs
+    // (what one class group of a wave allocates): one atomic per word touched
+    if (A[j]) {
+      const u32 l0 = (u32)__builtin_ctzll(A[j]);
+      const u32 sb0 = rdl(sb[j], l0);
+      if (ballot(valid && sb[j] == sb0) == A[j]) {
+        const u32 sid0 = rdl(sid[j], l0);
+        if (sid0 < NSIZES) {
+          k[j] = ha_p.si[sid0].ncib;
+          const u32 sa = rdl(ich[j], l0) - l0 * k[j];
+          const bool ina = valid && ich[j] == sa + me * k[j];
+          const u64 Am = ballot(ina);
+          const u64 Rm = A[j] & ~Am;
+          u32 sbb = 0;
+          u64 Bm = 0;
+          if (Rm) {
+            const u32 l1 = (u32)__builtin_ctzll(Rm);
+            sbb = rdl(ich[j], l1) - l1 * k[j];
+            Bm = ballot(valid && !ina && ich[j] == sbb + me * k[j]);
+          }
+          fast[j] = (Am | Bm) == A[j];
+          st[j] = ina ? sa : sbb;
+          myseg[j] = ina ? Am : Bm;
         }
-        fast = (Am | Bm) == A;
-        st = ina ? sa : sbb;
-        myseg = ina ? Am : Bm;
       }
     }
-  }
-  if (fast) {
-    u32 wn, inn;
-    u64 un;
-    run_word_nibs(myseg, me, st, k, 0xFull, wn, un, inn);
-    u64 oldn = 0;
-    if (valid && inn == me) oldn = a_and(nw + wn, ~un);
-    oldn = shfl64(oldn, inn & 63);
-    nch = valid ? (u32)((oldn >> (4 * (ich & 15))) & 0xF) : 0u;
-    if (ballot(valid && nch == k) == A) {
-      u32 wb, ib;
-      u64 ub;
-      run_word_bits(myseg, me, st, k, wb, ub, ib);
-      if (valid && ib == me) a_and(bw + wb, ~ub);
-    } else if (nch) {
-      a_and(bw + ich / 64, ~(((1ull << nch) - 1) << (ich % 64)));
+    u64* nw = ha_p.nibs + (u64)sb[j] * ha_p.nib_words;
+    if (fast[j]) {
+      u32 wn;
+      u64 un;
+      run_word_nibs(myseg[j], me, st[j], k[j], 0xFull, wn, un, inn[j]);
+      if (valid && inn[j] == me) o[j] = a_and(nw + wn, ~un);
+    } else if (valid) {
+      o[j] = a_and(nw + ich[j] / 16, ~(0xFull << (4 * (ich[j] % 16))));
     }
-  } else if (valid) {
-    const u64 o = a_and(nw + ich / 16, ~(0xFull << (4 * (ich % 16))));
-    nch = (u32)((o >> (4 * (ich % 16))) & 0xF);
-    if (nch) a_and(bw + ich / 64, ~(((1ull << nch) - 1) << (ich % 64)));
   }
-  // sb_ctr_dec (src/slab.cuh:161-224): one atomic per slab touched
-  const u64 D = ballot(nch != 0);
-  u64 rem = D;
-  u32 my_change = 0;
-  bool leader = false;
-  while (rem) {
-    u32 l = (u32)__builtin_ctzll(rem);
-    u32 b = rdl(sb, l);
-    u64 G = ballot(sb == b) & rem;
-    u32 ch = 0;
-    u64 Gr = G;
-    while (Gr) {
-      u32 l2 = (u32)__builtin_ctzll(Gr);
-      u32 v = rdl(nch, l2);
-      u64 V = ballot(nch == v) & Gr;
-      ch += v * (u32)__builtin_popcountll(V);
-      Gr &= ~V;
+  // bits
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    const bool valid = act[j] && cs[j] != 0;
+    u64* bw = ha_p.bits + (u64)sb[j] * ha_p.bit_words;
+    if (fast[j]) {
+      const u64 oldn = shfl64(o[j], inn[j] & 63);
+      nch[j] = valid ? (u32)((oldn >> (4 * (ich[j] & 15))) & 0xF) : 0u;
+      if (ballot(valid && nch[j] == k[j]) == A[j]) {
+        u32 wb, ib;
+        u64 ub;
+        run_word_bits(myseg[j], me, st[j], k[j], wb, ub, ib);
+        if (valid && ib == me) a_and(bw + wb, ~ub);
+      } else if (nch[j]) {
+        a_and(bw + ich[j] / 64, ~(((1ull << nch[j]) - 1) << (ich[j] % 64)));
+      }
+    } else if (valid) {
+      nch[j] = (u32)((o[j] >> (4 * (ich[j] % 16))) & 0xF);
+      if (nch[j]) a_and(bw + ich[j] / 64, ~(((1ull << nch[j]) - 1) << (ich[j] % 64)));
     }
-    if (me == l) { leader = true; my_change = ch; }
-    rem &= ~G;
   }
-  u64 oldc = 0;
-  if (leader) oldc = a_sub(ctr_of(sb), (u64)my_change << CTR_COUNT_SH);
-  u64 T = ballot(leader && needs_transition(sb, oldc, my_change));
-  while (T) {
-    u32 l = (u32)__builtin_ctzll(T);
-    if (me == l) ctr_transitions(sb, oldc, my_change);
-    T &= T - 1;
+  // sb_ctr_dec (src/slab.cuh:161-224): one atomic per slab touched, issued for
+  // every round in round order; then the transitions, round by round, each
+  // from its own round's old count (as NR one-round calls would see them)
+  u64 oldc[NR];
+  u32 change[NR];
+  bool lead[NR];
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    u64 rem = ballot(nch[j] != 0);
+    u32 my_change = 0;
+    bool leader = false;
+    while (rem) {
+      const u32 l = (u32)__builtin_ctzll(rem);
+      const u32 b = rdl(sb[j], l);
+      const u64 G = ballot(sb[j] == b) & rem;
+      u32 ch = 0;
+      u64 Gr = G;
+      while (Gr) {
+        const u32 l2 = (u32)__builtin_ctzll(Gr);
+        const u32 v = rdl(nch[j], l2);
+        const u64 V = ballot(nch[j] == v) & Gr;
+        ch += v * (u32)__builtin_popcountll(V);
+        Gr &= ~V;
+      }
+      if (me == l) {
+        leader = true;
+        my_change = ch;
+      }
+      rem &= ~G;
+    }
+    oldc[j] = 0;
+    if (leader) oldc[j] = a_sub(ctr_of(sb[j]), (u64)my_change << CTR_COUNT_SH);
+    change[j] = my_change;
+    lead[j] = leader;
+  }
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    const bool tr = lead[j] && !ctr_is_head(oldc[j]) && sid[j] < NSIZES && cid[j] != SZ_NONE &&
+                    sb_class(ctr_count(oldc[j]), ha_p.si[sid[j]]) !=
+                        sb_class(ctr_count(oldc[j]) - change[j], ha_p.si[sid[j]]);
+    u64 T = ballot(tr);
+    while (T) {
+      const u32 l = (u32)__builtin_ctzll(T);
+      if (me == l) ctr_transitions(sb[j], oldc[j], change[j]);
+      T &= T - 1;
+    }
+  }
+}
+
+HA_DEV void free_small(u64 off) {
+  const bool act = true;
+  free_rounds<1>(&off, &act);
+}
+
+// ------------------------------------------------------------ large blocks
+// Requests above MAX_BLOCK_SZ come from the large region (the share of the
+// pool outside the slabs, src/halloc.cu:285-289): class c holds blocks of
+// 4 KiB << c.  A free block goes on its class's lock-free stack (an ABA tag in
+// the head's high word); an empty stack is refilled from a bump pointer.
+// Blocks keep their class for the life of the heap.
+HA_DEV u32 large_class(u64 nbytes) {
+  const u64 units = (nbytes + (1ull << LARGE_UNIT_SH) - 1) >> LARGE_UNIT_SH;
+  return units <= 1 ? 0u : 64u - (u32)__builtin_clzll(units - 1);
+}
+
+HA_DEV void* large_alloc(u64 nbytes) {
+  const u32 c = large_class(nbytes);
+  if (c >= ha_p.lnclasses) return nullptr;
+  u64* hp = ha_p.lstack + (u64)c * LSTACK_STRIDE;
+  for (;;) {
+    const u64 h = ld64(hp);
+    const u32 idx = (u32)h;
+    if (!idx) break;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the link written before idx was pushed
+    const u32 nxt = ld32(ha_p.llink + (idx - 1));
+    if (atomicCAS((unsigned long long*)hp, h, (((h >> 32) + 1) << 32) | nxt) == h)
+      return ha_p.large + ((u64)(idx - 1) << LARGE_UNIT_SH);
+  }
+  const u64 nu = 1ull << c;
+  for (;;) {
+    const u64 b = ld64(ha_p.lbump);
+    if (b + nu > ha_p.lunits) return nullptr;
+    if (atomicCAS((unsigned long long*)ha_p.lbump, b, b + nu) == b) {
+      st32(ha_p.lclass + b, c);
+      return ha_p.large + (b << LARGE_UNIT_SH);
+    }
+  }
+}
+
+HA_DEV void large_free(u64 off) {
+  if (off & ((1ull << LARGE_UNIT_SH) - 1)) return;  // not a block start
+  const u32 unit = (u32)(off >> LARGE_UNIT_SH);
+  const u32 c = ld32(ha_p.lclass + unit);
+  if (c >= ha_p.lnclasses) return;
+  u64* hp = ha_p.lstack + (u64)c * LSTACK_STRIDE;
+  for (;;) {
+    const u64 h = ld64(hp);
+    st32(ha_p.llink + unit, (u32)h);
+    // the link, and with a cached arena the caller's stores to the block,
+    // before the block is visible on the stack
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (atomicCAS((unsigned long long*)hp, h, (((h >> 32) + 1) << 32) | (unit + 1)) == h) return;
   }
 }
 
@@ -744,9 +925,10 @@ HA_DEV void free_small(u64 off) {
 
 // ---------------------------------------------------------- public device API
 __device__ void* hamalloc(size_t nbytes) {
-  // requests above 3072 B go to the HIP device heap (src/halloc.cu:185-195)
-  if (nbytes > ha::MAX_BLOCK_SZ) return malloc(nbytes);
-  return ha::malloc_small((ha::u32)nbytes, ha::rfl(ha::xcc_id() % ha::ha_p.nheads));
+  // requests above 3072 B go to the large-block region (src/halloc.cu:185-195
+  // sends them to the CUDA device heap)
+  if (nbytes > ha::MAX_BLOCK_SZ) return ha::large_alloc(nbytes);
+  return ha::malloc_small((ha::u32)nbytes, ha::rfl(ha::head_set()));
 }
 
 // Batched allocation: lane l gets cnt (<= HA_MAX_BATCH) blocks of nbytes into
@@ -756,12 +938,12 @@ __device__ int hamalloc_batch(size_t nbytes, int cnt, void** out) {
   ha::u32 c = cnt > (int)ha::MAXB ? ha::MAXB : (ha::u32)cnt;
   int got = 0;
   if (nbytes > ha::MAX_BLOCK_SZ) {
-    for (ha::u32 j = 0; j < c; ++j) got += (out[j] = malloc(nbytes)) != nullptr;
+    for (ha::u32 j = 0; j < c; ++j) got += (out[j] = ha::large_alloc(nbytes)) != nullptr;
     return got;
   }
   const ha::u32 me = ha::lane_id();
   const ha::u32 sid = ha::size_id_from_nbytes((ha::u32)nbytes);
-  const ha::u32 ihead = ha::rfl(ha::xcc_id() % ha::ha_p.nheads);
+  const ha::u32 ihead = ha::rfl(ha::head_set());
   ha::u64 rem = ha::ballot(true);
   while (rem) {
     ha::u32 l0 = (ha::u32)__builtin_ctzll(rem);
@@ -774,24 +956,46 @@ __device__ int hamalloc_batch(size_t nbytes, int cnt, void** out) {
   return got;
 }
 
-// Batched free: lane l frees ptrs[0..cnt) (NULL entries are skipped), one
-// round per index.
+// Batched free: lane l frees ptrs[0..cnt) (NULL entries are skipped).  Slab
+// blocks go four rounds per call of the free path; large blocks and foreign
+// pointers one by one.
 __device__ void hafree_batch(void* const* ptrs, int cnt) {
-  for (int j = 0; j < cnt; ++j) hafree(ptrs[j]);
+  if (cnt <= 0) return;
+  const ha::u32 c = cnt > (int)ha::MAXB ? ha::MAXB : (ha::u32)cnt;
+  ha::u32 R = 0;  // the wave's largest count
+  for (ha::u32 j = 0; j < ha::MAXB; ++j)
+    if (ha::ballot(c > j)) R = j + 1;
+  for (ha::u32 j = 0; j < c; ++j) {
+    void* p = ptrs[j];
+    if (p && (ha::u64)((char*)p - ha::ha_p.arena) >= ha::ha_p.arena_bytes) hafree(p);
+  }
+  for (ha::u32 j0 = 0; j0 < R; j0 += 4) {
+    ha::u64 off[4];
+    bool act[4];
+#pragma unroll
+    for (ha::u32 t = 0; t < 4; ++t) {
+      void* p = j0 + t < c ? ptrs[j0 + t] : nullptr;
+      off[t] = (ha::u64)((char*)p - ha::ha_p.arena);
+      act[t] = p && off[t] < ha::ha_p.arena_bytes;
+    }
+    ha::free_rounds<4>(off, act);
+  }
 }
 
 // hamalloc with the head set chosen by the caller instead of by the XCD id;
 // the replay harness uses it to pin behaviour across head sets
 __device__ void* ha_malloc_on_head(size_t nbytes, unsigned ihead) {
-  if (nbytes > ha::MAX_BLOCK_SZ) return malloc(nbytes);
+  if (nbytes > ha::MAX_BLOCK_SZ) return ha::large_alloc(nbytes);
   return ha::malloc_small((ha::u32)nbytes, ha::rfl(ihead % ha::ha_p.nheads));
 }
 
 __device__ void hafree(void* p) {
   if (!p) return;  // src/halloc.cu:240
-  ha::u64 off = (ha::u64)((char*)p - ha::ha_p.arena);
+  const ha::u64 off = (ha::u64)((char*)p - ha::ha_p.arena);
+  const ha::u64 loff = (ha::u64)((char*)p - ha::ha_p.large);
   if (off < ha::ha_p.arena_bytes) ha::free_small(off);
-  else free(p);
+  else if (loff < ha::ha_p.large_bytes) ha::large_free(loff);
+  else free(p);  // not from this heap: the device heap's (src/halloc.cu:244-246)
 }
 
 // ------------------------------------------------------------------ host side
@@ -971,11 +1175,12 @@ static int init_locked(const ha_opts_c& o) {
   P.nib_words = P.sb_sz / BLOCK_STEP / 16;
   P.cuda_memory = cuda_memory;
   if (!build_size_infos(P.sb_sz, o, P.si)) return fail(HA_E_INVALID, "no size table for this slab size");
-  if (cuda_memory) {
-    hipError_t e = hipDeviceSetLimit(hipLimitMallocHeapSize, cuda_memory);
-    if (e != hipSuccess) return fail(HA_E_HIP, std::string("hipDeviceSetLimit: ") + hipGetErrorString(e));
-  }
   P.arena_bytes = (u64)P.nsbs * P.sb_sz;
+  // the large-block region follows the slabs in the same allocation
+  P.lunits = (u32)std::min<u64>(cuda_memory >> LARGE_UNIT_SH, 0xfffffff0ull);
+  P.large_bytes = (u64)P.lunits << LARGE_UNIT_SH;
+  P.lnclasses = 0;
+  while (P.lnclasses < LARGE_NCLASSES && (1ull << P.lnclasses) <= P.lunits) ++P.lnclasses;
   // metadata offsets, each section 256 B aligned
   u64 off = 0;
   u64 o_bits = off; off = align_up(off + (u64)P.nsbs * P.bit_words * 8, 256);
@@ -991,8 +1196,13 @@ static int init_locked(const ha_opts_c& o) {
   u64 o_ticket = off; off = align_up(off + hot_bytes, 256);
   u64 o_clock = off; off = align_up(off + (u64)NSIZES * HOT_STRIDE * 4, 256);
   u64 o_sep = off; off = align_up(off + hot_bytes, 256);
+  u64 o_active = off; off = align_up(off + 4, 256);
+  u64 o_lstack = off; off = align_up(off + (u64)LARGE_NCLASSES * LSTACK_STRIDE * 8, 256);
+  u64 o_lbump = off; off = align_up(off + 8, 256);
+  u64 o_llink = off; off = align_up(off + (u64)P.lunits * 4, 256);
+  u64 o_lclass = off; off = align_up(off + (u64)P.lunits * 4, 256);
   void* arena = nullptr;
-  int rc = arena_get(dev, P.arena_bytes, o.cached_arena != 0, &arena);
+  int rc = arena_get(dev, P.arena_bytes + P.large_bytes, o.cached_arena != 0, &arena);
   if (rc) return rc;
   void* meta = nullptr;
   if (hipMalloc(&meta, off) != hipSuccess) {
@@ -1013,6 +1223,12 @@ static int init_locked(const ha_opts_c& o) {
   P.ticket = (u32*)(m + o_ticket);
   P.class_lock = (u32*)(m + o_clock);
   P.search_ep = (u32*)(m + o_sep);
+  P.active = (u32*)(m + o_active);
+  P.large = P.arena + P.arena_bytes;
+  P.lstack = (u64*)(m + o_lstack);
+  P.lbump = (u64*)(m + o_lbump);
+  P.llink = (u32*)(m + o_llink);
+  P.lclass = (u32*)(m + o_lclass);
   P.initialized = 1;
   HA_HIP(hipMemset(meta, 0, off));
   u32 n = std::max(std::max(P.nsbs, P.nsw), P.nheads * NSIZES);
@@ -1035,18 +1251,6 @@ __global__ void free_mem_k(u64* out) {
   u32 used = cs ? chunk_mul(ctr_count(ha_p.ctr[(u64)i * CTR_STRIDE]), cs) : 0u;
   atomicAdd((unsigned long long*)&out[0], (unsigned long long)(ha_p.sb_sz - used));
   if (cs == 0) atomicAdd((unsigned long long*)&out[1], (unsigned long long)ha_p.sb_sz);
-}
-// largest block the device heap can serve now, by bisection (src/statistics.cuh:14-30)
-__global__ void max_alloc_k(u64 hi, u64* out) {
-  if (blockIdx.x || threadIdx.x) return;
-  u64 lo = 0;
-  const u64 min_diff = 1ull << 20;
-  while (hi - lo > min_diff) {
-    u64 mid = lo + (hi - lo) / 2;
-    void* q = malloc(mid);
-    if (q) { lo = mid; free(q); } else hi = mid;
-  }
-  out[0] = lo;
 }
 
 static int stats_locked(u64* free_mem, u64* free_slab_mem) {
@@ -1077,7 +1281,7 @@ void ha_default_opts(ha_opts_c* o) {
   o->roomy_fraction = d.roomy_fraction;
   o->sparse_fraction = d.sparse_fraction;
   o->sb_sz_sh = d.sb_sz_sh;
-  o->nheads = 8;
+  o->nheads = 16;
   o->hash_step_frac = 0.0;
   o->cached_arena = 0;
   o->pad = 0;
@@ -1106,18 +1310,23 @@ int ha_extfrag_c(int ideal, double* out) {
   ha::u64 free_mem = 0, free_slabs = 0;
   int rc = ha::stats_locked(&free_mem, &free_slabs);
   if (rc) return rc;
+  // largest block the large region can hand out now (the counterpart of
+  // find_max_alloc_k's bisection of the CUDA heap, src/statistics.cuh:14-30)
+  const ha::Params& P = ha::g_host;
   ha::u64 max_alloc = 0;
-  if (ha::g_host.cuda_memory) {
-    ha::u64* d = nullptr;
-    HA_HIP(hipMalloc(&d, 8));
-    HA_HIP(hipMemset(d, 0, 8));
-    ha::max_alloc_k<<<1, 64>>>(ha::g_host.cuda_memory, d);
-    HA_HIP(hipGetLastError());
-    HA_HIP(hipMemcpy(&max_alloc, d, 8, hipMemcpyDeviceToHost));
-    HA_HIP(hipFree(d));
+  if (P.lunits) {
+    std::vector<ha::u64> heads((size_t)ha::LARGE_NCLASSES * ha::LSTACK_STRIDE);
+    ha::u64 bump = 0;
+    HA_HIP(hipDeviceSynchronize());
+    HA_HIP(hipMemcpy(heads.data(), P.lstack, heads.size() * 8, hipMemcpyDeviceToHost));
+    HA_HIP(hipMemcpy(&bump, P.lbump, 8, hipMemcpyDeviceToHost));
+    max_alloc = bump < P.lunits ? (ha::u64)(P.lunits - bump) << ha::LARGE_UNIT_SH : 0;
+    for (ha::u32 c = 0; c < P.lnclasses; ++c)
+      if ((ha::u32)heads[(size_t)c * ha::LSTACK_STRIDE] != 0)
+        max_alloc = std::max<ha::u64>(max_alloc, 1ull << (c + ha::LARGE_UNIT_SH));
   }
   if (ideal) max_alloc += free_slabs;
-  free_mem += ha::g_host.cuda_memory;
+  free_mem += P.large_bytes;
   *out = free_mem ? 1.0 - (double)max_alloc / (double)free_mem : 0.0;
   return HA_OK;
 }
@@ -1130,7 +1339,7 @@ int ha_geometry_c(ha_geometry* g) {
   std::memset(g, 0, sizeof *g);
   g->nsbs = P.nsbs; g->sb_sz = P.sb_sz; g->sb_sz_sh = P.sb_sz_sh; g->nheads = P.nheads;
   g->nsw = P.nsw; g->bit_words = P.bit_words; g->nib_words = P.nib_words;
-  g->arena_bytes = P.arena_bytes; g->cuda_memory = P.cuda_memory;
+  g->arena_bytes = P.arena_bytes; g->large_bytes = P.large_bytes;
   for (ha::u32 s = 0; s < ha::NSIZES; ++s) std::memcpy(g->size_table + 12 * s, &P.si[s], 12 * sizeof(uint32_t));
   return HA_OK;
 }

@@ -420,8 +420,15 @@ __device__ __forceinline__ u32 poisson8(u32 seed, u32 job, u32 node) {
   return k;
 }
 __device__ __forceinline__ u32 adj_bytes(u32 deg) { return deg ? 4u * deg : 16u; }
+// The adjacency array is written and read as 8-byte pairs: blocks are 8-byte
+// aligned and their class size is a multiple of 8 that is at least 4*deg, so
+// the pair holding the last entry of an odd degree stays inside the block.
 __device__ __forceinline__ void fill_node(NodeHdr* h, u32* a, u32 seed, u32 job, u32 node, u32 deg, u32 n) {
-  for (u32 e = 0; e < deg; ++e) a[e] = gh(seed, job, node, e) % n;
+  for (u32 e = 0; e < deg; e += 2) {
+    const u32 v0 = gh(seed, job, node, e) % n;
+    if (e + 1 < deg) *(uint2*)(a + e) = make_uint2(v0, gh(seed, job, node, e + 1) % n);
+    else a[e] = v0;
+  }
   h->node = node;
   h->deg = deg;
   h->adj = a;
@@ -429,7 +436,11 @@ __device__ __forceinline__ void fill_node(NodeHdr* h, u32* a, u32 seed, u32 job,
 __device__ __forceinline__ u32 check_node(const NodeHdr* h, u32 seed, u32 job, u32 node, u32 n, u32 deg) {
   if (h->node != node || h->deg != deg || !h->adj) return 1;
   u32 bad = 0;
-  for (u32 e = 0; e < deg; ++e) bad += h->adj[e] != gh(seed, job, node, e) % n;
+  for (u32 e = 0; e < deg; e += 2) {
+    const uint2 v = *(const uint2*)(h->adj + e);
+    bad += v.x != gh(seed, job, node, e) % n;
+    if (e + 1 < deg) bad += v.y != gh(seed, job, node, e + 1) % n;
+  }
   return bad ? 1 : 0;
 }
 __device__ __forceinline__ u32 job_of(const u64* offs, u32 njobs, u64 gi) {
@@ -547,13 +558,7 @@ __device__ void graph_chunk_build(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr**
     const u32 T = class_ranks<B>(cls, cnt, c, lt, rank);
     const u32 q = (T + 63) / 64, lo = me * q;  // lane share: [lo, lo + k)
     const u32 k = lo >= T ? 0u : (T - lo < q ? T - lo : q);
-    void* tmp[B];
-#pragma unroll
-    for (int t = 0; t < B; ++t) tmp[t] = nullptr;
-    if (k) hamalloc_batch(ha::ha_p.si[c].block_sz, (int)k, tmp);
-#pragma unroll
-    for (int t = 0; t < B; ++t)
-      if ((u32)t < k) lds[lo + t] = tmp[t];
+    if (k) hamalloc_batch(ha::ha_p.si[c].block_sz, (int)k, lds + lo);  // straight into LDS
     wave_lds_sync();
 #pragma unroll
     for (int j = 0; j < B; ++j)
@@ -585,21 +590,59 @@ __device__ void graph_chunk_free(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr** 
   u32 cls[B];
   void* hdr[B];
   void* adj[B];
-  u32 cnt = 0;
+  u32 cnt = 0, deg[B];
+  // the lane's nodes are checked together: table entries, then headers, then
+  // the adjacency pairs, each set of loads issued before any is used
 #pragma unroll
   for (int j = 0; j < B; ++j) {
     const u32 v = base + (u32)j * 64u + me;
     cnt += v < n;
-    const u32 deg = v < n ? poisson8(seed, job, v) : 0u;
-    cls[j] = ha::size_id_from_nbytes(adj_bytes(deg));
-    NodeHdr* h = v < n ? tab[v] : nullptr;
-    hdr[j] = h;
-    adj[j] = nullptr;
-    if (h) {
-      if (check_node(h, seed, job, v, n, deg)) bad++;  // a damaged node's adjacency is not trusted
-      else adj[j] = h->adj;
+    hdr[j] = v < n ? tab[v] : nullptr;
+  }
+  constexpr int H = B < 8 ? B : 8;  // headers in groups of 8 bound the registers they hold
+  u32 maxd = 0;
+#pragma unroll
+  for (int j0 = 0; j0 < B; j0 += H) {
+    uint4 hv[H];
+#pragma unroll
+    for (int t = 0; t < H; ++t) hv[t] = hdr[j0 + t] ? *(const uint4*)hdr[j0 + t] : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+    for (int t = 0; t < H; ++t) {
+      const int j = j0 + t;
+      const u32 v = base + (u32)j * 64u + me;
+      deg[j] = (u32)j < cnt ? poisson8(seed, job, v) : 0u;
+      cls[j] = ha::size_id_from_nbytes(adj_bytes(deg[j]));
+      adj[j] = nullptr;
+      if (hdr[j]) {
+        const u32* a = (const u32*)(((u64)hv[t].w << 32) | hv[t].z);
+        if (hv[t].x != v || hv[t].y != deg[j] || !a) bad++;  // a damaged node's adjacency is not trusted
+        else adj[j] = (void*)a;
+      }
+      if (adj[j] && deg[j] > maxd) maxd = deg[j];
     }
   }
+  u32 mism[B];
+#pragma unroll
+  for (int j = 0; j < B; ++j) mism[j] = 0;
+  for (u32 e = 0; e < maxd; e += 2) {
+    uint2 pv[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j)
+      pv[j] = adj[j] && e < deg[j] ? *(const uint2*)((const u32*)adj[j] + e) : make_uint2(0u, 0u);
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      if (!adj[j] || e >= deg[j]) continue;
+      const u32 v = base + (u32)j * 64u + me;
+      mism[j] += pv[j].x != gh(seed, job, v, e) % n;
+      if (e + 1 < deg[j]) mism[j] += pv[j].y != gh(seed, job, v, e + 1) % n;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < B; ++j)
+    if (mism[j]) {
+      bad++;
+      adj[j] = nullptr;
+    }
   for (u32 cs = classes_present<B>(cls, cnt); cs; cs &= cs - 1) {
     const u32 c = (u32)__builtin_ctz(cs);
     u32 rank[B];
@@ -610,11 +653,8 @@ __device__ void graph_chunk_free(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr** 
     wave_lds_sync();
     const u32 q = (T + 63) / 64, lo = me * q;
     const u32 k = lo >= T ? 0u : (T - lo < q ? T - lo : q);
-    void* tmp[B];
-#pragma unroll
-    for (int t = 0; t < B; ++t) tmp[t] = (u32)t < k ? lds[lo + t] : nullptr;
+    if (k) hafree_batch(lds + lo, (int)k);  // straight from LDS
     wave_lds_sync();
-    if (k) hafree_batch(tmp, (int)k);
   }
   if (cnt) hafree_batch(hdr, (int)cnt);
 }

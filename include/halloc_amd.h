@@ -34,7 +34,9 @@ typedef struct ha_opts_c {
   double roomy_fraction;     /* default 0.6 */
   double sparse_fraction;    /* default 0.012 */
   int32_t sb_sz_sh;          /* default 22 (4 MiB slabs), 21..26 */
-  int32_t nheads;            /* head slabs per size class; lanes pick head xcc_id % nheads; default 8 */
+  int32_t nheads;            /* head slabs per size class (1..64), default 16: a wave uses its XCD's
+                                set (xcc_id % nheads); above 8 (a multiple of 8), one of its XCD's
+                                nheads / 8 sets, picked by the wave's CU */
   double hash_step_frac;     /* 0 (default): the probe stride's prime is the largest below
                                 nblocks/256 + nblocks/64 (src/halloc.cu:378-381); else below
                                 hash_step_frac * nblocks, 0 < hash_step_frac < 1 */
@@ -45,7 +47,7 @@ typedef struct ha_opts_c {
   int32_t pad;
 } ha_opts_c;
 
-/* Fills *o with the defaults of src/halloc.h:30-32 and nheads = 8. */
+/* Fills *o with the defaults of src/halloc.h:30-32 and nheads = 16. */
 void ha_default_opts(ha_opts_c *o);
 
 /* Replaces ha_init (src/halloc.cu:267-424).  Initializes the current device. */
@@ -65,7 +67,7 @@ const char *ha_last_error(void);
    hash_step roomy busy block_sz nblocks tmod chunk_bytes). */
 typedef struct ha_geometry {
   uint32_t nsbs, sb_sz, sb_sz_sh, nheads, nsw, bit_words, nib_words, pad;
-  uint64_t arena_bytes, cuda_memory;
+  uint64_t arena_bytes, large_bytes;
   uint32_t size_table[16 * 12];
 } ha_geometry;
 int ha_geometry_c(ha_geometry *g);

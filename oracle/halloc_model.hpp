@@ -91,6 +91,7 @@ constexpr u32 MAXB = 16;            // most blocks one lane takes in one batched
 // such a head is filled up to (busy + nchunks) / 2 instead of busy
 constexpr u32 HEAD_EXT = 0x80000000u;
 constexpr u32 MAX_STEALS = 4;       // idle heads one head search may free before it gives up
+constexpr u32 BATCH_PIPE = 4;       // rounds of a batched call whose first probes go out together
 
 // slab counter layout (64-bit)
 constexpr u32 CTR_CHUNK_SH = 8;
@@ -225,6 +226,7 @@ class Heap {
   std::vector<std::atomic<u32>> d_size, d_chunk, d_csz, d_head, sb_lock;
   std::vector<std::atomic<u32>> head, cached, head_lock, ticket, search_ep;
   std::atomic<u32> class_lock[NSIZES] = {};
+  std::atomic<u32> active{0};  // pairs that hold a head
   std::vector<uint8_t> arena;  // payload memory (allocated lazily by users that touch data)
   std::atomic<u64> large_ctr{0};
 
@@ -410,6 +412,7 @@ class Heap {
           bool freed = false;
           if (ok) {
             word.store(SB_NONE);
+            if (which == 1) active.fetch_sub(1);  // the pair no longer holds a head
             lock(sb_lock[sb]);
             freed = detach_head(sb);
             unlock(sb_lock[sb]);
@@ -428,7 +431,16 @@ class Heap {
     for (u32 i = 0; i < nsw; ++i) n += (u32)__builtin_popcountll(w[i].load());
     return n;
   }
-  bool free_scarce() { return nheads > 1 && free_slab_count() < nheads * NSIZES / 2; }
+  // with several head sets, free slabs are scarce below the largest of
+  // nheads, half the (head set, class) pairs that hold a head, and a quarter
+  // of the heap
+  bool free_scarce() {
+    if (nheads <= 1) return false;
+    u32 lim = active.load() / 2;
+    lim = lim > nheads ? lim : nheads;
+    lim = lim > nsbs / 4 ? lim : nsbs / 4;
+    return free_slab_count() < lim;
+  }
 
   // find_sb_for_size (src/slab.cuh:259-372): roomy -> sparse -> free -> busy;
   // then, for a head needed now, another XCD's head of the class, then an
@@ -546,6 +558,7 @@ class Heap {
           }
           if (nh != SB_NONE) {
             head[hi].store(nh);
+            if (cur == SB_NONE) active.fetch_add(1);
             if (cur != SB_NONE) {
               u32 csb = cur & ~HEAD_EXT;
               lock(sb_lock[csb]); detach_head(csb); unlock(sb_lock[csb]);
@@ -570,12 +583,17 @@ class Heap {
   // One probe sequence (sb_alloc_in, src/slab.cuh:453-535) in slab hsb for the
   // lanes P with initial chunks ich0; ich receives each lane's last position.
   // Returns the lanes that reserved; itry advances for the whole group.
-  u64 probe_in(const SizeInfo& s, u32 hsb, u32 limit, u64 P, const u32* ich0, u32* ich, u32& itry) {
+  // skip_first: the lanes P already failed a first attempt at ich0 (made and
+  // rolled back by the caller); go on from the second probe.
+  u64 probe_in(const SizeInfo& s, u32 hsb, u32 limit, u64 P, const u32* ich0, u32* ich, u32& itry,
+               bool skip_first = false) {
     const u64 mask1 = (1ull << s.ncib) - 1;
     std::atomic<u64>* bw = &bits[(size_t)hsb * bit_words];
     u64 probing = P, reserved = 0;
     for (u32 l = 0; l < WAVE; ++l) if (probing >> l & 1) ich[l] = ich0[l];
-    for (;;) {
+    for (bool first = true;; first = false) {
+      if (skip_first && first) goto next_probe;
+      {
       u64 oldw[WAVE];
       for (u32 l = 0; l < WAVE; ++l) if (probing >> l & 1) {
         u64 m = mask1 << (ich[l] % 64);
@@ -590,6 +608,8 @@ class Heap {
       reserved |= now;
       probing &= ~now;
       if (!probing) break;
+      }
+    next_probe:
       if (itry % CHECK_NTRIES == CHECK_NTRIES - 1) {
         if (ctr_count(ctr[hsb].load()) >= limit) break;
       }
@@ -697,13 +717,41 @@ class Heap {
       need = true;
     } else {
       u32 total = 0;
-      for (u32 j = 0; j < R; ++j) {
-        u32 ich0[WAVE] = {0};
-        for (u32 l = 0; l < WAVE; ++l) if (pend[j] >> l & 1) ich0[l] = ichunk_init(base + j * L + l, s);
-        u32 itry = 0;
-        resv[j] = probe_in(s, hsb, limit, pend[j], ich0, ich_r[j], itry);
-        if (resv[j] != pend[j]) need = true;
-        total += (u32)__builtin_popcountll(resv[j]);
+      const u64 mask1 = (1ull << s.ncib) - 1;
+      std::atomic<u64>* bw = &bits[(size_t)hsb * bit_words];
+      // rounds in groups of BATCH_PIPE: the first probe of every round in the
+      // group is made before any is checked; then, round by round, the lanes
+      // that failed give their new bits back and go on from the second probe
+      for (u32 j0 = 0; j0 < R; j0 += BATCH_PIPE) {
+        const u32 j1 = j0 + BATCH_PIPE < R ? j0 + BATCH_PIPE : R;
+        static thread_local u32 ich0_r[BATCH_PIPE][WAVE];
+        static thread_local u64 oldw_r[BATCH_PIPE][WAVE];
+        for (u32 j = j0; j < j1; ++j)
+          for (u32 l = 0; l < WAVE; ++l) if (pend[j] >> l & 1) {
+            const u32 c = ichunk_init(base + j * L + l, s);
+            ich0_r[j - j0][l] = c;
+            ich_r[j][l] = c;
+            oldw_r[j - j0][l] = bw[c / 64].fetch_or(mask1 << (c % 64));
+          }
+        for (u32 j = j0; j < j1; ++j) {
+          u64 ok = 0;
+          for (u32 l = 0; l < WAVE; ++l) if (pend[j] >> l & 1) {
+            const u32 c = ich0_r[j - j0][l];
+            const u64 m = mask1 << (c % 64), ow = oldw_r[j - j0][l];
+            if (!(ow & m)) ok |= 1ull << l;
+            else if (~ow & m) bw[c / 64].fetch_and(~(m & ~ow));
+          }
+          resv[j] = ok;
+          const u64 fail = pend[j] & ~ok;
+          if (fail) {
+            u32 itry = 0;
+            u32 ich_f[WAVE];
+            resv[j] |= probe_in(s, hsb, limit, fail, ich0_r[j - j0], ich_f, itry, true);
+            for (u32 l = 0; l < WAVE; ++l) if (fail >> l & 1) ich_r[j][l] = ich_f[l];
+          }
+          if (resv[j] != pend[j]) need = true;
+          total += (u32)__builtin_popcountll(resv[j]);
+        }
       }
       if (total) {
         u32 change = s.ncib * total;

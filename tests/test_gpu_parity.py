@@ -259,10 +259,10 @@ def test_size_sweep_lists_random_free_order(halloc):
 
 
 def test_phase_workload_invariants(halloc):
-    # 24 slabs of 2 MiB for the 9 classes of 16..256 B
+    # 48 slabs of 2 MiB for the 9 classes of 16..256 B
     say("phase 2^16 lanes x 4 slots")
-    opts = halloc(H.HallocOpts(memory=64 << 20, sb_sz_sh=21))
-    r = H.Halloc.run_phase(nthreads=1 << 16, slots=4, nphases=3, max_steps=60)
+    opts = halloc(H.HallocOpts(memory=128 << 20, sb_sz_sh=21))
+    r = H.Halloc.run_phase(nthreads=1 << 16, slots=8, nphases=3, max_steps=60)
     say(f"phase steps={r['phase_steps']} nulls={r['nulls']}")
     assert r["errors"] == 0
     assert r["mallocs"] == r["frees"]
@@ -308,8 +308,45 @@ def test_autotune_small_space():
     from halloc_amd import autotune
     say("autotune small space")
     nn = graph_sizes(np.random.default_rng(9), 24)
-    best, ms, hist = autotune.tune(nn, memory=1 << 30, sweeps=1, log=say,
-                                   space={"batch": [4, 16], "nheads": [1, 8], "hash_step_frac": [0.0, 0.05]})
+    space = {"batch": [4, 16], "nheads": [1, 8], "hash_step_frac": [0.0, 0.05]}
+    best, ms, hist = autotune.tune(nn, memory=1 << 30, sweeps=1, log=say, space=space)
     assert np.isfinite(ms) and ms <= hist[0][1]
-    assert len(hist) == 1 + 4  # batch 4 and 16 besides the start's 8, nheads 1, hash_step_frac 0.05
+    # one sweep tries every value that differs from the configuration it has reached
+    assert 1 + 3 <= len(hist) <= 1 + sum(len(v) for v in space.values())
     assert best["batch"] in (4, 8, 16)
+
+
+def _large_block(n):
+    """Bytes of the large-path block that serves a request of n bytes."""
+    units = (int(n) + 4095) // 4096
+    return 4096 << max(0, (units - 1).bit_length())
+
+
+def test_large_blocks_reuse_and_exhaustion(halloc):
+    # requests above 3 KiB: 4 KiB-aligned power-of-two blocks from the region
+    # outside the slabs; freed blocks come back on reuse; running out gives
+    # NULL, never an overlap
+    say("large blocks")
+    halloc(H.HallocOpts(memory=64 << 20))
+    g = H.Halloc.geometry()
+    assert g["large_bytes"] == (64 << 20) - int(0.75 * (64 << 20))  # the share outside the slabs
+    rng = np.random.default_rng(12)
+    sizes = rng.integers(3073, 65537, 160).astype(np.uint32)
+    p1 = H.Halloc.malloc_sizes(sizes)
+    assert (p1 != 0).all() and (p1 % 4096 == 0).all()
+    assert (H.Halloc.read_words(p1) == sizes).all()
+    ext = np.array([_large_block(s) for s in sizes], dtype=np.uint64)
+    o = np.argsort(p1)
+    assert (p1[o][:-1] + ext[o][:-1] <= p1[o][1:]).all()
+    H.Halloc.free_ptrs(p1)
+    p2 = H.Halloc.malloc_sizes(sizes)
+    assert set(p2.tolist()) == set(p1.tolist())
+    H.Halloc.free_ptrs(p2)
+    big = np.full(400, 65536, dtype=np.uint32)
+    p3 = H.Halloc.malloc_sizes(big)
+    got = p3[p3 != 0]
+    assert 0 < len(got) < len(big)
+    assert len(got) * 65536 <= g["large_bytes"]
+    s3 = np.sort(got)
+    assert (s3[:-1] + 65536 <= s3[1:]).all()
+    H.Halloc.free_ptrs(p3)
