@@ -214,6 +214,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-probe", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="only the plain 16 B workload")
+    ap.add_argument("--extras", default="batching,serving,phase,sweep",
+                    help="comma-separated workloads to run after the plain one")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the allocator keeps per-device "
@@ -275,14 +277,20 @@ def main():
                                           "rate of 64-bit atomics on distinct cache lines",
                                   "probe": {"lane_random": float(probe[0]), "wave_run": float(probe[1]),
                                             "one_word": float(probe[2])}}
-    if not args.no_extras:
-        bt = bench_batching(H)
-        sv = bench_serving(H)
-        out["batched_vs_plain"] = bt["batched_vs_plain"]
-        out["p99_serve_alloc_ns"] = sv["p99_alloc_ns"]
-        out["batching"] = bt
-        out["serving"] = sv
+    extras = [] if args.no_extras else [e for e in args.extras.split(",") if e]
+    for e in extras:
+        if e not in ("batching", "serving", "phase", "sweep"):
+            print(f"unknown workload {e!r}", file=sys.stderr)
+            sys.exit(2)
+    if "batching" in extras:
+        out["batching"] = bench_batching(H)
+        out["batched_vs_plain"] = out["batching"]["batched_vs_plain"]
+    if "serving" in extras:
+        out["serving"] = bench_serving(H)
+        out["p99_serve_alloc_ns"] = out["serving"]["p99_alloc_ns"]
+    if "phase" in extras:
         out["phase"] = bench_phase(H)
+    if "sweep" in extras:
         out["sweep"] = bench_sweep(H)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_budget)
