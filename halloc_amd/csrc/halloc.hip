@@ -472,7 +472,7 @@ HA_DEV u64 probe_run(const SizeInfo& s, u32 hsb, u32 limit, u32 old, u64 P, u32 
     c = c >= nchunks ? c - nchunks : c;
     const u64 mym = ((1ull << k) - 1) << (c & 63);
     const bool ok = inP && !(oldw & mym);
-    const u64 S = ballot(ok);
+    const u64 S = ballot(ok) & P;  // P: this lane's group (other groups probe alongside)
     const u64 F = P & ~S;
     if (F) {
       // give back the bits the failing lanes set (src/slab.cuh:480-483)

@@ -52,9 +52,22 @@
 //   * when no set holds a slab for a new head, an EMPTY head or cached head of
 //     another (head, class) pair is taken over (under a try-lock of its head
 //     lock), so slabs parked as idle heads do not make hamalloc return NULL.
-//   * with several head sets, free slabs are "scarce" when fewer than
-//     nheads * 16 / 2 remain: then a cached head is not refilled from the free
-//     set, and a new head shares its class's head before taking a free slab.
+//   * with several head sets, free slabs are "scarce" below the largest of
+//     nheads, half the (head set, class) pairs holding a head, and a quarter of
+//     the heap: then a cached head is not refilled from the free set, and a new
+//     head shares its class's head before taking a free slab.
+//   * one head search per class at a time (a class lock); a search frees at
+//     most 4 idle heads of other pairs; waiters on a head lock whose holder's
+//     search failed return NULL instead of searching again.
+//   * hash_step_frac: the probe stride's prime may be bounded by a fraction of
+//     nblocks instead of the reference's nblocks/256 + nblocks/64.
+//   * batched calls (hamalloc_batch): one ticket for all rounds, the first
+//     probes of 4 rounds made before any is checked, one counter update.
+//
+// Parity unpinned: the large-block path (requests above 3072 B) is not in the
+// model; the model records such a request as REC_LARGE.  Its device behaviour
+// is pinned by invariant tests only (alignment, disjointness, reuse, NULL on
+// exhaustion).
 #pragma once
 
 #include <atomic>
