@@ -659,8 +659,10 @@ __device__ void graph_chunk_free(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr** 
   if (cnt) hafree_batch(hdr, (int)cnt);
 }
 
+// at least 3 waves per SIMD (<= 168 VGPRs): measured faster than the
+// unconstrained 2 at B = 16, despite a few spills
 template <int B>
-__global__ void __launch_bounds__(256) graph_batch_k(const u64* seq, u32 nseq, const u32* coffs, const u64* offs,
+__global__ void __launch_bounds__(256, 3) graph_batch_k(const u64* seq, u32 nseq, const u32* coffs, const u64* offs,
                                                      const u32* nnodes, u32 seed, NodeHdr** table, u32* next,
                                                      u32* built, unsigned long long* stats) {
   __shared__ void* lds_all[4][64 * B];
