@@ -509,70 +509,87 @@ __device__ __forceinline__ u32 rfl32(u32 v) { return (u32)__builtin_amdgcn_readf
 // LDS of one wave is written and read back by other lanes of the same wave
 __device__ __forceinline__ void wave_lds_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
-// the wave's nodes of class c, ranked class-major; T = how many
-template <int B>
-__device__ __forceinline__ u32 class_ranks(const u32* cls, u32 cnt, u32 c, u64 lt, u32* rank) {
-  u32 T = 0;
-#pragma unroll
-  for (int j = 0; j < B; ++j) {
-    const u64 m = __ballot((u32)j < cnt && cls[j] == c);
-    rank[j] = T + (u32)__builtin_popcountll(m & lt);
-    T += (u32)__builtin_popcountll(m);
-  }
-  return T;
-}
+// A node's degree, adjacency size class and class-major slot in one word:
+// deg | cls << 8 | slot << 16 (deg < 64, cls < 32, slot < 64*B <= 2^16)
+__device__ __forceinline__ u32 nd_deg(u32 x) { return x & 0xFFu; }
+__device__ __forceinline__ u32 nd_cls(u32 x) { return (x >> 8) & 0xFFu; }
+__device__ __forceinline__ u32 nd_slot(u32 x) { return x >> 16; }
 
 template <int B>
-__device__ __forceinline__ u32 classes_present(const u32* cls, u32 cnt) {
+__device__ __forceinline__ u32 classes_present(const u32* nd, u32 cnt) {
   u32 mine = 0;
 #pragma unroll
   for (int j = 0; j < B; ++j)
-    if ((u32)j < cnt) mine |= 1u << cls[j];
+    if ((u32)j < cnt) mine |= 1u << nd_cls(nd[j]);
   u32 all = 0;
   for (u32 c = 0; c < ha::NSIZES; ++c)
     if (__ballot((mine >> c) & 1u)) all |= 1u << c;
   return all;
 }
 
+// Slots the wave's nodes of class c class-major from `at` on (nodes in (j,
+// lane) order) and returns how many there are.
+template <int B>
+__device__ __forceinline__ u32 slot_class(u32* nd, u32 cnt, u32 c, u32 at, u64 lt) {
+  u32 T = 0;
+#pragma unroll
+  for (int j = 0; j < B; ++j) {
+    const bool mine = (u32)j < cnt && nd_cls(nd[j]) == c;
+    const u64 m = __ballot(mine);
+    if (mine) nd[j] |= (at + T + (u32)__builtin_popcountll(m & lt)) << 16;
+    T += (u32)__builtin_popcountll(m);
+  }
+  return T;
+}
+
+// LDS row of one lane: its own hamalloc_batch/hafree_batch pointer list.  The
+// stride B + 1 puts the 64 lanes' rows in different banks.
+template <int B>
+__device__ __forceinline__ void** lane_row(void** lds, u32 me) {
+  return lds + me * (B + 1);
+}
+
 template <int B>
 __device__ void graph_chunk_build(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr** tab, void** lds, u32 me, u64 lt,
                                   u32& nulls) {
   const u32 base = chunk * 64u * B;
-  u32 deg[B], cls[B];
+  u32 nd[B];
   void* hdr[B];
-  void* adj[B];
   u32 cnt = 0;  // this lane's nodes: a prefix of j
 #pragma unroll
   for (int j = 0; j < B; ++j) {
     const u32 v = base + (u32)j * 64u + me;
     cnt += v < n;
-    deg[j] = v < n ? poisson8(seed, job, v) : 0u;
-    cls[j] = ha::size_id_from_nbytes(adj_bytes(deg[j]));
-    hdr[j] = nullptr;
-    adj[j] = nullptr;
+    const u32 d = v < n ? poisson8(seed, job, v) : 0u;
+    nd[j] = d | (ha::size_id_from_nbytes(adj_bytes(d)) << 8);
   }
-  if (cnt) hamalloc_batch(sizeof(NodeHdr), (int)cnt, hdr);
-  for (u32 cs = classes_present<B>(cls, cnt); cs; cs &= cs - 1) {
+  // headers: the lane's cnt headers into its LDS row, then into registers
+  void** row = lane_row<B>(lds, me);
+  if (cnt) hamalloc_batch(sizeof(NodeHdr), (int)cnt, row);
+#pragma unroll
+  for (int j = 0; j < B; ++j) hdr[j] = (u32)j < cnt ? row[j] : nullptr;
+  wave_lds_sync();
+  // adjacency arrays, class-major: class c's blocks fill LDS slots
+  // [at, at + T) from ONE hamalloc_batch in which every lane takes an equal
+  // share
+  u32 at = 0;
+  for (u32 cs = classes_present<B>(nd, cnt); cs; cs &= cs - 1) {
     const u32 c = (u32)__builtin_ctz(cs);
-    u32 rank[B];
-    const u32 T = class_ranks<B>(cls, cnt, c, lt, rank);
+    const u32 T = slot_class<B>(nd, cnt, c, at, lt);
     const u32 q = (T + 63) / 64, lo = me * q;  // lane share: [lo, lo + k)
     const u32 k = lo >= T ? 0u : (T - lo < q ? T - lo : q);
-    if (k) hamalloc_batch(ha::ha_p.si[c].block_sz, (int)k, lds + lo);  // straight into LDS
-    wave_lds_sync();
-#pragma unroll
-    for (int j = 0; j < B; ++j)
-      if ((u32)j < cnt && cls[j] == c) adj[j] = lds[rank[j]];
-    wave_lds_sync();
+    if (k) hamalloc_batch(ha::ha_p.si[c].block_sz, (int)k, lds + at + lo);  // straight into LDS
+    at += T;
   }
+  wave_lds_sync();
 #pragma unroll
   for (int j = 0; j < B; ++j) {
     if ((u32)j >= cnt) continue;
     const u32 v = base + (u32)j * 64u + me;
     NodeHdr* h = (NodeHdr*)hdr[j];
-    u32* a = (u32*)adj[j];
+    u32* a = (u32*)lds[nd_slot(nd[j])];
     if (h && a) {
-      fill_node(h, a, seed, job, v, deg[j], n);
+      fill_node(h, a, seed, job, v, nd_deg(nd[j]), n);
     } else {
       nulls++;
       if (a) hafree(a);
@@ -581,16 +598,17 @@ __device__ void graph_chunk_build(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr**
     }
     tab[v] = h;
   }
+  wave_lds_sync();  // the LDS slots are read before the next item reuses them
 }
 
 template <int B>
 __device__ void graph_chunk_free(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr** tab, void** lds, u32 me, u64 lt,
                                  u32& bad) {
   const u32 base = chunk * 64u * B;
-  u32 cls[B];
+  u32 nd[B];
   void* hdr[B];
   void* adj[B];
-  u32 cnt = 0, deg[B];
+  u32 cnt = 0;
   // the lane's nodes are checked together: table entries, then headers, then
   // the adjacency pairs, each set of loads issued before any is used
 #pragma unroll
@@ -610,53 +628,67 @@ __device__ void graph_chunk_free(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr** 
     for (int t = 0; t < H; ++t) {
       const int j = j0 + t;
       const u32 v = base + (u32)j * 64u + me;
-      deg[j] = (u32)j < cnt ? poisson8(seed, job, v) : 0u;
-      cls[j] = ha::size_id_from_nbytes(adj_bytes(deg[j]));
+      const u32 d = (u32)j < cnt ? poisson8(seed, job, v) : 0u;
+      nd[j] = d | (ha::size_id_from_nbytes(adj_bytes(d)) << 8);
       adj[j] = nullptr;
       if (hdr[j]) {
         const u32* a = (const u32*)(((u64)hv[t].w << 32) | hv[t].z);
-        if (hv[t].x != v || hv[t].y != deg[j] || !a) bad++;  // a damaged node's adjacency is not trusted
+        if (hv[t].x != v || hv[t].y != d || !a) bad++;  // a damaged node's adjacency is not trusted
         else adj[j] = (void*)a;
       }
-      if (adj[j] && deg[j] > maxd) maxd = deg[j];
+      if (adj[j] && d > maxd) maxd = d;
     }
   }
-  u32 mism[B];
-#pragma unroll
-  for (int j = 0; j < B; ++j) mism[j] = 0;
-  for (u32 e = 0; e < maxd; e += 2) {
-    uint2 pv[B];
-#pragma unroll
-    for (int j = 0; j < B; ++j)
-      pv[j] = adj[j] && e < deg[j] ? *(const uint2*)((const u32*)adj[j] + e) : make_uint2(0u, 0u);
-#pragma unroll
-    for (int j = 0; j < B; ++j) {
-      if (!adj[j] || e >= deg[j]) continue;
-      const u32 v = base + (u32)j * 64u + me;
-      mism[j] += pv[j].x != gh(seed, job, v, e) % n;
-      if (e + 1 < deg[j]) mism[j] += pv[j].y != gh(seed, job, v, e + 1) % n;
-    }
-  }
+  // the headers are read: free them (the lane's list through its LDS row)
+  void** row = lane_row<B>(lds, me);
 #pragma unroll
   for (int j = 0; j < B; ++j)
-    if (mism[j]) {
-      bad++;
-      adj[j] = nullptr;
-    }
-  for (u32 cs = classes_present<B>(cls, cnt); cs; cs &= cs - 1) {
-    const u32 c = (u32)__builtin_ctz(cs);
-    u32 rank[B];
-    const u32 T = class_ranks<B>(cls, cnt, c, lt, rank);
+    if ((u32)j < cnt) row[j] = hdr[j];
+  if (cnt) hafree_batch(row, (int)cnt);
+  wave_lds_sync();
+  u32 mism = 0;  // bit j: node j's adjacency differs from its hash
+  for (u32 e = 0; e < maxd; e += 4) {  // two pairs per node in flight
+    uint2 pv[B], pw[B];
 #pragma unroll
-    for (int j = 0; j < B; ++j)
-      if ((u32)j < cnt && cls[j] == c) lds[rank[j]] = adj[j];
-    wave_lds_sync();
+    for (int j = 0; j < B; ++j) {
+      const u32 d = nd_deg(nd[j]);
+      const u32* a = (const u32*)adj[j] + e;
+      pv[j] = adj[j] && e < d ? *(const uint2*)a : make_uint2(0u, 0u);
+      pw[j] = adj[j] && e + 2 < d ? *(const uint2*)(a + 2) : make_uint2(0u, 0u);
+    }
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      const u32 d = nd_deg(nd[j]);
+      if (!adj[j] || e >= d) continue;
+      const u32 v = base + (u32)j * 64u + me;
+      bool x = pv[j].x != gh(seed, job, v, e) % n;
+      if (e + 1 < d) x |= pv[j].y != gh(seed, job, v, e + 1) % n;
+      if (e + 2 < d) x |= pw[j].x != gh(seed, job, v, e + 2) % n;
+      if (e + 3 < d) x |= pw[j].y != gh(seed, job, v, e + 3) % n;
+      if (x) mism |= 1u << j;
+    }
+  }
+  bad += (u32)__builtin_popcount(mism);
+  // adjacency arrays, class-major into LDS, then ONE hafree_batch per class
+  // in which every lane frees an equal share: each round is a contiguous run
+  u32 at = 0;
+  for (u32 cs = classes_present<B>(nd, cnt); cs; cs &= cs - 1) at += slot_class<B>(nd, cnt, (u32)__builtin_ctz(cs), at, lt);
+#pragma unroll
+  for (int j = 0; j < B; ++j)
+    if ((u32)j < cnt) lds[nd_slot(nd[j])] = (mism >> j) & 1u ? nullptr : adj[j];
+  wave_lds_sync();
+  at = 0;
+  for (u32 cs = classes_present<B>(nd, cnt); cs; cs &= cs - 1) {
+    const u32 c = (u32)__builtin_ctz(cs);
+    u32 T = 0;
+#pragma unroll
+    for (int j = 0; j < B; ++j) T += (u32)__builtin_popcountll(__ballot((u32)j < cnt && nd_cls(nd[j]) == c));
     const u32 q = (T + 63) / 64, lo = me * q;
     const u32 k = lo >= T ? 0u : (T - lo < q ? T - lo : q);
-    if (k) hafree_batch(lds + lo, (int)k);  // straight from LDS
-    wave_lds_sync();
+    if (k) hafree_batch(lds + at + lo, (int)k);  // straight from LDS
+    at += T;
   }
-  if (cnt) hafree_batch(hdr, (int)cnt);
+  wave_lds_sync();
 }
 
 // at least 3 waves per SIMD (<= 168 VGPRs): measured faster than the
@@ -665,7 +697,7 @@ template <int B>
 __global__ void __launch_bounds__(256, 3) graph_batch_k(const u64* seq, u32 nseq, const u32* coffs, const u64* offs,
                                                      const u32* nnodes, u32 seed, NodeHdr** table, u32* next,
                                                      u32* built, unsigned long long* stats) {
-  __shared__ void* lds_all[4][64 * B];
+  __shared__ void* lds_all[4][64 * (B + 1)];
   void** lds = lds_all[threadIdx.x >> 6];
   const u32 me = threadIdx.x & 63;
   const u64 lt = me ? (~0ull >> (64 - me)) : 0ull;
