@@ -249,9 +249,6 @@ HA_DEV bool steal_idle_head(u32 ihead, u32 size_id) {
   return false;
 }
 
-// hamalloc_batch issues the first probes of this many rounds together
-constexpr u32 BATCH_PIPE = 4;
-
 // A head search frees at most this many idle heads before it gives up.
 constexpr u32 MAX_STEALS = 4;
 
@@ -571,6 +568,31 @@ HA_DEV void* malloc_group(u32 sid, u64 g, u32 ihead, u32 me) {
   return p;
 }
 
+// The first probe of one round of a batched call: the round's ticket base
+// `old` fixes where each lane's slot starts (st), the bit word it falls in
+// (wreal, unwrapped modulo the slab), the union of the group's fields in that
+// word (ub) and the lane that issues the word's atomic (ib); myseg = the lanes
+// of this lane's run.
+struct BatchProbe {
+  u32 st, ib, wreal;
+  u64 ub, myseg;
+};
+HA_DEV BatchProbe batch_probe(const SizeInfo& s, u32 old, u32 me, u64 P) {
+  const u32 k = s.ncib, nchunks = s.nchunks, nwb = nchunks >> 6;
+  const u32 slot0 = old & (RUN - 1), split = RUN - slot0;
+  const u32 runA = (((old & ~(RUN - 1)) * THREAD_FREQ) * k) % nchunks;
+  const u32 runB = ((((old & ~(RUN - 1)) + RUN) * THREAD_FREQ) * k) % nchunks;
+  const bool inA = me < split;
+  const u64 amask = split >= 64 ? ~0ull : ((1ull << split) - 1);
+  BatchProbe b;
+  b.myseg = inA ? amask : ~amask;
+  b.st = inA ? runA + slot0 * k : runB - split * k;
+  u32 wb;
+  run_word_bits(P & b.myseg, me, b.st, k, wb, b.ub, b.ib);
+  b.wreal = wb >= nwb ? wb - nwb : wb;
+  return b;
+}
+
 // hamalloc_batch for one class group: lane l wants cnt (<= MAXB) blocks into
 // out[0..cnt).  One ticket for all R rounds (round j based where R back-to-back
 // calls would be), every round probed in the current head, ONE counter atomic
@@ -601,78 +623,59 @@ HA_DEV void malloc_batch_group(u32 sid, u64 g, u32 ihead, u32 me, u32 cnt, void*
   } else {
     u32 total = 0;
     char* sbase = ha_p.arena + (u64)hsb * ha_p.sb_sz;
-    const u32 nchunks = s.nchunks, nwb = nchunks >> 6, nwn = nchunks >> 4;
+    const u32 nchunks = s.nchunks, nwn = nchunks >> 4;
     u64* bw = ha_p.bits + (u64)hsb * ha_p.bit_words;
     u64* nw = ha_p.nibs + (u64)hsb * ha_p.nib_words;
-    // rounds in groups of BATCH_PIPE: the first probe of every round in the
-    // group is issued before any result is awaited; rounds with lanes left
-    // over then go on from the second probe, in round order
-    for (u32 j0 = 0; j0 < R; j0 += BATCH_PIPE) {
-      u64 Pj[BATCH_PIPE], oldw[BATCH_PIPE], myseg[BATCH_PIPE];
-      u32 st[BATCH_PIPE], ib[BATCH_PIPE], wreal[BATCH_PIPE];
-#pragma unroll
-      for (u32 t = 0; t < BATCH_PIPE; ++t) {
-        const u32 j = j0 + t;
-        Pj[t] = j < R ? ballot(cnt > j) : 0ull;
-        oldw[t] = 0;
-        st[t] = 0;
-        ib[t] = 64;
-        wreal[t] = 0;
-        myseg[t] = 0;
-        if (!Pj[t]) continue;
-        const u32 old = base + j * L;
-        const u32 slot0 = old & (RUN - 1), split = RUN - slot0;
-        const u32 runA = (((old & ~(RUN - 1)) * THREAD_FREQ) * k) % nchunks;
-        const u32 runB = ((((old & ~(RUN - 1)) + RUN) * THREAD_FREQ) * k) % nchunks;
-        const bool inA = me < split;
-        const u64 amask = split >= 64 ? ~0ull : ((1ull << split) - 1);
-        myseg[t] = inA ? amask : ~amask;
-        st[t] = inA ? runA + slot0 * k : runB - split * k;
-        u32 wb;
-        u64 ub;
-        run_word_bits(Pj[t] & myseg[t], me, st[t], k, wb, ub, ib[t]);
-        wreal[t] = wb >= nwb ? wb - nwb : wb;
-        if (((Pj[t] >> me) & 1) && ib[t] == me) oldw[t] = a_or(bw + wreal[t], ub);
+    // the first probe of every round is issued before any result is awaited;
+    // then, round by round, the lanes that failed give their new bits back and
+    // go on from the second probe.  ow is indexed by the wave-uniform round, so
+    // it stays in registers.
+    u64 ow[MAXB];
+#pragma nounroll
+    for (u32 j = 0; j < R; ++j) {
+      const u64 P = ballot(cnt > j);
+      const BatchProbe b = batch_probe(s, base + j * L, me, P);
+      u64 v = 0;
+      if (((P >> me) & 1) && b.ib == me) v = a_or(bw + b.wreal, b.ub);
+      ow[j] = v;
+    }
+#pragma nounroll
+    for (u32 j = 0; j < R; ++j) {
+      const u64 P = ballot(cnt > j);
+      const BatchProbe b = batch_probe(s, base + j * L, me, P);
+      const u64 oww = shfl64(ow[j], b.ib & 63);
+      u32 c = b.st + me * k;
+      c = c >= nchunks ? c - nchunks : c;
+      const u64 mym = ((1ull << k) - 1) << (c & 63);
+      const bool ok = ((P >> me) & 1) && !(oww & mym);
+      const u64 S = ballot(ok);
+      const u64 F = P & ~S;
+      if (F) {
+        u32 wf, iff;
+        u64 uf;
+        run_word_bits(F & b.myseg, me, b.st, k, wf, uf, iff);
+        const u64 clr = uf & ~oww;
+        if (((F >> me) & 1) && iff == me && clr) a_and(bw + b.wreal, ~clr);
       }
-#pragma unroll
-      for (u32 t = 0; t < BATCH_PIPE; ++t) {
-        const u32 j = j0 + t;
-        if (!Pj[t]) continue;
-        const u64 P = Pj[t];
-        const u64 ow = shfl64(oldw[t], ib[t] & 63);
-        u32 c = st[t] + me * k;
-        c = c >= nchunks ? c - nchunks : c;
-        const u64 mym = ((1ull << k) - 1) << (c & 63);
-        const bool ok = ((P >> me) & 1) && !(ow & mym);
-        const u64 S = ballot(ok);
-        const u64 F = P & ~S;
-        if (F) {
-          u32 wf, iff;
-          u64 uf;
-          run_word_bits(F & myseg[t], me, st[t], k, wf, uf, iff);
-          const u64 clr = uf & ~ow;
-          if (((F >> me) & 1) && iff == me && clr) a_and(bw + wreal[t], ~clr);
-        }
-        if (S) {
-          u32 wn, inn;
-          u64 un;
-          run_word_nibs(S & myseg[t], me, st[t], k, (u64)k, wn, un, inn);
-          const u32 wnreal = wn >= nwn ? wn - nwn : wn;
-          if (ok && inn == me) a_or(nw + wnreal, un);
-          if (ok) out[j] = sbase + chunk_mul(c, s.chunk_sz);
-        }
-        u64 Sall = S;
-        if (F) {
-          // resume at the second probe (itry 1) for the lanes left over
-          u32 itry = 1, my_ich = 0;
-          const u64 off1 = (u64)((1u * STEP_FREQ * k) % s.hash_step);
-          const u64 S2 = probe_run(s, hsb, limit, base + j * L, F, me, itry, my_ich, off1);
-          if ((S2 >> me) & 1) out[j] = sbase + chunk_mul(my_ich, s.chunk_sz);
-          Sall |= S2;
-        }
-        if (Sall != P) need = true;
-        total += (u32)__builtin_popcountll(Sall);
+      if (S) {
+        u32 wn, inn;
+        u64 un;
+        run_word_nibs(S & b.myseg, me, b.st, k, (u64)k, wn, un, inn);
+        const u32 wnreal = wn >= nwn ? wn - nwn : wn;
+        if (ok && inn == me) a_or(nw + wnreal, un);
+        if (ok) out[j] = sbase + chunk_mul(c, s.chunk_sz);
       }
+      u64 Sall = S;
+      if (F) {
+        // resume at the second probe (itry 1) for the lanes left over
+        u32 itry = 1, my_ich = 0;
+        const u64 off1 = (u64)((1u * STEP_FREQ * k) % s.hash_step);
+        const u64 S2 = probe_run(s, hsb, limit, base + j * L, F, me, itry, my_ich, off1);
+        if ((S2 >> me) & 1) out[j] = sbase + chunk_mul(my_ich, s.chunk_sz);
+        Sall |= S2;
+      }
+      if (Sall != P) need = true;
+      total += (u32)__builtin_popcountll(Sall);
     }
     if (total) {
       const u32 change = k * total;

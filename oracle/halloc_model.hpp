@@ -104,7 +104,7 @@ constexpr u32 MAXB = 16;            // most blocks one lane takes in one batched
 // such a head is filled up to (busy + nchunks) / 2 instead of busy
 constexpr u32 HEAD_EXT = 0x80000000u;
 constexpr u32 MAX_STEALS = 4;       // idle heads one head search may free before it gives up
-constexpr u32 BATCH_PIPE = 4;       // rounds of a batched call whose first probes go out together
+constexpr u32 BATCH_PIPE = MAXB;    // rounds of a batched call whose first probes go out together (all)
 
 // slab counter layout (64-bit)
 constexpr u32 CTR_CHUNK_SH = 8;
