@@ -83,4 +83,16 @@ HA_HD inline void run_word_nibs(u64 B, u32 me, u32 s, u32 k, u64 val, u32& word,
   issuer = sub ? lo + (u32)__builtin_ctzll(sub) : 64u;
 }
 
+// x mod n without a division, for x < 2^32 and 1 <= n, given m = mod_magic(n)
+// = floor(2^32 / n) (0 for n = 1): q = floor(x * m / 2^32) is floor(x / n) or
+// one less, since x * m / 2^32 lies in (x / n - 1, x / n]; one correction step
+// finishes
+HA_HD inline u32 mod_magic(u32 n) { return n < 2 ? 0u : (u32)(0x100000000ull / n); }
+HA_HD inline u32 mod_by(u32 x, u32 n, u32 m) {
+  const u32 q = (u32)(((u64)x * m) >> 32);
+  u32 r = x - q * n;
+  r = r >= n ? r - n : r;
+  return n < 2 ? 0u : r;
+}
+
 }  // namespace ha

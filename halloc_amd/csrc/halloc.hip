@@ -605,15 +605,17 @@ HA_DEV void malloc_batch_group(u32 sid, u64 g, u32 ihead, u32 me, u32 cnt, void*
   u32 R = 0;  // the group's largest cnt
   for (u32 j = 0; j < MAXB; ++j)
     if (ballot(cnt > j)) R = j + 1;
+  // the ticket and the head load go out together: no head search touches a
+  // ticket, so taking it before a missing head is replaced changes nothing
+  u32 base = 0;
+  if (me == lead) base = a_add32(hot(ha_p.ticket, ihead, sid), R * L);
   u32 hword = rfl(ld32(hot(ha_p.head, ihead, sid)));
+  base = rdl(base, lead);
   if (hword == SB_NONE) {  // no head yet: get one first
     u32 nh = 0;
     if (me == lead) nh = new_sb_for_size(sid, ihead, SB_NONE);
     hword = rdl(nh, lead);
   }
-  u32 base = 0;
-  if (me == lead) base = a_add32(hot(ha_p.ticket, ihead, sid), R * L);
-  base = rdl(base, lead);
   for (u32 j = 0; j < cnt; ++j) out[j] = nullptr;
   bool need = false;
   const u32 hsb = hword == SB_NONE ? SB_NONE : (hword & ~HEAD_EXT);

@@ -423,10 +423,11 @@ __device__ __forceinline__ u32 adj_bytes(u32 deg) { return deg ? 4u * deg : 16u;
 // The adjacency array is written and read as 8-byte pairs: blocks are 8-byte
 // aligned and their class size is a multiple of 8 that is at least 4*deg, so
 // the pair holding the last entry of an odd degree stays inside the block.
-__device__ __forceinline__ void fill_node(NodeHdr* h, u32* a, u32 seed, u32 job, u32 node, u32 deg, u32 n) {
+// Targets are hash mod n; m = mod_magic(n) replaces the division.
+__device__ __forceinline__ void fill_node(NodeHdr* h, u32* a, u32 seed, u32 job, u32 node, u32 deg, u32 n, u32 m) {
   for (u32 e = 0; e < deg; e += 2) {
-    const u32 v0 = gh(seed, job, node, e) % n;
-    if (e + 1 < deg) *(uint2*)(a + e) = make_uint2(v0, gh(seed, job, node, e + 1) % n);
+    const u32 v0 = ha::mod_by(gh(seed, job, node, e), n, m);
+    if (e + 1 < deg) *(uint2*)(a + e) = make_uint2(v0, ha::mod_by(gh(seed, job, node, e + 1), n, m));
     else a[e] = v0;
   }
   h->node = node;
@@ -435,11 +436,12 @@ __device__ __forceinline__ void fill_node(NodeHdr* h, u32* a, u32 seed, u32 job,
 }
 __device__ __forceinline__ u32 check_node(const NodeHdr* h, u32 seed, u32 job, u32 node, u32 n, u32 deg) {
   if (h->node != node || h->deg != deg || !h->adj) return 1;
+  const u32 m = ha::mod_magic(n);
   u32 bad = 0;
   for (u32 e = 0; e < deg; e += 2) {
     const uint2 v = *(const uint2*)(h->adj + e);
-    bad += v.x != gh(seed, job, node, e) % n;
-    if (e + 1 < deg) bad += v.y != gh(seed, job, node, e + 1) % n;
+    bad += v.x != ha::mod_by(gh(seed, job, node, e), n, m);
+    if (e + 1 < deg) bad += v.y != ha::mod_by(gh(seed, job, node, e + 1), n, m);
   }
   return bad ? 1 : 0;
 }
@@ -462,7 +464,7 @@ __global__ void graph_plain_build_k(const u64* offs, const u32* nnodes, u32 njob
   NodeHdr* h = (NodeHdr*)hamalloc(sizeof(NodeHdr));
   u32* a = (u32*)hamalloc(adj_bytes(deg));
   if (h && a) {
-    fill_node(h, a, seed, job, node, deg, n);
+    fill_node(h, a, seed, job, node, deg, n, ha::mod_magic(n));
   } else {
     atomicAdd(&stats[0], 1ull);
     if (a) hafree(a);
@@ -554,7 +556,7 @@ __device__ void graph_chunk_build(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr**
                                   u32& nulls) {
   const u32 base = chunk * 64u * B;
   u32 nd[B];
-  void* hdr[B];
+  u32 hdr[B];  // header offsets in the arena, in 16 B units (~0u: none)
   u32 cnt = 0;  // this lane's nodes: a prefix of j
 #pragma unroll
   for (int j = 0; j < B; ++j) {
@@ -567,7 +569,7 @@ __device__ void graph_chunk_build(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr**
   void** row = lane_row<B>(lds, me);
   if (cnt) hamalloc_batch(sizeof(NodeHdr), (int)cnt, row);
 #pragma unroll
-  for (int j = 0; j < B; ++j) hdr[j] = (u32)j < cnt ? row[j] : nullptr;
+  for (int j = 0; j < B; ++j) hdr[j] = (u32)j < cnt && row[j] ? (u32)(((char*)row[j] - ha::ha_p.arena) >> 4) : ~0u;
   wave_lds_sync();
   // adjacency arrays, class-major: class c's blocks fill LDS slots
   // [at, at + T) from ONE hamalloc_batch in which every lane takes an equal
@@ -582,14 +584,15 @@ __device__ void graph_chunk_build(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr**
     at += T;
   }
   wave_lds_sync();
+  const u32 m = ha::mod_magic(n);
 #pragma unroll
   for (int j = 0; j < B; ++j) {
     if ((u32)j >= cnt) continue;
     const u32 v = base + (u32)j * 64u + me;
-    NodeHdr* h = (NodeHdr*)hdr[j];
+    NodeHdr* h = hdr[j] == ~0u ? nullptr : (NodeHdr*)(ha::ha_p.arena + ((u64)hdr[j] << 4));
     u32* a = (u32*)lds[nd_slot(nd[j])];
     if (h && a) {
-      fill_node(h, a, seed, job, v, nd_deg(nd[j]), n);
+      fill_node(h, a, seed, job, v, nd_deg(nd[j]), n, m);
     } else {
       nulls++;
       if (a) hafree(a);
@@ -646,6 +649,7 @@ __device__ void graph_chunk_free(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr** 
     if ((u32)j < cnt) row[j] = hdr[j];
   if (cnt) hafree_batch(row, (int)cnt);
   wave_lds_sync();
+  const u32 m = ha::mod_magic(n);
   u32 mism = 0;  // bit j: node j's adjacency differs from its hash
   for (u32 e = 0; e < maxd; e += 4) {  // two pairs per node in flight
     uint2 pv[B], pw[B];
@@ -661,10 +665,10 @@ __device__ void graph_chunk_free(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr** 
       const u32 d = nd_deg(nd[j]);
       if (!adj[j] || e >= d) continue;
       const u32 v = base + (u32)j * 64u + me;
-      bool x = pv[j].x != gh(seed, job, v, e) % n;
-      if (e + 1 < d) x |= pv[j].y != gh(seed, job, v, e + 1) % n;
-      if (e + 2 < d) x |= pw[j].x != gh(seed, job, v, e + 2) % n;
-      if (e + 3 < d) x |= pw[j].y != gh(seed, job, v, e + 3) % n;
+      bool x = pv[j].x != ha::mod_by(gh(seed, job, v, e), n, m);
+      if (e + 1 < d) x |= pv[j].y != ha::mod_by(gh(seed, job, v, e + 1), n, m);
+      if (e + 2 < d) x |= pw[j].x != ha::mod_by(gh(seed, job, v, e + 2), n, m);
+      if (e + 3 < d) x |= pw[j].y != ha::mod_by(gh(seed, job, v, e + 3), n, m);
       if (x) mism |= 1u << j;
     }
   }
@@ -1334,6 +1338,9 @@ int ha_run_graphs(const ha_graph_args* a, const uint32_t* nnodes, ha_graph_resul
       (a->batch != 4 && a->batch != 8 && a->batch != 16) || a->waves < 1 || a->plain_chunk_nodes < 1 ||
       a->repeats < 1)
     return hfail(HA_E_INVALID, "bad graph arguments");
+  // the batched kernel keeps header offsets in 32 bits of 16 B units
+  if (a->run_batch && ha::host_params().arena_bytes > (64ull << 30))
+    return hfail(HA_E_INVALID, "the batched graph form supports arenas up to 64 GiB");
   const u32 njobs = (u32)a->njobs;
   std::vector<u64> offs(njobs + 1, 0);
   for (u32 j = 0; j < njobs; ++j) {

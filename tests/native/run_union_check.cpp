@@ -114,6 +114,21 @@ int main() {
       CHECK(ha::spread_bits(x, f) == want);
     }
   }
+  // mod_by against %: every n up to 2^17, then random n, with random x and
+  // the edge values around multiples of n and 2^32
+  for (u32 n = 1; n <= (1u << 17); ++n) {
+    const u32 m = ha::mod_magic(n);
+    const u32 xs[6] = {0u, n - 1, n, 0xFFFFFFFFu, 0xFFFFFFFFu - (0xFFFFFFFFu % n), (u32)rng()};
+    for (u32 x : xs) CHECK(ha::mod_by(x, n, m) == x % n);
+    for (int t = 0; t < 8; ++t) {
+      const u32 x = (u32)rng();
+      CHECK(ha::mod_by(x, n, m) == x % n);
+    }
+  }
+  for (int t = 0; t < 2000000; ++t) {
+    const u32 n = 1u + (u32)(rng() % 0xFFFFFFFFull), x = (u32)rng();
+    CHECK(ha::mod_by(x, n, ha::mod_magic(n)) == x % n);
+  }
   if (fails) {
     std::printf("%d failures\n", fails);
     return 1;
