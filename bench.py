@@ -79,23 +79,27 @@ def pct(a, q):
 
 
 def bench_batching(H, njobs=4096, memory=4 << 30, seed=0):
-    """Both forms under the library defaults and under the autotuned options;
-    the headline ratio puts the batched form with the tuned options against
-    the faster of the two plain runs."""
+    """Both forms under the library defaults, the batched form under its
+    autotuned options and the plain form under its own; the headline ratio
+    puts the tuned batched form against the fastest plain run."""
     import numpy as np
     from halloc_amd import autotune
     from halloc_amd.workloads import graph_sizes
     cfg = autotune.load()
+    pcfg = autotune.load_plain()
     nn = graph_sizes(np.random.default_rng(seed), njobs)
     H.Halloc.init(autotune.opts_of(cfg, memory))
     t = H.Halloc.run_graphs(nn, batch=cfg["batch"], waves=cfg["waves"], repeats=3)
+    H.Halloc.shutdown()
+    H.Halloc.init(autotune.opts_of(pcfg, memory))
+    p = H.Halloc.run_graphs(nn, run_batch=False, repeats=3)
     H.Halloc.shutdown()
     d0 = autotune.DEFAULT
     H.Halloc.init(H.HallocOpts(memory=memory))
     d = H.Halloc.run_graphs(nn, batch=d0["batch"], waves=d0["waves"], repeats=3)
     H.Halloc.shutdown()
     pairs = 2 * t["nodes"]  # a header and an adjacency array per node
-    plain_best = min(t["plain_ms"], d["plain_ms"])
+    plain_best = min(t["plain_ms"], p["plain_ms"], d["plain_ms"])
     return {
         "workload": f"{njobs} random-graph builds, n_i log-uniform on [1k, 64k], seed {seed}, "
                     f"{memory >> 30} GiB pool",
@@ -104,11 +108,13 @@ def bench_batching(H, njobs=4096, memory=4 << 30, seed=0):
         "batched_pairs_per_s": pairs / (t["batch_ms"] / 1e3),
         "plain_pairs_per_s": pairs / (plain_best / 1e3),
         "batched_vs_plain": plain_best / t["batch_ms"],
-        "tuned": cfg, "tuned_from_file": os.path.exists(autotune.TUNED),
+        "tuned": cfg, "tuned_plain": {k: pcfg[k] for k in autotune.OPT_KEYS},
+        "tuned_from_file": os.path.exists(autotune.TUNED),
         "tuned_opts": {"batched_ms": t["batch_ms"], "plain_ms": t["plain_ms"]},
+        "plain_tuned_opts": {"plain_ms": p["plain_ms"]},
         "default_opts": {"batched_ms": d["batch_ms"], "plain_ms": d["plain_ms"],
                          "batched_vs_plain": d["plain_ms"] / d["batch_ms"]},
-        "violations": t["nulls"] + t["errors"] + d["nulls"] + d["errors"],
+        "violations": t["nulls"] + t["errors"] + p["nulls"] + p["errors"] + d["nulls"] + d["errors"],
     }
 
 

@@ -48,7 +48,8 @@ HA_HD inline u64 spread_bits(u64 x, u32 f) {
 }
 
 // A run of 64 k-chunk slots starts at chunk s (slot of lane l = s + l*k,
-// unwrapped).  For lane `me` and the lanes in bitmap B, return the unwrapped
+// unwrapped); k is a power of two (1, 2, 4 or 8), so the divisions by k are
+// shifts.  For lane `me` and the lanes in bitmap B, return the unwrapped
 // index of the word holding me's slot, the union of B-lanes' fields inside that
 // word, and the lowest B-lane in that word (64 if none).
 //   bit array:    64 chunks per word, field = k one-bits
@@ -56,9 +57,10 @@ HA_HD inline u64 spread_bits(u64 x, u32 f) {
 HA_HD inline void run_word_bits(u64 B, u32 me, u32 s, u32 k, u32& word, u64& uni, u32& issuer) {
   u32 c = s + me * k;
   word = c >> 6;
+  const u32 ks = (u32)__builtin_ctz(k);
   int lo_u = (int)(word << 6) - (int)s;
-  u32 lo = lo_u <= 0 ? 0u : (u32)lo_u / k;
-  u32 hi = (u32)(lo_u + 63) / k;
+  u32 lo = lo_u <= 0 ? 0u : (u32)lo_u >> ks;
+  u32 hi = (u32)(lo_u + 63) >> ks;
   hi = hi > 63 ? 63 : hi;
   u32 n = hi - lo + 1;
   u64 m = n >= 64 ? ~0ull : ((1ull << n) - 1);
@@ -71,9 +73,10 @@ HA_HD inline void run_word_nibs(u64 B, u32 me, u32 s, u32 k, u64 val, u32& word,
                                               u32& issuer) {
   u32 c = s + me * k;
   word = c >> 4;
+  const u32 ks = (u32)__builtin_ctz(k);
   int lo_u = (int)(word << 4) - (int)s;
-  u32 lo = lo_u <= 0 ? 0u : (u32)lo_u / k;
-  u32 hi = (u32)(lo_u + 15) / k;
+  u32 lo = lo_u <= 0 ? 0u : (u32)lo_u >> ks;
+  u32 hi = (u32)(lo_u + 15) >> ks;
   hi = hi > 63 ? 63 : hi;
   u32 n = hi - lo + 1;
   u64 m = n >= 64 ? ~0ull : ((1ull << n) - 1);

@@ -61,7 +61,8 @@ constexpr u32 CTR_COUNT_SH = 32;
 struct SizeInfo {
   u32 nchunks, chunk_sz, chunk_id, ncib, sparse_thr, hash_step, roomy_thr, busy_thr;
   u32 block_sz, nblocks, tmod, chunk_bytes;
-  u32 pad[4];
+  u32 nchunks_m, hash_step_m;  // mod_magic of nchunks and hash_step
+  u32 pad[2];
 };
 
 struct Desc {
@@ -95,7 +96,8 @@ struct Params {
   u32 lunits, lnclasses;
   u32 nsbs, sb_sz, sb_sz_sh, nheads;
   u32 nsw, bit_words, nib_words, initialized;
-  u32 cached_arena, pad;  // cached_arena: hafree releases before publishing
+  u32 cached_arena;       // hafree releases before publishing
+  u32 nh_m, nh8_m, pad;   // mod_magic of nheads and of nheads / 8
   u64 cuda_memory;
   SizeInfo si[NSIZES];
 };

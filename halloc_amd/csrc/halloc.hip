@@ -62,8 +62,8 @@ HA_DEV u32 cu_bits() { return __builtin_amdgcn_s_getreg((4) | (8 << 6) | ((8 - 1
 // of 8), one of the XCD's nheads / 8 sets picked by the wave's CU.
 HA_DEV u32 head_set() {
   const u32 x = xcc_id(), n = ha_p.nheads;
-  if (n <= 8) return x % n;
-  return (x + 8u * (cu_bits() % (n / 8u))) % n;
+  if (n <= 8) return mod_by(x, n, ha_p.nh_m);
+  return x + 8u * mod_by(cu_bits(), n / 8u, ha_p.nh8_m);  // x < 8: already below n
 }
 
 // ------------------------------------------------ agent-scope atomic helpers
@@ -429,6 +429,12 @@ HA_DEV u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen) {
   }
 }
 
+// x mod nchunks for a probe position: the offset stays far below 2^32 in
+// practice (the division-free form); the 64-bit remainder covers the rest
+HA_DEV u32 mod_chunks(const SizeInfo& s, u64 x) {
+  return (x >> 32) ? (u32)(x % s.nchunks) : mod_by((u32)x, s.nchunks, s.nchunks_m);
+}
+
 // ------------------------------------------------------------- malloc path
 // Lane l of a class group takes ticket cv = old + l.  With runs of 64 blocks
 // (ichunk_init, src/halloc.cu:107-122, tmod = 64) the lanes fill the tail of
@@ -447,8 +453,8 @@ HA_DEV u64 probe_run(const SizeInfo& s, u32 hsb, u32 limit, u32 old, u64 P, u32 
                      u64 off0 = 0) {
   const u32 k = s.ncib, nchunks = s.nchunks, nwb = nchunks >> 6, nwn = nchunks >> 4;
   const u32 slot0 = old & (RUN - 1), split = RUN - slot0;
-  const u32 runA = (((old & ~(RUN - 1)) * THREAD_FREQ) * k) % nchunks;
-  const u32 runB = ((((old & ~(RUN - 1)) + RUN) * THREAD_FREQ) * k) % nchunks;
+  const u32 runA = mod_by(((old & ~(RUN - 1)) * THREAD_FREQ) * k, nchunks, s.nchunks_m);
+  const u32 runB = mod_by((((old & ~(RUN - 1)) + RUN) * THREAD_FREQ) * k, nchunks, s.nchunks_m);
   const bool inA = me < split;
   const u64 amask = split >= 64 ? ~0ull : ((1ull << split) - 1);
   const u64 myseg = inA ? amask : ~amask;
@@ -457,7 +463,7 @@ HA_DEV u64 probe_run(const SizeInfo& s, u32 hsb, u32 limit, u32 old, u64 P, u32 
   u64 reserved = 0;
   u64 off = off0;
   for (;;) {
-    const u32 pA = (u32)(((u64)runA + off) % nchunks), pB = (u32)(((u64)runB + off) % nchunks);
+    const u32 pA = mod_chunks(s, (u64)runA + off), pB = mod_chunks(s, (u64)runB + off);
     const u32 st = inA ? pA + slot0 * k : pB - split * k;
     u32 wb, ib;
     u64 ub;
@@ -496,7 +502,7 @@ HA_DEV u64 probe_run(const SizeInfo& s, u32 hsb, u32 limit, u32 old, u64 P, u32 
     if (itry % CHECK_NTRIES == CHECK_NTRIES - 1) {
       if (ctr_count(ld64(ctr_of(hsb))) >= limit) break;
     }
-    const u32 step = ((itry + 1) * STEP_FREQ * k) % s.hash_step;
+    const u32 step = mod_by((itry + 1) * STEP_FREQ * k, s.hash_step, s.hash_step_m);
     off = (u64)(itry + 1) * step;
     if (++itry % MAX_NTRIES >= MAX_NTRIES - 1) break;
   }
@@ -580,8 +586,8 @@ struct BatchProbe {
 HA_DEV BatchProbe batch_probe(const SizeInfo& s, u32 old, u32 me, u64 P) {
   const u32 k = s.ncib, nchunks = s.nchunks, nwb = nchunks >> 6;
   const u32 slot0 = old & (RUN - 1), split = RUN - slot0;
-  const u32 runA = (((old & ~(RUN - 1)) * THREAD_FREQ) * k) % nchunks;
-  const u32 runB = ((((old & ~(RUN - 1)) + RUN) * THREAD_FREQ) * k) % nchunks;
+  const u32 runA = mod_by(((old & ~(RUN - 1)) * THREAD_FREQ) * k, nchunks, s.nchunks_m);
+  const u32 runB = mod_by((((old & ~(RUN - 1)) + RUN) * THREAD_FREQ) * k, nchunks, s.nchunks_m);
   const bool inA = me < split;
   const u64 amask = split >= 64 ? ~0ull : ((1ull << split) - 1);
   BatchProbe b;
@@ -671,7 +677,7 @@ HA_DEV void malloc_batch_group(u32 sid, u64 g, u32 ihead, u32 me, u32 cnt, void*
       if (F) {
         // resume at the second probe (itry 1) for the lanes left over
         u32 itry = 1, my_ich = 0;
-        const u64 off1 = (u64)((1u * STEP_FREQ * k) % s.hash_step);
+        const u64 off1 = (u64)mod_by(STEP_FREQ * k, s.hash_step, s.hash_step_m);
         const u64 S2 = probe_run(s, hsb, limit, base + j * L, F, me, itry, my_ich, off1);
         if ((S2 >> me) & 1) out[j] = sbase + chunk_mul(my_ich, s.chunk_sz);
         Sall |= S2;
@@ -1295,6 +1301,8 @@ static bool build_size_infos(u32 sb_sz, const ha_opts_c& o, SizeInfo* out) {
     u32 p = max_prime_below(bound, s.nblocks);
     if (!p || !s.nblocks) return false;
     s.hash_step = s.ncib * p;
+    s.nchunks_m = mod_magic(s.nchunks);
+    s.hash_step_m = mod_magic(s.hash_step);
     s.roomy_thr = (u32)(o.roomy_fraction * s.nchunks);
     s.busy_thr = (u32)(o.busy_fraction * s.nchunks);
     s.sparse_thr = (u32)(o.sparse_fraction * s.nchunks);
@@ -1367,6 +1375,8 @@ static int init_locked(const ha_opts_c& o) {
   P.nsbs = (u32)(halloc_memory / P.sb_sz);
   if (P.nsbs == 0 || P.nsbs > MAX_NSBS) return fail(HA_E_INVALID, "memory gives no slab or too many slabs");
   P.nheads = (u32)o.nheads;
+  P.nh_m = mod_magic(P.nheads);
+  P.nh8_m = mod_magic(P.nheads / 8 ? P.nheads / 8 : 1);
   P.cached_arena = (u32)o.cached_arena;
   P.nsw = (P.nsbs + 63) / 64;
   P.bit_words = P.sb_sz / BLOCK_STEP / 64;

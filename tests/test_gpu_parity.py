@@ -314,6 +314,11 @@ def test_autotune_small_space():
     # one sweep tries every value that differs from the configuration it has reached
     assert 1 + 3 <= len(hist) <= 1 + sum(len(v) for v in space.values())
     assert best["batch"] in (4, 8, 16)
+    # the plain form is tuned on its own, over allocator options only
+    pspace = {"nheads": [1, 8], "hash_step_frac": [0.0, 0.05]}
+    pbest, pms, phist = autotune.tune(nn, memory=1 << 30, sweeps=1, log=say, space=pspace, form="plain")
+    assert np.isfinite(pms) and pms <= phist[0][1]
+    assert 1 + 2 <= len(phist) <= 1 + sum(len(v) for v in pspace.values())
 
 
 def _large_block(n):
