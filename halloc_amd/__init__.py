@@ -229,7 +229,9 @@ class Halloc:
 
     @staticmethod
     def replay_batch(ops: np.ndarray):
-        """ops: (nsteps, lanes, 4) uint32 [kind, arg, count, 0]; returns (recs (nsteps, lanes, 16, 4), xcc)."""
+        """ops: (nsteps, lanes, 4) uint32 [kind, arg, count, 0] (kind 1 hamalloc_batch of count blocks of arg
+        bytes; 2 hafree_batch of op arg's blocks; 3 the same, keeping the pointers for a later free);
+        returns (recs (nsteps, lanes, 16, 4), xcc)."""
         ops = np.ascontiguousarray(ops, dtype=np.uint32)
         nsteps, lanes = ops.shape[0], ops.shape[1]
         recs = np.zeros((nsteps, lanes, 16, 4), dtype=np.uint32)

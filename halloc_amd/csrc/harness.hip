@@ -114,7 +114,9 @@ __global__ void __launch_bounds__(64) replay_k(const u32* ops, int nsteps, int l
 
 // ------------------------------------------------------------ batch replay
 // One wavefront: per step the kind-1 lanes call hamalloc_batch together, then
-// the kind-2 lanes call hafree_batch on an earlier op's blocks.
+// the kind-2 and kind-3 lanes call hafree_batch on an earlier op's blocks;
+// kind 2 forgets the pointers, kind 3 keeps them, so that a later op can free
+// the same blocks again.
 __global__ void __launch_bounds__(64) replay_batch_k(const u32* ops, int nsteps, int lanes, u32* recs, u32* xcc,
                                                      void** ptrs) {
   const u32 l = threadIdx.x;
@@ -145,7 +147,7 @@ __global__ void __launch_bounds__(64) replay_batch_k(const u32* ops, int nsteps,
       }
     }
     __syncthreads();
-    if (kind == 2) {
+    if (kind == 2 || kind == 3) {
       void** fp = ptrs + (size_t)arg * ha::MAXB;
       const u32 c = ops[4 * (size_t)arg + 2] > ha::MAXB ? ha::MAXB : ops[4 * (size_t)arg + 2];
       for (u32 j = 0; j < c; ++j) {
@@ -167,7 +169,8 @@ __global__ void __launch_bounds__(64) replay_batch_k(const u32* ops, int nsteps,
         }
       }
       hafree_batch(fp, (int)c);
-      for (u32 j = 0; j < c; ++j) fp[j] = nullptr;
+      if (kind == 2)
+        for (u32 j = 0; j < c; ++j) fp[j] = nullptr;
     }
     __syncthreads();
   }
@@ -1312,8 +1315,8 @@ int ha_replay_batch(const uint32_t* ops, int nsteps, int lanes, uint32_t* recs, 
   const size_t nops = (size_t)nsteps * lanes;
   for (size_t i = 0; i < nops; ++i) {
     const uint32_t kind = ops[4 * i], arg = ops[4 * i + 1];
-    if (kind > 2) return hfail(HA_E_INVALID, "bad op kind");
-    if (kind == 2 && (arg >= nops || arg / lanes >= i / lanes || ops[4 * (size_t)arg] != 1))
+    if (kind > 3) return hfail(HA_E_INVALID, "bad op kind");
+    if ((kind == 2 || kind == 3) && (arg >= nops || arg / lanes >= i / lanes || ops[4 * (size_t)arg] != 1))
       return hfail(HA_E_INVALID, "a batch free must name a batch malloc of an earlier step");
   }
   DBuf dops, drecs, dxcc, dptrs;

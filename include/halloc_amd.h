@@ -86,9 +86,10 @@ int ha_replay(const uint32_t *ops, int nsteps, int lanes, const uint32_t *iheads
               uint32_t *xcc);
 
 /* Batched replay on ONE wavefront (include/halloc_batch.h): ops hold 4 u32
-   per (step, lane): kind (0 none, 1 hamalloc_batch, 2 hafree_batch), arg
-   (bytes, or the index of an earlier kind-1 op whose blocks are freed), count
-   (kind 1), 0.  recs hold 4 u32 per (op, j < 16), as for ha_replay.  xcc[step]
+   per (step, lane): kind (0 none, 1 hamalloc_batch, 2 hafree_batch, 3
+   hafree_batch that keeps the op's pointers, so that a later op frees the same
+   blocks again), arg (bytes, or the index of an earlier kind-1 op whose blocks
+   are freed), count (kind 1), 0.  recs hold 4 u32 per (op, j < 16), as for ha_replay.  xcc[step]
    receives the wave's XCD id. */
 int ha_replay_batch(const uint32_t *ops, int nsteps, int lanes, uint32_t *recs, uint32_t *xcc);
 

@@ -135,7 +135,8 @@ class ModelHeap:
         return recs, ptrs
 
     def replay_batch(self, ops: np.ndarray, iheads: np.ndarray | None = None):
-        """ops: (nsteps, lanes, 4) uint32 [kind, arg, count, 0]; returns recs (nsteps, lanes, MAXB, 4)."""
+        """ops: (nsteps, lanes, 4) uint32 [kind, arg, count, 0] (kind 1 hamalloc_batch, 2 hafree_batch of
+        op arg's blocks, 3 the same keeping the pointers); returns recs (nsteps, lanes, MAXB, 4)."""
         ops = np.ascontiguousarray(ops, dtype=np.uint32)
         nsteps, lanes = ops.shape[0], ops.shape[1]
         recs = np.zeros((nsteps, lanes, MAXB, 4), dtype=np.uint32)

@@ -155,8 +155,9 @@ int hm_free_threads(void* hv, int nthreads, const uint64_t* ptrs, int n) {
 }
 
 // Batched replay: ops are 4 u32 per (step, lane): kind (0 none, 1 hamalloc_batch,
-// 2 hafree_batch), arg (bytes, or the index of an earlier kind-1 op whose
-// blocks are freed), count (blocks, kind 1), pad.  recs: 4 u32 per (op, j)
+// 2 hafree_batch, 3 hafree_batch that keeps the op's pointers so a later op
+// frees the same blocks again), arg (bytes, or the index of an earlier kind-1
+// op whose blocks are freed), count (blocks, kind 1), pad.  recs: 4 u32 per (op, j)
 // for j < MAXB; ptrs: arena offset + 1 per (op, j).
 int hm_replay_batch(void* hv, const uint32_t* ops, const uint32_t* iheads, int nsteps, int lanes,
                     uint32_t* recs, uint64_t* ptrs) {
@@ -166,12 +167,13 @@ int hm_replay_batch(void* hv, const uint32_t* ops, const uint32_t* iheads, int n
     const uint32_t* row = ops + (size_t)st * lanes * 4;
     u64 am = 0;
     u32 nbytes[WAVE] = {0}, cnt[WAVE] = {0}, fref[WAVE] = {0};
-    u64 fm = 0;
+    u64 fm = 0, fz = 0;  // lanes that free; of them, those that forget the pointers
     for (int l = 0; l < lanes; ++l) {
       if (row[4 * l] == 1) { am |= 1ull << l; nbytes[l] = row[4 * l + 1]; cnt[l] = row[4 * l + 2]; }
-      else if (row[4 * l] == 2) {
+      else if (row[4 * l] == 2 || row[4 * l] == 3) {
         if ((size_t)row[4 * l + 1] >= (size_t)st * lanes) return -2;
         fm |= 1ull << l; fref[l] = row[4 * l + 1];
+        if (row[4 * l] == 2) fz |= 1ull << l;
       }
     }
     if (am) {
@@ -203,7 +205,7 @@ int hm_replay_batch(void* hv, const uint32_t* ops, const uint32_t* iheads, int n
         h->wave_free(m, fp, frec);
         for (int l = 0; l < lanes; ++l) if (m >> l & 1) {
           size_t i = (size_t)st * lanes + l;
-          ptrs[(size_t)fref[l] * MAXB + j] = 0;
+          if (fz >> l & 1) ptrs[(size_t)fref[l] * MAXB + j] = 0;
           std::memcpy(recs + 4 * (i * MAXB + j), &frec[l], 16);
         }
       }

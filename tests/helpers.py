@@ -91,6 +91,27 @@ def random_batch_ops(rng, nsteps, lanes, sizes, max_count=16, p_free=0.4):
     return ops
 
 
+def wave_refree_ops(lanes, sizes):
+    """Batched replay program, per size: the wave allocates 16 blocks per lane
+    and ragged counts; frees the 16-block op keeping its pointers (kind 3);
+    frees it again (kind 2) while its slab still holds the ragged op; frees
+    the ragged op; then a partial wave allocates and frees.  Every free frees
+    a lane's own blocks, so each round is one slab's runs."""
+    steps = []
+    for size in sizes:
+        base = len(steps)
+        op = lambda st, l: (base + st) * lanes + l
+        prog = [[(1, size, 16, 0)] * lanes,
+                [(1, size, 1 + l % 7, 0) for l in range(lanes)],
+                [(3, op(0, l), 0, 0) for l in range(lanes)],
+                [(2, op(0, l), 0, 0) for l in range(lanes)],
+                [(2, op(1, l), 0, 0) for l in range(lanes)],
+                [(1, size, 5, 0) if l % 3 == 0 else (0, 0, 0, 0) for l in range(lanes)],
+                [(2, op(5, l), 0, 0) if l % 3 == 0 else (0, 0, 0, 0) for l in range(lanes)]]
+        steps.extend(prog)
+    return np.array(steps, dtype=np.uint32)
+
+
 def _hash3(a, b, c):
     """The drivers' 32-bit mixing hash (halloc_amd/csrc/harness.hip hash3), on uint32 arrays."""
     u = np.uint32

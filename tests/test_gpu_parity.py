@@ -12,7 +12,7 @@ import pytest
 import halloc_amd as H
 from oracle import model as M
 from tests.helpers import (BLOCK_SIZES, REC_NULL, assert_disjoint, live_intervals, pressure_program,
-                           graph_sizes, random_batch_ops, random_ops, sweep_sizes)
+                           graph_sizes, random_batch_ops, random_ops, sweep_sizes, wave_refree_ops)
 
 pytestmark = pytest.mark.gpu
 
@@ -285,6 +285,26 @@ def test_replay_batch(halloc, nheads):
     recs_m = heap.replay_batch(ops, xcc % nheads)
     assert (recs_g == recs_m).all()
     assert_same_state(heap, opts)
+
+
+@pytest.mark.parametrize("nheads", [1, 8])
+def test_replay_batch_runs_and_refree(halloc, nheads):
+    # every round of these frees is one slab's runs (the batched run path);
+    # the second free of the same blocks finds every length 0 and frees
+    # nothing; bit-exact against the model for block lengths of 1, 2, 4 and
+    # 8 chunks
+    say(f"replay batch runs nheads={nheads}")
+    opts = halloc(H.HallocOpts(memory=256 << 20, sb_sz_sh=21, nheads=nheads))
+    ops = wave_refree_ops(64, (16, 48, 96, 192, 24, 700))
+    recs_g, xcc = H.Halloc.replay_batch(ops)
+    heap = M.ModelHeap(model_opts(opts))
+    recs_m = heap.replay_batch(ops, xcc % nheads)
+    assert (recs_g == recs_m).all()
+    # the double frees freed nothing: the heap ends empty and consistent
+    assert_same_state(heap, opts)
+    g = H.Halloc.geometry()
+    d = H.Halloc.dump()
+    assert ((d[:g["nsbs"]] >> np.uint64(32)) == 0).all()
 
 
 @pytest.mark.parametrize("batch", [4, 16])

@@ -10,7 +10,7 @@ import pytest
 
 from oracle import model as M
 from tests.helpers import (BLOCK_SIZES, assert_disjoint, live_intervals, pressure_program, random_batch_ops,
-                           random_ops)
+                           random_ops, wave_refree_ops)
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -353,6 +353,33 @@ def test_batch_equals_back_to_back_calls():
         assert (brecs[0, :, :R, :3].transpose(1, 0, 2) == precs[:, :, :3]).all(), (size, R)
         # the batch touched the ticket once, the counter once: same end state
         assert M.check_dump(opts, a.dump())[0] == 0
+
+
+def test_model_refree_frees_nothing():
+    # the wave/refree program: the first free of each block records its
+    # length, the second records nothing, and the end state equals that of
+    # the same program without the second free
+    lanes, sizes = 64, (16, 48, 96, 192)
+    ops = wave_refree_ops(lanes, sizes)
+    opts = M.make_opts(memory=128 << 20, sb_sz_sh=21, nheads=4)
+    a = M.ModelHeap(opts)
+    recs = a.replay_batch(ops)
+    skip = ops.copy()
+    for i in range(len(sizes)):
+        skip[7 * i + 3] = 0  # drop the second free
+    b = M.ModelHeap(opts)
+    b.replay_batch(skip)
+    assert (a.dump() == b.dump()).all()
+    assert M.check_dump(opts, a.dump())[0] == 0
+    # blocks of 16, 48, 96 and 192 B span 1, 2, 4 and 8 chunks (16 B chunks
+    # for 2*unit blocks, 24 B chunks for 3*unit blocks)
+    for i, (size, k) in enumerate(zip(sizes, (1, 2, 4, 8))):
+        first, second = recs[7 * i + 2], recs[7 * i + 3]
+        assert (first[:, :, 0] == k).all()
+        assert (second[:, :, 0] == 0).all() and (second[:, :, 1] == M.REC_NULL).all()
+    # the heap ends empty
+    nsbs = a.nsbs
+    assert ((a.dump()[:nsbs] >> np.uint64(32)) == 0).all()
 
 
 @pytest.mark.parametrize("nheads", [1, 8])
