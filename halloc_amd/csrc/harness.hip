@@ -16,6 +16,7 @@
 #include "../../include/halloc.h"
 #include "../../include/halloc_amd.h"
 #include "ha_internal.h"
+#include "ha_workload.h"
 
 __device__ void* ha_malloc_on_head(size_t nbytes, unsigned ihead);
 __device__ int hamalloc_batch(size_t nbytes, int cnt, void** out);
@@ -410,16 +411,15 @@ struct NodeHdr {
 __device__ __forceinline__ u32 gh(u32 seed, u32 job, u32 node, u32 e) {
   return hash3(seed ^ (job * 0x9E3779B9u), node, e);
 }
+// Poisson(8) by inversion (the thresholds: ha_workload.h): the degree is the
+// number of thresholds at or below a 24-bit uniform from the hash.
+// Branch-free: every lane does the same compares, where a product-of-uniforms
+// loop would run as many hash rounds as the wave's largest degree.
 __device__ __forceinline__ u32 poisson8(u32 seed, u32 job, u32 node) {
-  // Knuth's product-of-uniforms method, lambda = 8
-  const float L = 3.3546262790251185e-4f;  // exp(-8)
-  float p = 1.0f;
+  const u32 u = gh(seed, job, node, 1000u) >> 8;
   u32 k = 0;
-  for (;;) {
-    p *= ((float)(gh(seed, job, node, 1000u + k) >> 8) + 0.5f) * (1.0f / 16777216.0f);
-    if (p <= L || k >= 63) break;
-    ++k;
-  }
+#pragma unroll
+  for (u32 i = 0; i < ha::POISSON8_N; ++i) k += u >= ha::kPoisson8.t[i];
   return k;
 }
 __device__ __forceinline__ u32 adj_bytes(u32 deg) { return deg ? 4u * deg : 16u; }
@@ -519,6 +519,9 @@ __device__ __forceinline__ void wave_lds_sync() { __builtin_amdgcn_fence(__ATOMI
 __device__ __forceinline__ u32 nd_deg(u32 x) { return x & 0xFFu; }
 __device__ __forceinline__ u32 nd_cls(u32 x) { return (x >> 8) & 0xFFu; }
 __device__ __forceinline__ u32 nd_slot(u32 x) { return x >> 16; }
+// in the free path the slot bits hold the owner lane (6 bits) and list round (5 bits)
+__device__ __forceinline__ u32 nd_owner(u32 x) { return (x >> 21) & 63u; }
+__device__ __forceinline__ u32 nd_round(u32 x) { return (x >> 16) & 31u; }
 
 template <int B>
 __device__ __forceinline__ u32 classes_present(const u32* nd, u32 cnt) {
@@ -547,11 +550,16 @@ __device__ __forceinline__ u32 slot_class(u32* nd, u32 cnt, u32 c, u32 at, u64 l
   return T;
 }
 
-// LDS row of one lane: its own hamalloc_batch/hafree_batch pointer list.  The
-// stride B + 1 puts the 64 lanes' rows in different banks.
+// LDS row of one lane: its own hamalloc_batch/hafree_batch pointer list of up
+// to max(B, 16) entries.  The odd stride puts the 64 lanes' rows in different
+// banks.
+template <int B>
+constexpr u32 row_stride() {
+  return B + 1 > 17 ? B + 1 : 17;
+}
 template <int B>
 __device__ __forceinline__ void** lane_row(void** lds, u32 me) {
-  return lds + me * (B + 1);
+  return lds + me * row_stride<B>();
 }
 
 template <int B>
@@ -676,26 +684,49 @@ __device__ void graph_chunk_free(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr** 
     }
   }
   bad += (u32)__builtin_popcount(mism);
-  // adjacency arrays, class-major into LDS, then ONE hafree_batch per class
-  // in which every lane frees an equal share: each round is a contiguous run
-  u32 at = 0;
-  for (u32 cs = classes_present<B>(nd, cnt); cs; cs &= cs - 1) at += slot_class<B>(nd, cnt, (u32)__builtin_ctz(cs), at, lt);
-#pragma unroll
-  for (int j = 0; j < B; ++j)
-    if ((u32)j < cnt) lds[nd_slot(nd[j])] = (mism >> j) & 1u ? nullptr : adj[j];
-  wave_lds_sync();
-  at = 0;
+  // adjacency arrays: class c's T blocks are shared out as in their
+  // allocation (lane l frees ranks [l*q, l*q + q), q = ceil(T/64), rank
+  // class-major), and each lane lists its shares of every class one after
+  // another: round off_c + i of the lists is round i of class c, one slab's
+  // contiguous runs.  hafree_batch takes the lists 16 rounds at a time
+  // through each lane's LDS row, so all classes go in one or two calls.  A
+  // node's owner lane and list round replace its slot bits.
+  u32 nr = 0;  // rounds of the lists
   for (u32 cs = classes_present<B>(nd, cnt); cs; cs &= cs - 1) {
     const u32 c = (u32)__builtin_ctz(cs);
     u32 T = 0;
 #pragma unroll
     for (int j = 0; j < B; ++j) T += (u32)__builtin_popcountll(__ballot((u32)j < cnt && nd_cls(nd[j]) == c));
-    const u32 q = (T + 63) / 64, lo = me * q;
-    const u32 k = lo >= T ? 0u : (T - lo < q ? T - lo : q);
-    if (k) hafree_batch(lds + at + lo, (int)k);  // straight from LDS
-    at += T;
+    const u32 q = (T + 63) / 64, qm = ha::mod_magic(q);
+    u32 r = 0;
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      const bool mine = (u32)j < cnt && nd_cls(nd[j]) == c;
+      const u64 mk = __ballot(mine);
+      if (mine) {
+        const u32 rank = r + (u32)__builtin_popcountll(mk & lt);
+        u32 own = q == 1 ? rank : (u32)(((u64)rank * qm) >> 32);  // rank / q, or one less
+        own += rank - own * q >= q;
+        nd[j] |= (own << 5 | (nr + rank - own * q)) << 16;
+      }
+      r += (u32)__builtin_popcountll(mk);
+    }
+    nr += q;
   }
-  wave_lds_sync();
+  for (u32 r0 = 0; r0 < nr; r0 += ha::MAXB) {
+    const u32 nrr = nr - r0 < ha::MAXB ? nr - r0 : ha::MAXB;
+    for (u32 i = 0; i < nrr; ++i) row[i] = nullptr;
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      const u32 rd = nd_round(nd[j]);
+      if ((u32)j < cnt && rd >= r0 && rd < r0 + nrr)
+        lds[nd_owner(nd[j]) * row_stride<B>() + (rd - r0)] = (mism >> j) & 1u ? nullptr : adj[j];
+    }
+    wave_lds_sync();
+    hafree_batch(row, (int)nrr);  // straight from LDS
+    wave_lds_sync();
+  }
 }
 
 // at least 3 waves per SIMD (<= 168 VGPRs): measured faster than the
@@ -704,7 +735,7 @@ template <int B>
 __global__ void __launch_bounds__(256, 3) graph_batch_k(const u64* seq, u32 nseq, const u32* coffs, const u64* offs,
                                                      const u32* nnodes, u32 seed, NodeHdr** table, u32* next,
                                                      u32* built, unsigned long long* stats) {
-  __shared__ void* lds_all[4][64 * (B + 1)];
+  __shared__ void* lds_all[4][64 * row_stride<B>()];
   void** lds = lds_all[threadIdx.x >> 6];
   const u32 me = threadIdx.x & 63;
   const u64 lt = me ? (~0ull >> (64 - me)) : 0ull;
