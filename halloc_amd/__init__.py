@@ -84,7 +84,7 @@ class _GraphArgs(ctypes.Structure):
     _fields_ = [("njobs", ctypes.c_int32), ("bs", ctypes.c_int32), ("batch", ctypes.c_int32),
                 ("waves", ctypes.c_int32), ("plain_chunk_nodes", ctypes.c_int32), ("repeats", ctypes.c_int32),
                 ("run_plain", ctypes.c_int32), ("run_batch", ctypes.c_int32), ("seed", ctypes.c_uint32),
-                ("pad", ctypes.c_uint32)]
+                ("cross", ctypes.c_uint32)]
 
 
 class _GraphResult(ctypes.Structure):
@@ -320,11 +320,13 @@ class Halloc:
 
     @staticmethod
     def run_graphs(nnodes, batch=8, waves=1024, bs=256, plain_chunk_nodes=1 << 23, repeats=1,
-                   run_plain=True, run_batch=True, seed=0):
-        """Random-graph builds, plain per-call form and batched persistent form."""
+                   run_plain=True, run_batch=True, seed=0, cross=False):
+        """Random-graph builds, plain per-call form and batched persistent form.
+        cross: instead of timing, build with each form and check and free with
+        the other (errors counts nodes that fail the other form's check)."""
         nn = np.ascontiguousarray(nnodes, dtype=np.uint32)
         a = _GraphArgs(len(nn), bs, batch, waves, plain_chunk_nodes, repeats, int(run_plain), int(run_batch),
-                       seed, 0)
+                       seed, int(cross))
         r = _GraphResult()
         _check(lib().ha_run_graphs(ctypes.byref(a), nn.ctypes.data, ctypes.byref(r)))
         return {"nodes": int(r.nodes), "nulls": int(r.nulls), "errors": int(r.errors),

@@ -28,6 +28,8 @@ int fail(int code, const std::string& msg);
 }
 
 using ha::u32;
+using u16 = uint16_t;
+using u8 = uint8_t;
 using ha::u64;
 
 namespace {
@@ -535,19 +537,74 @@ __device__ __forceinline__ u32 classes_present(const u32* nd, u32 cnt) {
   return all;
 }
 
+// A node in the rank table: j (4 bits), its lane (6 bits), its degree (6 bits)
+__device__ __forceinline__ u16 node_rec(u32 j, u32 lane, u32 deg) { return (u16)(j | lane << 4 | deg << 10); }
+static_assert(ha::POISSON8_N < 64, "degrees must fit the node record");
+
 // Slots the wave's nodes of class c class-major from `at` on (nodes in (j,
-// lane) order) and returns how many there are.
+// lane) order), records each slot's node in the rank table and returns how
+// many there are.
 template <int B>
-__device__ __forceinline__ u32 slot_class(u32* nd, u32 cnt, u32 c, u32 at, u64 lt) {
+__device__ __forceinline__ u32 slot_class(u32* nd, u32 cnt, u32 c, u32 at, u64 lt, u16* rk, u32 me) {
   u32 T = 0;
 #pragma unroll
   for (int j = 0; j < B; ++j) {
     const bool mine = (u32)j < cnt && nd_cls(nd[j]) == c;
     const u64 m = __ballot(mine);
-    if (mine) nd[j] |= (at + T + (u32)__builtin_popcountll(m & lt)) << 16;
+    if (mine) {
+      const u32 slot = at + T + (u32)__builtin_popcountll(m & lt);
+      nd[j] |= slot << 16;
+      rk[slot] = node_rec((u32)j, me, nd_deg(nd[j]));
+    }
     T += (u32)__builtin_popcountll(m);
   }
   return T;
+}
+
+// The adjacency contents of a chunk, class-major.  Class c's T blocks come
+// from one batched allocation in which lane l took ranks [l*q, l*q + q),
+// round i of the allocation holding rank l*q + i for every lane: one slab's
+// contiguous run.  The wave goes through the runs 8 bytes (two entries) per
+// lane at a time, so that consecutive lanes touch consecutive pieces of the
+// run instead of 64 scattered blocks.  slots[at + r] is rank r's block (NULL:
+// skip), rk[at + r] its node.  WRITE fills the blocks (entries past the
+// degree are 0); otherwise each piece is checked and flags[j*64 + lane] set
+// for a node whose contents differ.
+template <int B, bool WRITE>
+__device__ void adjacency_runs(const u32* nd, u32 cnt, u32 job, u32 base, u32 n, u32 seed, void* const* slots,
+                               const u16* rk, u8* flags, u32 me) {
+  const u32 m = ha::mod_magic(n);
+  u32 at = 0;
+  for (u32 cs = classes_present<B>(nd, cnt); cs; cs &= cs - 1) {
+    const u32 c = (u32)__builtin_ctz(cs);
+    u32 T = 0;
+#pragma unroll
+    for (int j = 0; j < B; ++j) T += (u32)__builtin_popcountll(__ballot((u32)j < cnt && nd_cls(nd[j]) == c));
+    const u32 q = (T + 63) / 64;
+    const u32 ppb = ha::ha_p.si[c].block_sz / 8u, pm = ha::mod_magic(ppb);  // 8 B pieces per block
+    for (u32 i = 0; i < q; ++i) {
+      for (u32 x = me; x < 64u * ppb; x += 64u) {
+        u32 b = ppb == 1 ? x : (u32)(((u64)x * pm) >> 32);  // x / ppb, or one less
+        b += x - b * ppb >= ppb;
+        const u32 r = b * q + i, e = 2u * (x - b * ppb);
+        if (r >= T) continue;
+        u32* a = (u32*)slots[at + r];
+        if (!a) continue;
+        const u32 rec = rk[at + r], j = rec & 15u, l = (rec >> 4) & 63u, d = rec >> 10;
+        if (!WRITE && e >= d) continue;
+        const u32 v = base + j * 64u + l;
+        const u32 w0 = e < d ? ha::mod_by(gh(seed, job, v, e), n, m) : 0u;
+        const u32 w1 = e + 1 < d ? ha::mod_by(gh(seed, job, v, e + 1), n, m) : 0u;
+        if (WRITE) {
+          *(uint2*)(a + e) = make_uint2(w0, w1);
+        } else {
+          const uint2 got = *(const uint2*)(a + e);
+          if (got.x != w0 || (e + 1 < d && got.y != w1)) flags[j * 64u + l] = 1;
+        }
+      }
+    }
+    at += T;
+  }
 }
 
 // LDS row of one lane: its own hamalloc_batch/hafree_batch pointer list of up
@@ -563,8 +620,8 @@ __device__ __forceinline__ void** lane_row(void** lds, u32 me) {
 }
 
 template <int B>
-__device__ void graph_chunk_build(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr** tab, void** lds, u32 me, u64 lt,
-                                  u32& nulls) {
+__device__ void graph_chunk_build(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr** tab, void** lds, u16* rk, u32 me,
+                                  u64 lt, u32& nulls) {
   const u32 base = chunk * 64u * B;
   u32 nd[B];
   u32 hdr[B];  // header offsets in the arena, in 16 B units (~0u: none)
@@ -588,14 +645,15 @@ __device__ void graph_chunk_build(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr**
   u32 at = 0;
   for (u32 cs = classes_present<B>(nd, cnt); cs; cs &= cs - 1) {
     const u32 c = (u32)__builtin_ctz(cs);
-    const u32 T = slot_class<B>(nd, cnt, c, at, lt);
+    const u32 T = slot_class<B>(nd, cnt, c, at, lt, rk, me);
     const u32 q = (T + 63) / 64, lo = me * q;  // lane share: [lo, lo + k)
     const u32 k = lo >= T ? 0u : (T - lo < q ? T - lo : q);
     if (k) hamalloc_batch(ha::ha_p.si[c].block_sz, (int)k, lds + at + lo);  // straight into LDS
     at += T;
   }
   wave_lds_sync();
-  const u32 m = ha::mod_magic(n);
+  // headers and table entries per node; a node that got one block but not
+  // the other gives it back (its slot goes NULL before the contents pass)
 #pragma unroll
   for (int j = 0; j < B; ++j) {
     if ((u32)j >= cnt) continue;
@@ -603,21 +661,26 @@ __device__ void graph_chunk_build(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr**
     NodeHdr* h = hdr[j] == ~0u ? nullptr : (NodeHdr*)(ha::ha_p.arena + ((u64)hdr[j] << 4));
     u32* a = (u32*)lds[nd_slot(nd[j])];
     if (h && a) {
-      fill_node(h, a, seed, job, v, nd_deg(nd[j]), n, m);
+      h->node = v;
+      h->deg = nd_deg(nd[j]);
+      h->adj = a;
     } else {
       nulls++;
       if (a) hafree(a);
       if (h) hafree(h);
+      lds[nd_slot(nd[j])] = nullptr;
       h = nullptr;
     }
     tab[v] = h;
   }
+  wave_lds_sync();
+  adjacency_runs<B, true>(nd, cnt, job, base, n, seed, lds, rk, nullptr, me);
   wave_lds_sync();  // the LDS slots are read before the next item reuses them
 }
 
 template <int B>
-__device__ void graph_chunk_free(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr** tab, void** lds, u32 me, u64 lt,
-                                 u32& bad) {
+__device__ void graph_chunk_free(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr** tab, void** lds, u16* rk, u8* flags,
+                                 u32 me, u64 lt, u32& bad) {
   const u32 base = chunk * 64u * B;
   u32 nd[B];
   void* hdr[B];
@@ -632,7 +695,6 @@ __device__ void graph_chunk_free(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr** 
     hdr[j] = v < n ? tab[v] : nullptr;
   }
   constexpr int H = B < 8 ? B : 8;  // headers in groups of 8 bound the registers they hold
-  u32 maxd = 0;
 #pragma unroll
   for (int j0 = 0; j0 < B; j0 += H) {
     uint4 hv[H];
@@ -650,7 +712,6 @@ __device__ void graph_chunk_free(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr** 
         if (hv[t].x != v || hv[t].y != d || !a) bad++;  // a damaged node's adjacency is not trusted
         else adj[j] = (void*)a;
       }
-      if (adj[j] && d > maxd) maxd = d;
     }
   }
   // the headers are read: free them (the lane's list through its LDS row)
@@ -660,29 +721,24 @@ __device__ void graph_chunk_free(u32 job, u32 chunk, u32 n, u32 seed, NodeHdr** 
     if ((u32)j < cnt) row[j] = hdr[j];
   if (cnt) hafree_batch(row, (int)cnt);
   wave_lds_sync();
-  const u32 m = ha::mod_magic(n);
+  // the adjacency contents, class-major in the layout the build allocated
+  // them in (same nodes, same classes, same ranks)
+#pragma unroll
+  for (int j = 0; j < B; ++j) flags[j * 64 + me] = 0;
+  u32 at = 0;
+  for (u32 cs = classes_present<B>(nd, cnt); cs; cs &= cs - 1) at += slot_class<B>(nd, cnt, (u32)__builtin_ctz(cs), at, lt, rk, me);
+#pragma unroll
+  for (int j = 0; j < B; ++j)
+    if ((u32)j < cnt) lds[nd_slot(nd[j])] = adj[j];
+  wave_lds_sync();
+  adjacency_runs<B, false>(nd, cnt, job, base, n, seed, lds, rk, flags, me);
+  wave_lds_sync();
   u32 mism = 0;  // bit j: node j's adjacency differs from its hash
-  for (u32 e = 0; e < maxd; e += 4) {  // two pairs per node in flight
-    uint2 pv[B], pw[B];
 #pragma unroll
-    for (int j = 0; j < B; ++j) {
-      const u32 d = nd_deg(nd[j]);
-      const u32* a = (const u32*)adj[j] + e;
-      pv[j] = adj[j] && e < d ? *(const uint2*)a : make_uint2(0u, 0u);
-      pw[j] = adj[j] && e + 2 < d ? *(const uint2*)(a + 2) : make_uint2(0u, 0u);
-    }
+  for (int j = 0; j < B; ++j)
+    if (flags[j * 64 + me]) mism |= 1u << j;
 #pragma unroll
-    for (int j = 0; j < B; ++j) {
-      const u32 d = nd_deg(nd[j]);
-      if (!adj[j] || e >= d) continue;
-      const u32 v = base + (u32)j * 64u + me;
-      bool x = pv[j].x != ha::mod_by(gh(seed, job, v, e), n, m);
-      if (e + 1 < d) x |= pv[j].y != ha::mod_by(gh(seed, job, v, e + 1), n, m);
-      if (e + 2 < d) x |= pw[j].x != ha::mod_by(gh(seed, job, v, e + 2), n, m);
-      if (e + 3 < d) x |= pw[j].y != ha::mod_by(gh(seed, job, v, e + 3), n, m);
-      if (x) mism |= 1u << j;
-    }
-  }
+  for (int j = 0; j < B; ++j) nd[j] &= 0xFFFFu;  // the slot bits are reused below
   bad += (u32)__builtin_popcount(mism);
   // adjacency arrays: class c's T blocks are shared out as in their
   // allocation (lane l frees ranks [l*q, l*q + q), q = ceil(T/64), rank
@@ -735,8 +791,13 @@ template <int B>
 __global__ void __launch_bounds__(256, 3) graph_batch_k(const u64* seq, u32 nseq, const u32* coffs, const u64* offs,
                                                      const u32* nnodes, u32 seed, NodeHdr** table, u32* next,
                                                      u32* built, unsigned long long* stats) {
+  static_assert(B <= 16, "the node record holds j in 4 bits");
   __shared__ void* lds_all[4][64 * row_stride<B>()];
+  __shared__ u16 rk_all[4][64 * B];   // rank table: the node of each class-major slot
+  __shared__ u8 flags_all[4][64 * B];  // per node: its adjacency check failed
   void** lds = lds_all[threadIdx.x >> 6];
+  u16* rk = rk_all[threadIdx.x >> 6];
+  u8* flags = flags_all[threadIdx.x >> 6];
   const u32 me = threadIdx.x & 63;
   const u64 lt = me ? (~0ull >> (64 - me)) : 0ull;
   u32 nulls = 0, bad = 0;
@@ -754,9 +815,9 @@ __global__ void __launch_bounds__(256, 3) graph_batch_k(const u64* seq, u32 nseq
         while (rfl32(__hip_atomic_load(&built[job], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < nch)
           __builtin_amdgcn_s_sleep(1);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        graph_chunk_free<B>(job, chunk, nnodes[job], seed, table + offs[job], lds, me, lt, bad);
+        graph_chunk_free<B>(job, chunk, nnodes[job], seed, table + offs[job], lds, rk, flags, me, lt, bad);
       } else {
-        graph_chunk_build<B>(job, chunk, nnodes[job], seed, table + offs[job], lds, me, lt, nulls);
+        graph_chunk_build<B>(job, chunk, nnodes[job], seed, table + offs[job], lds, rk, me, lt, nulls);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the chunk and its table entries before the count
         if (me == 0) atomicAdd(&built[job], 1u);
       }
@@ -1467,8 +1528,56 @@ int ha_run_graphs(const ha_graph_args* a, const uint32_t* nnodes, ha_graph_resul
 #undef HA_GRAPH_BATCH
   HH(hipStreamEndCapture(s.s, &graph));
   HH(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  if (a->cross) {
+    // each form's graphs checked by the other form's code: the batched kernel
+    // builds (build items only) and the plain kernels check and free; then
+    // the plain kernels build and the batched kernel checks and frees (check
+    // items only, every job's build count preset to complete)
+    std::vector<u64> sb, sf;
+    std::vector<u32> nch(njobs);
+    for (u32 j = 0; j < njobs; ++j) {
+      nch[j] = coffs[j + 1] - coffs[j];
+      for (u32 c = 0; c < nch[j]; ++c) {
+        sb.push_back(((u64)j << 32) | c);
+        sf.push_back((1ull << 63) | ((u64)j << 32) | c);
+      }
+    }
+    // the batched kernel over `items`; preset: every job's build count starts complete
+    auto launch_items = [&](const std::vector<u64>& items, bool preset) -> int {
+      graph_reset_k<<<16, 256, 0, s.s>>>((u32*)dnext.p, (u32*)dbuilt.p, njobs);
+      if (preset) HH(hipMemcpyAsync(dbuilt.p, nch.data(), (size_t)njobs * 4, hipMemcpyHostToDevice, s.s));
+      HH(hipMemcpyAsync(dseq.p, items.data(), items.size() * 8, hipMemcpyHostToDevice, s.s));
+#define HA_GRAPH_ITEMS(BB)                                                                                   \
+  graph_batch_k<BB><<<bgrid, 256, 0, s.s>>>((const u64*)dseq.p, (u32)items.size(), (const u32*)dcoffs.p,     \
+                                            (const u64*)doffs.p, (const u32*)dn.p, a->seed, (NodeHdr**)dtab.p, \
+                                            (u32*)dnext.p, (u32*)dbuilt.p, st)
+      if (a->batch == 4)
+        HA_GRAPH_ITEMS(4);
+      else if (a->batch == 8)
+        HA_GRAPH_ITEMS(8);
+      else
+        HA_GRAPH_ITEMS(16);
+#undef HA_GRAPH_ITEMS
+      HH(hipGetLastError());
+      return HA_OK;
+    };
+    if (int rc = launch_items(sb, false)) return rc;
+    for (auto& c : chunks) {
+      const unsigned grid = (unsigned)((c.second - c.first + a->bs - 1) / a->bs);
+      graph_plain_free_k<<<grid, a->bs, 0, s.s>>>((const u64*)doffs.p, (const u32*)dn.p, njobs, a->seed, c.first,
+                                                   c.second, (NodeHdr**)dtab.p, st);
+    }
+    for (auto& c : chunks) {
+      const unsigned grid = (unsigned)((c.second - c.first + a->bs - 1) / a->bs);
+      graph_plain_build_k<<<grid, a->bs, 0, s.s>>>((const u64*)doffs.p, (const u32*)dn.p, njobs, a->seed, c.first,
+                                                    c.second, (NodeHdr**)dtab.p, st);
+    }
+    if (int rc = launch_items(sf, true)) return rc;
+    HH(hipStreamSynchronize(s.s));
+    HH(hipGetLastError());
+  }
   double best_plain = 1e30, best_batch = 1e30;
-  for (int r = 0; r < a->repeats; ++r) {
+  for (int r = 0; r < (a->cross ? 0 : a->repeats); ++r) {
     if (a->run_plain) {
       HH(hipStreamSynchronize(s.s));
       HH(hipEventRecord(e0, s.s));

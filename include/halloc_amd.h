@@ -170,7 +170,8 @@ int ha_run_phase(const ha_phase_args *a, ha_phase_result *res, double *trace, in
    errors counts wrong graph nodes. */
 typedef struct ha_graph_args {
   int32_t njobs, bs, batch, waves, plain_chunk_nodes, repeats, run_plain, run_batch;
-  uint32_t seed, pad;
+  uint32_t seed;
+  uint32_t cross; /* 1: instead of timing, build with each form and check and free with the other */
 } ha_graph_args;
 typedef struct ha_graph_result {
   uint64_t nodes, nulls, errors;

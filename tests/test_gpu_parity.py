@@ -322,6 +322,21 @@ def test_graph_builds_plain_and_batched(halloc, batch):
     assert M.check_dump(model_opts(opts), d)[0] == 0
 
 
+@pytest.mark.parametrize("batch", [4, 16])
+def test_graph_forms_check_each_other(halloc, batch):
+    # the batched kernel's graphs pass the plain kernel's per-node check and
+    # the plain kernel's graphs pass the batched kernel's class-major check:
+    # the two forms agree on every node's header and adjacency contents
+    say(f"graph cross check batch={batch}")
+    halloc(H.HallocOpts(memory=1 << 30))
+    nn = graph_sizes(np.random.default_rng(8), 40)
+    r = H.Halloc.run_graphs(nn, batch=batch, waves=512, cross=True)
+    assert r["nulls"] == 0 and r["errors"] == 0, r
+    g = H.Halloc.geometry()
+    d = H.Halloc.dump()
+    assert ((d[:g["nsbs"]] >> np.uint64(32)) == 0).all()  # every block came back
+
+
 def test_autotune_small_space():
     # the tuner's candidates run the batched graph form; every accepted one is
     # violation-free and the best is no slower than the start
