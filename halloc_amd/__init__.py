@@ -29,7 +29,7 @@ EXPORTS = ("ha_default_opts", "ha_init_c", "ha_shutdown_c", "ha_extfrag_c", "ha_
            "ha_last_error", "ha_geometry_c", "ha_replay", "ha_dump_words", "ha_dump",
            "ha_run_plain", "ha_malloc_sizes", "ha_free_ptrs", "ha_read_words", "ha_probe_atomics",
            "ha_run_lists", "ha_run_phase", "ha_replay_batch", "ha_run_graphs", "ha_run_serve",
-           "ha_probe_copy")
+           "ha_probe_copy", "ha_run_test")
 
 
 class HallocError(RuntimeError):
@@ -78,6 +78,20 @@ class _PhaseResult(ctypes.Structure):
     _fields_ = [("mallocs", ctypes.c_uint64), ("frees", ctypes.c_uint64), ("nulls", ctypes.c_uint64),
                 ("errors", ctypes.c_uint64), ("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
                 ("steps", ctypes.c_int32), ("trace_len", ctypes.c_int32), ("phase_steps", ctypes.c_int32 * 8)]
+
+
+class _TestOpts(ctypes.Structure):
+    _fields_ = [("test", ctypes.c_int32), ("nthreads", ctypes.c_int32), ("bs", ctypes.c_int32),
+                ("ntries", ctypes.c_int32), ("nallocs", ctypes.c_int32), ("niters", ctypes.c_int32),
+                ("alloc_sz", ctypes.c_uint32), ("max_alloc_sz", ctypes.c_uint32), ("max_alloc_sh", ctypes.c_uint32),
+                ("distr", ctypes.c_uint32), ("period_mask", ctypes.c_uint32), ("group_sh", ctypes.c_uint32),
+                ("seed", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("probabs", (ctypes.c_float * 2) * 2)]
+
+
+class _TestResult(ctypes.Structure):
+    _fields_ = [("mallocs", ctypes.c_uint64), ("frees", ctypes.c_uint64), ("nulls", ctypes.c_uint64),
+                ("checked", ctypes.c_uint64), ("checks_failed", ctypes.c_uint64), ("malloc_ms", ctypes.c_double),
+                ("free_ms", ctypes.c_double), ("tries_run", ctypes.c_int32), ("failed_try", ctypes.c_int32)]
 
 
 class _GraphArgs(ctypes.Structure):
@@ -165,6 +179,7 @@ def lib() -> ctypes.CDLL:
         "ha_run_graphs": (i32, [P(_GraphArgs), vp, P(_GraphResult)]),
         "ha_run_serve": (i32, [P(_ServeIO), P(_ServeResult)]),
         "ha_probe_copy": (i32, [u64, i32, P(ctypes.c_double)]),
+        "ha_run_test": (i32, [P(_TestOpts), P(_TestResult)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -317,6 +332,19 @@ class Halloc:
         return {"mallocs": int(r.mallocs), "frees": int(r.frees), "nulls": int(r.nulls), "errors": int(r.errors),
                 "kernel_ms": r.kernel_ms, "wall_ms": r.wall_ms, "steps": int(r.steps),
                 "phase_steps": list(r.phase_steps)[:nphases], "trace": trace[:r.trace_len]}
+
+    @staticmethod
+    def run_test(test, nthreads, bs, ntries, nallocs, niters, alloc_sz, max_alloc_sz, max_alloc_sh, distr,
+                 period_mask, group_sh, probabs, seed=0):
+        """One of the reference test programs' kernels (test 0 checkptr, 1
+        prob-checkptr, 2 throughput, 3 priv-throughput) over a CommonOpts option
+        set; see halloc_amd.harness for the command line."""
+        pr = ((ctypes.c_float * 2) * 2)(*[(ctypes.c_float * 2)(*row) for row in probabs])
+        a = _TestOpts(test, nthreads, bs, ntries, nallocs, niters, alloc_sz, max_alloc_sz, max_alloc_sh, distr,
+                      period_mask, group_sh, seed, 0, pr)
+        r = _TestResult()
+        _check(lib().ha_run_test(ctypes.byref(a), ctypes.byref(r)))
+        return {k: getattr(r, k) for k, _ in _TestResult._fields_}
 
     @staticmethod
     def run_graphs(nnodes, batch=8, waves=1024, bs=256, plain_chunk_nodes=1 << 23, repeats=1,

@@ -1040,6 +1040,157 @@ __global__ void __launch_bounds__(256) srv_loop_k(SrvArgs a) {
   }
 }
 
+// ------------------------------------------------------------ test harness
+// The reference test programs' device side (tst/common/common.h:213-416):
+// active threads (tid < n and not (tid >> group_sh) & period_mask), sizes
+// from the uniform / exponential-uniform / exponential-equal distributions,
+// and the two-state alloc/free action chain.  The reference draws from a
+// per-thread state array; here every draw is a hash of (seed, tid, draw
+// index), so a run is reproducible.
+struct TOpts {
+  u32 n, nallocs, niters, alloc_sz, max_alloc_sz, max_alloc_sh, distr, period_mask, group_sh, seed;
+  float p[2][2];  // p[phase][allocated]: the probability of acting
+};
+__device__ __forceinline__ bool t_inactive(const TOpts& o, u32 i) {
+  return i >= o.n || ((i >> o.group_sh) & o.period_mask);
+}
+// drandom(a, b) of tst/common/common.h:96-99 on a hashed draw
+__device__ __forceinline__ u32 t_between(u32 r, u32 a, u32 b) { return a + (r & 0x00ffffffu) % (b - a + 1); }
+__device__ __forceinline__ u32 t_draw(const TOpts& o, u32 i, u32 k) { return hash3(o.seed, i, k); }
+// next_alloc_sz (tst/common/common.h:236-276); k numbers the thread's draws
+__device__ u32 t_size(const TOpts& o, u32 i, u32 k) {
+  if (o.alloc_sz == o.max_alloc_sz) return o.alloc_sz;
+  const u32 r1 = t_draw(o, i, 2 * k), r2 = t_draw(o, i, 2 * k + 1);
+  u32 sh = 0;
+  if (o.distr == 1) return t_between(r1, o.alloc_sz, o.max_alloc_sz);  // uniform
+  if (o.distr == 2) sh = t_between(r1, 0, o.max_alloc_sh);             // exponential-uniform
+  else sh = (u32)__builtin_ffs((int)t_between(r1, 1, 1u << (o.max_alloc_sh + 1))) - 1;  // exponential-equal
+  const u32 sz = t_between(r2, o.alloc_sz << sh, (o.alloc_sz << (sh + 1)) - 1);
+  return sz < o.max_alloc_sz ? sz : o.max_alloc_sz;
+}
+// drandomb(p) of tst/common/common.h:109-114 (drandomf: 20-bit steps)
+__device__ __forceinline__ bool t_bernoulli(float p, u32 r) {
+  if (0.0f < p && p < 1.0f) return (1.0f / (1024.0f * 1024.0f)) * (float)t_between(r, 0, 1024u * 1024u - 1) <= p;
+  return p >= 1.0f;
+}
+// stats: mallocs, frees, nulls
+__device__ __forceinline__ void t_stats(unsigned long long* st, u32 m, u32 f, u32 nn) {
+  const unsigned long long a = wave_sum64(m), b = wave_sum64(f), c = wave_sum64(nn);
+  if ((threadIdx.x & 63) == 0) {
+    if (a) atomicAdd(&st[0], a);
+    if (b) atomicAdd(&st[1], b);
+    if (c) atomicAdd(&st[2], c);
+  }
+}
+// malloc_k / malloc_corr_k (tst/common/common.h:347-372): stamp writes the size into the block
+__global__ void t_malloc_k(TOpts o, void** ptrs, u32 try_id, int stamp, unsigned long long* st) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  u32 m = 0, nn = 0;
+  if (!t_inactive(o, i)) {
+    for (u32 a = 0; a < o.nallocs; ++a) {
+      const u32 sz = t_size(o, i, try_id * o.nallocs + a);
+      void* q = hamalloc(sz);
+      ptrs[i + (size_t)o.n * a] = q;
+      if (q && stamp) *(u32*)q = sz;
+      m += q != nullptr;
+      nn += q == nullptr;
+    }
+  }
+  t_stats(st, m, 0, nn);
+}
+// free_k (tst/common/common.h:375-383)
+__global__ void t_free_k(TOpts o, void** ptrs, unsigned long long* st) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  u32 f = 0;
+  if (!t_inactive(o, i)) {
+    for (u32 a = 0; a < o.nallocs; ++a) {
+      void*& q = ptrs[i + (size_t)o.n * a];
+      f += q != nullptr;
+      hafree(q);
+      q = nullptr;
+    }
+  }
+  t_stats(st, 0, f, 0);
+}
+// throughput_malloc_free_k (tst/perf/priv-throughput/priv-throughput.cu:14-27)
+__global__ void t_priv_k(TOpts o, void** ptrs, u32 try_id, unsigned long long* st) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  u32 m = 0, f = 0, nn = 0;
+  if (!t_inactive(o, i)) {
+    for (u32 it = 0; it < o.niters; ++it) {
+      for (u32 a = 0; a < o.nallocs; ++a) {
+        void* q = hamalloc(t_size(o, i, (try_id * o.niters + it) * o.nallocs + a));
+        ptrs[i + (size_t)o.n * a] = q;
+        m += q != nullptr;
+        nn += q == nullptr;
+      }
+      for (u32 a = 0; a < o.nallocs; ++a) {
+        void*& q = ptrs[i + (size_t)o.n * a];
+        f += q != nullptr;
+        hafree(q);
+        q = nullptr;
+      }
+    }
+  }
+  t_stats(st, m, f, nn);
+}
+// prob_corr_k (tst/corr/prob-checkptr/prob-checkptr.cu:13-45): per iteration a
+// thread in state allocated/free acts with p[phase][state], phase alternating
+// with (try * niters + iter); ctrs[i] = blocks the thread holds
+__global__ void t_prob_k(TOpts o, void** ptrs, u32* ctrs, u32 try_id, unsigned long long* st) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  u32 m = 0, f = 0, nn = 0;
+  if (!t_inactive(o, i)) {
+    u32 ctr = ctrs[i];
+    for (u32 it = 0; it < o.niters; ++it) {
+      const u32 step = try_id * o.niters + it, phase = step % 2;
+      const u32 r = t_draw(o, i, 0x80000000u | step);
+      if (!t_bernoulli(o.p[phase][ctr > 0], r)) continue;
+      if (ctr == 0) {
+        for (u32 a = 0; a < o.nallocs; ++a) {
+          const u32 sz = t_size(o, i, step * o.nallocs + a);
+          void* q = hamalloc(sz);
+          ptrs[i + (size_t)o.n * a] = q;
+          if (q) *(u32*)q = sz;
+          m += q != nullptr;
+          nn += q == nullptr;
+        }
+        ctr = o.nallocs;
+      } else {
+        for (u32 a = 0; a < o.nallocs; ++a) {
+          void*& q = ptrs[i + (size_t)o.n * a];
+          f += q != nullptr;
+          hafree(q);
+          q = nullptr;
+        }
+        ctr = 0;
+      }
+    }
+    ctrs[i] = ctr;
+  }
+  t_stats(st, m, f, nn);
+}
+// free_rest_k (tst/common/common.h:387-398)
+__global__ void t_free_rest_k(TOpts o, void** ptrs, u32* ctrs, unsigned long long* st) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  u32 f = 0;
+  if (!t_inactive(o, i)) {
+    for (u32 a = 0; a < ctrs[i]; ++a) {
+      void*& q = ptrs[i + (size_t)o.n * a];
+      f += q != nullptr;
+      hafree(q);
+      q = nullptr;
+    }
+    ctrs[i] = 0;
+  }
+  t_stats(st, 0, f, 0);
+}
+// the stamped size of every non-NULL pointer (0 for NULL)
+__global__ void t_sizes_k(void* const* ptrs, size_t n, u32* out) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+    out[k] = ptrs[k] ? *(const u32*)ptrs[k] : 0u;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1212,6 +1363,137 @@ int ha_probe_atomics(double* out) {
     out[shape] = (double)NT * iters / (best * 1e-3);
   }
   HH(hipGetLastError());
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return HA_OK;
+}
+
+
+int ha_run_test(const ha_test_opts* a, ha_test_result* r) {
+  if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
+  if (!a || !r || a->test < 0 || a->test > 3 || a->nthreads < 1 || a->nthreads > (1 << 26) || a->bs < 1 ||
+      a->bs > 1024 || a->ntries < 1 || a->nallocs < 1 || a->nallocs > 64 || a->niters < 1 ||
+      a->max_alloc_sz < a->alloc_sz || a->distr < 1 || a->distr > 3 || a->group_sh > 31 || a->max_alloc_sh > 24 ||
+      ((uint64_t)a->alloc_sz << (a->max_alloc_sh + 1)) > 0xFFFFFFFFull ||
+      (uint64_t)a->nthreads * (uint64_t)a->nallocs > (1ull << 28))
+    return hfail(HA_E_INVALID, "bad test options");
+  const u32 n = (u32)a->nthreads;
+  const size_t nptrs = (size_t)n * (u32)a->nallocs;
+  TOpts o{};
+  o.n = n; o.nallocs = (u32)a->nallocs; o.niters = (u32)a->niters;
+  o.alloc_sz = a->alloc_sz; o.max_alloc_sz = a->max_alloc_sz; o.max_alloc_sh = a->max_alloc_sh;
+  o.distr = a->distr; o.period_mask = a->period_mask; o.group_sh = a->group_sh; o.seed = a->seed;
+  for (int x = 0; x < 2; ++x)
+    for (int y = 0; y < 2; ++y) o.p[x][y] = a->probabs[x][y];
+  DBuf dp, dc, ds, dsz;
+  HH(dp.alloc(nptrs * 8));
+  HH(dc.alloc((size_t)n * 4));
+  HH(ds.alloc(3 * 8));
+  HH(hipMemset(dp.p, 0, nptrs * 8));
+  HH(hipMemset(dc.p, 0, (size_t)n * 4));
+  HH(hipMemset(ds.p, 0, 3 * 8));
+  const bool stamped = a->test == 0 || a->test == 1;
+  if (stamped) HH(dsz.alloc(nptrs * 4));
+  Stream s;
+  HH(hipStreamCreate(&s.s));
+  hipEvent_t e0, e1;
+  HH(hipEventCreate(&e0));
+  HH(hipEventCreate(&e1));
+  auto timed = [&](auto&& launch, double& acc) -> int {
+    HH(hipEventRecord(e0, s.s));
+    launch();
+    HH(hipGetLastError());
+    HH(hipEventRecord(e1, s.s));
+    HH(hipStreamSynchronize(s.s));
+    float t = 0;
+    HH(hipEventElapsedTime(&t, e0, e1));
+    acc += t;
+    return HA_OK;
+  };
+  std::memset(r, 0, sizeof *r);
+  r->failed_try = -1;
+  std::vector<void*> hp(nptrs);
+  std::vector<u32> hc(n), hs(stamped ? nptrs : 0);
+  // check_alloc (tst/common/common.cu:412-529): every pointer an active
+  // thread holds is non-NULL, and sorted, each leaves room for its size
+  // before the next (the requested size when fixed, else the stamped one)
+  auto check = [&](bool with_ctrs) -> int {
+    HH(hipMemcpy(hp.data(), dp.p, nptrs * 8, hipMemcpyDeviceToHost));
+    if (with_ctrs) HH(hipMemcpy(hc.data(), dc.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (stamped) {
+      t_sizes_k<<<1024, 256, 0, s.s>>>((void* const*)dp.p, nptrs, (u32*)dsz.p);
+      HH(hipGetLastError());
+      HH(hipStreamSynchronize(s.s));
+      HH(hipMemcpy(hs.data(), dsz.p, nptrs * 4, hipMemcpyDeviceToHost));
+    }
+    std::vector<std::pair<uint64_t, u32>> live;
+    uint64_t bad = 0;
+    for (u32 i = 0; i < n; ++i) {
+      if (i >= n || ((i >> o.group_sh) & o.period_mask)) continue;
+      const u32 cnt = with_ctrs ? hc[i] : o.nallocs;
+      for (u32 k = 0; k < cnt; ++k) {
+        const size_t idx = i + (size_t)n * k;
+        if (!hp[idx]) {
+          ++bad;
+          continue;
+        }
+        const u32 sz = o.alloc_sz == o.max_alloc_sz ? o.alloc_sz : (stamped ? hs[idx] : 0u);
+        live.push_back({(uint64_t)hp[idx], sz});
+      }
+    }
+    std::sort(live.begin(), live.end());
+    for (size_t k = 0; k + 1 < live.size(); ++k)
+      if (live[k].first + live[k].second > live[k + 1].first) ++bad;
+    r->checked += live.size();
+    r->checks_failed += bad;
+    return HA_OK;
+  };
+  const unsigned grid = (n + (u32)a->bs - 1) / (u32)a->bs;
+  unsigned long long* st = (unsigned long long*)ds.p;
+  for (int t = 0; t < a->ntries; ++t) {
+    const uint64_t before = r->checks_failed;
+    switch (a->test) {
+      case 0:  // checkptr
+        if (int rc = timed([&] { t_malloc_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32)t, 1, st); }, r->malloc_ms))
+          return rc;
+        if (int rc = check(false)) return rc;
+        if (int rc = timed([&] { t_free_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, st); }, r->free_ms)) return rc;
+        break;
+      case 1:  // prob-checkptr
+        if (int rc = timed([&] { t_prob_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32*)dc.p, (u32)t, st); },
+                           r->malloc_ms))
+          return rc;
+        if (int rc = check(true)) return rc;
+        break;
+      case 2: {  // throughput: check_nz only
+        if (int rc = timed([&] { t_malloc_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32)t, 0, st); }, r->malloc_ms))
+          return rc;
+        HH(hipMemcpy(hp.data(), dp.p, nptrs * 8, hipMemcpyDeviceToHost));
+        for (u32 i = 0; i < n; ++i) {
+          if ((i >> o.group_sh) & o.period_mask) continue;
+          for (u32 k = 0; k < o.nallocs; ++k) r->checks_failed += hp[i + (size_t)n * k] == nullptr;
+        }
+        if (int rc = timed([&] { t_free_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, st); }, r->free_ms)) return rc;
+        break;
+      }
+      default:  // priv-throughput
+        if (int rc = timed([&] { t_priv_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32)t, st); }, r->malloc_ms))
+          return rc;
+        break;
+    }
+    r->tries_run = t + 1;
+    if (r->checks_failed != before && r->failed_try < 0) r->failed_try = t;
+  }
+  if (a->test == 1) {
+    t_free_rest_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32*)dc.p, st);
+    HH(hipGetLastError());
+  }
+  HH(hipStreamSynchronize(s.s));
+  unsigned long long h[3];
+  HH(hipMemcpy(h, ds.p, sizeof h, hipMemcpyDeviceToHost));
+  r->mallocs = h[0];
+  r->frees = h[1];
+  r->nulls = h[2];
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return HA_OK;

@@ -1,0 +1,369 @@
+"""The reference test harness's command line over the MI355X allocator.
+
+The reference builds one CUDA program per test, all parsing the option set of
+``CommonOpts`` (tst/common/common.h:118-298, tst/common/common.cu:130-355), and
+sweeps them with tst/corr/run-all-tests.pl.  This module keeps that option set,
+its derived quantities and its tests, so a reference sweep can be re-run
+one-for-one; the kernels are the library's (``ha_run_test``):
+
+    python -m halloc_amd.harness checkptr -l1 -t4 -s64 -n1000000
+    python -m halloc_amd.harness prob-checkptr -i5 -t8 -f0.5 -F0.5 -e0.75 -g5 -l4 -s8 -S64 -duniform
+    python -m halloc_amd.harness throughput -s16 -S3072 -dexpequal -n262144 -l4
+    python -m halloc_amd.harness priv-throughput -n1048576 -T256 -s16 -l1
+    python -m halloc_amd.harness freeslabs
+    python -m halloc_amd.harness run-all [--quick]
+
+Options (same letters and ranges as the reference): -a allocator (halloc),
+-m memory, -C halloc fraction, -B busy fraction, -R roomy fraction, -b slab
+size shift, -D device, -n threads, -t tries, -T block size, -s / -S smallest /
+largest allocation, -l allocations per thread, -i iterations, -q period shift,
+-g group shift, -d distribution (uniform, expuniform, expequal), -f / -F / -e
+allocation, free and execution fractions.  Random draws come from a
+counter-based hash of (seed, thread, draw) instead of the reference's
+per-thread state array.
+"""
+from __future__ import annotations
+
+import dataclasses
+import getopt
+import math
+import sys
+
+DISTRS = {"uniform": 1, "expuniform": 2, "expequal": 3}
+TESTS = {"checkptr": 0, "prob-checkptr": 1, "throughput": 2, "priv-throughput": 3}
+NBYTES_IN_GIB = 1024.0 ** 3
+
+
+class OptionError(ValueError):
+    pass
+
+
+def parse_int(s: str, lo: int = 0, hi: int | None = None) -> int:
+    """parse_int of tst/common/common.cu:66-86: an integer within [lo, hi]."""
+    try:
+        v = int(s)
+    except ValueError:
+        raise OptionError(f"{s} is not an integer") from None
+    if v < lo or (hi is not None and v > hi):
+        raise OptionError(f"{v} is not in [{lo}, {hi if hi is not None else 'inf'}]")
+    return v
+
+
+def parse_double(s: str) -> float:
+    try:
+        return float(s)
+    except ValueError:
+        raise OptionError(f"{s} is not a number") from None
+
+
+@dataclasses.dataclass
+class CommonOpts:
+    """CommonOpts with the defaults of tst/common/common.h:123-130."""
+    allocator: str = "halloc"
+    memory: int = 512 * 1024 * 1024
+    halloc_fraction: float = 0.75
+    busy_fraction: float = 0.835
+    roomy_fraction: float = 0.6
+    sparse_fraction: float = 0.0125  # the harness's value; the library default is 0.012
+    sb_sz_sh: int = 22
+    device: int = 0
+    nthreads: int = 1024 * 1024
+    ntries: int = 8
+    alloc_sz: int = 16
+    max_alloc_sz: int = 16
+    nallocs: int = 4
+    niters: int = 1
+    bs: int = 128
+    period_mask: int = 0
+    group_sh: int = 0
+    distr: str = "uniform"
+    alloc_fraction: float = 1.0
+    free_fraction: float = 0.0
+    exec_fraction: float = 1.0
+    max_alloc_sh: int = 0
+    probabs: tuple = ((1.0, 1.0), (1.0, 1.0))
+
+    def __post_init__(self):
+        self.recompute_fields()
+
+    # -- parse_cmdline (tst/common/common.cu:130-261)
+    def parse(self, argv: list[str], device_memory: int | None = None) -> list[str]:
+        """Applies the options in argv; returns the other arguments."""
+        opts, rest = getopt.getopt(argv, "ha:m:C:B:R:D:b:n:t:T:s:S:l:i:q:g:d:f:F:e:")
+        min_explicit = max_explicit = False
+        for c, v in opts:
+            if c == "-h":
+                raise OptionError(__doc__)
+            if c == "-a":
+                if v != "halloc":
+                    raise OptionError(f"allocator {v} is not supported (only halloc)")
+                self.allocator = v
+            elif c == "-m":
+                self.memory = parse_int(v, 4096)
+            elif c == "-C":
+                self.halloc_fraction = parse_double(v)
+            elif c == "-B":
+                self.busy_fraction = parse_double(v)
+            elif c == "-R":
+                self.roomy_fraction = parse_double(v)
+            elif c == "-b":
+                self.sb_sz_sh = parse_int(v, 20, 26)
+            elif c == "-D":
+                self.device = parse_int(v, 0)
+            elif c == "-n":
+                self.nthreads = parse_int(v, 0)
+            elif c == "-t":
+                self.ntries = parse_int(v, 1)
+            elif c == "-T":
+                self.bs = parse_int(v, 1, 1024)
+            elif c == "-s":
+                min_explicit = True
+                self.alloc_sz = parse_int(v, 0)
+                if max_explicit:
+                    if self.max_alloc_sz < self.alloc_sz:
+                        raise OptionError("max allocation size should be >= min allocation size")
+                else:
+                    self.max_alloc_sz = self.alloc_sz
+            elif c == "-S":
+                max_explicit = True
+                self.max_alloc_sz = parse_int(v, 0)
+                if min_explicit:
+                    if self.max_alloc_sz < self.alloc_sz:
+                        raise OptionError("max allocation size should be >= min allocation size")
+                else:
+                    self.alloc_sz = self.max_alloc_sz
+            elif c == "-l":
+                self.nallocs = parse_int(v, 1)
+            elif c == "-i":
+                self.niters = parse_int(v, 1)
+            elif c == "-q":
+                sh = parse_int(v, 0, 31)
+                self.period_mask = (1 << sh) - 1 if sh > 0 else 0
+            elif c == "-g":
+                self.group_sh = parse_int(v, 0, 31)
+            elif c == "-d":
+                if v not in DISTRS:
+                    raise OptionError(f"unknown distribution {v}")
+                self.distr = v
+            elif c == "-f":
+                self.alloc_fraction = parse_double(v)
+            elif c == "-F":
+                self.free_fraction = parse_double(v)
+            elif c == "-e":
+                self.exec_fraction = parse_double(v)
+        if device_memory:  # the pool is capped at 3/4 of the device
+            self.memory = min(self.memory, int(0.75 * device_memory))
+        self.recompute_fields()
+        return rest
+
+    # -- recompute_fields (tst/common/common.cu:294-341)
+    def recompute_fields(self):
+        sh = 0
+        while self.alloc_sz > 0 and self.max_alloc_sz >= self.alloc_sz << (sh + 1):
+            sh += 1
+        self.max_alloc_sh = sh
+        f, F, e = self.alloc_fraction, self.free_fraction, self.exec_fraction
+        if e + f + F > 2:
+            raise OptionError("too large change fraction")
+        if e < abs(f - F):
+            raise OptionError("too small change fraction")
+        # state 0 = holds nothing (next action: allocate), 1 = holds blocks (next action: free)
+        p00 = (e + f - F) / (2 * (1 - F)) if F < 1 else 1.0
+        p01 = (e - f + F) / (2 * F) if F > 0 else 1.0
+        p10 = (e - f + F) / (2 * (1 - f)) if f < 1 else 1.0
+        p11 = (e + f - F) / (2 * f) if f > 0 else 1.0  # the reference leaves it unset for f = 0
+        self.probabs = ((p00, p01), (p10, p11))
+
+    def is_random(self) -> bool:
+        return self.alloc_sz != self.max_alloc_sz
+
+    def group(self) -> int:
+        return 1 << self.group_sh
+
+    def period(self) -> int:
+        return self.period_mask + 1
+
+    def nptrs_cont(self, nts: int) -> int:
+        """Active threads among the first nts (tst/common/common.h:287-290)."""
+        gp = self.group() * self.period()
+        return nts // gp * self.group() + min(nts % gp, self.group())
+
+    def expected_sz(self) -> float:
+        """Expected allocation size (tst/common/common.cu:263-285).  For
+        expequal it keeps the reference's sum, which halves the weight at every
+        level including the last (weights sum to 1 - 2^-(max_alloc_sh+1))."""
+        a, b, m = self.alloc_sz, self.max_alloc_sz, self.max_alloc_sh
+        if a == b:
+            return float(a)
+        if self.distr == "uniform":
+            return (a + b) / 2
+        levels = [((a << sh), min((a << (sh + 1)) - 1, b)) for sh in range(m + 1)]
+        if self.distr == "expuniform":
+            return sum((lo + hi) / 2 for lo, hi in levels) / (m + 1)
+        e, p = 0.0, 1.0
+        for lo, hi in levels:
+            p /= 2
+            e += (lo + hi) / 2 * p
+        return e
+
+    def total_nallocs(self) -> float:
+        return float(self.nptrs_cont(self.nthreads)) * self.nallocs * self.niters * self.ntries
+
+    def total_sz(self) -> float:
+        return self.expected_sz() * self.total_nallocs()
+
+    def halloc_opts(self):
+        import halloc_amd as H
+        return H.HallocOpts(memory=self.memory, halloc_fraction=self.halloc_fraction,
+                            busy_fraction=self.busy_fraction, roomy_fraction=self.roomy_fraction,
+                            sparse_fraction=self.sparse_fraction, sb_sz_sh=self.sb_sz_sh)
+
+
+def _kernel(test: str, o: CommonOpts, seed: int = 0) -> dict:
+    import halloc_amd as H
+    return H.Halloc.run_test(TESTS[test], o.nthreads, o.bs, o.ntries, o.nallocs, o.niters, o.alloc_sz,
+                             o.max_alloc_sz, o.max_alloc_sh, DISTRS[o.distr], o.period_mask, o.group_sh,
+                             o.probabs, seed)
+
+
+def run_test(test: str, o: CommonOpts, out=print) -> bool:
+    """One reference test on a fresh heap: the test body of
+    tst/corr/checkptr, tst/corr/prob-checkptr, tst/corr/freeslabs,
+    tst/perf/throughput or tst/perf/priv-throughput.  Returns whether it
+    passed (the reference exits with -1 where this returns False)."""
+    import halloc_amd as H
+    if o.nthreads == 0:
+        return True
+    H.Halloc.init(o.halloc_opts())
+    try:
+        if test == "freeslabs":
+            return _freeslabs(o, out)
+        if test in ("throughput", "priv-throughput"):
+            # the reference's warm-up: min(4 * bs, n) threads, one try
+            w = dataclasses.replace(o, nthreads=min(4 * o.bs, o.nthreads), ntries=1)
+            if test == "throughput":
+                w.niters = 1
+            _kernel(test, w)
+        if test in ("checkptr", "throughput"):
+            o = dataclasses.replace(o, niters=1)
+        r = _kernel(test, o)
+    finally:
+        H.Halloc.shutdown()
+    if test in ("checkptr", "prob-checkptr"):
+        ok = r["checks_failed"] == 0
+        if not ok:
+            out(f"cannot allocate enough memory or pointers overlap: {r['checks_failed']} of "
+                f"{r['checked'] + r['checks_failed']} pointers, first at try {r['failed_try']}")
+        return ok
+    n = o.total_nallocs()
+    tm, tf = r["malloc_ms"] / 1e3, r["free_ms"] / 1e3
+    if test == "throughput":
+        out(f"malloc throughput {n / tm * 1e-6:.2f} Mmallocs/s")
+        out(f"free throughput {n / tf * 1e-6:.2f} Mfrees/s")
+        out(f"pair throughput {n / (tm + tf) * 1e-6:.2f} Mpairs/s")
+        out(f"malloc speed {o.total_sz() / tm / NBYTES_IN_GIB:.2f} GiB/s")
+        out(f"pair speed {o.total_sz() / (tm + tf) / NBYTES_IN_GIB:.2f} GiB/s")
+        return r["checks_failed"] == 0
+    out(f"pair throughput {n / tm * 1e-6:.2f} Mpairs/s")
+    out(f"pair speed {o.total_sz() / tm / NBYTES_IN_GIB:.2f} GiB/s")
+    return r["nulls"] == 0
+
+
+def _freeslabs(o: CommonOpts, out) -> bool:
+    """tst/corr/freeslabs/freeslabs.cu:12-71: per try, sizes 16..1024 (steps
+    8, 16 from 64, 128 from 256), each allocating 40% of the memory (at most
+    nthreads blocks), checked, then freed: every slab must come back."""
+    max_n = o.nthreads
+    for _ in range(o.ntries):
+        step, sz = 8, 16
+        while sz <= 1024:
+            n = min(max_n, int(math.floor(0.4 * o.memory / sz)))
+            t = dataclasses.replace(o, nthreads=n, nallocs=1, niters=1, ntries=1, period_mask=0, group_sh=0,
+                                    alloc_sz=sz, max_alloc_sz=sz)
+            r = _kernel("checkptr", t)
+            if r["checks_failed"]:
+                out(f"allocation size {sz}: {r['checks_failed']} pointers NULL or overlapping")
+                return False
+            step = 128 if sz >= 256 else 16 if sz >= 64 else 8
+            sz += step
+    return True
+
+
+def run_all_sweep(memory: int = 512 * 1024 * 1024, quick: bool = False):
+    """The test list of tst/corr/run-all-tests.pl:20-75 as (test, argv) pairs.
+    quick: every fourth checkptr size, one freeslabs try, and the
+    prob-checkptr configurations with an eighth of the threads and two tries
+    (the pool stays as large: the size classes need their slabs)."""
+    scale = 1 / 8 if quick else 1.0
+    tests = []
+    step, sz, k = 8, 16, 0
+    while sz <= 32 * 1024:
+        fraction = 0.25 if sz <= 2 * 1024 else 0.125
+        n = int(math.floor(fraction * memory / sz))
+        if n and (not quick or k % 4 == 0):
+            tests.append(("checkptr", ["-l1", "-t4", f"-s{sz}", f"-n{n}", f"-m{memory}"]))
+        k += 1
+        step = (1024 * 1024 if sz >= 1024 * 1024 else 128 * 1024 if sz >= 128 * 1024 else 16 * 1024
+                if sz >= 16 * 1024 else 2 * 1024 if sz >= 2 * 1024 else 256 if sz >= 256 else 16
+                if sz >= 64 else 8)
+        sz += step
+    tests.append(("freeslabs", [f"-m{memory}"] + (["-t1"] if quick else [])))
+    for group in (0, 5, 10):
+        for niters in (1, 5):
+            ntries = math.ceil((1024 if group == 1 else 16384) / niters)
+            if quick:
+                ntries = 2
+            fixed = [f"-i{niters}", f"-t{ntries}", "-f0.5", "-F0.5", "-e0.75", f"-g{group}", f"-m{memory}"]
+            n = lambda k: f"-n{max(1, int(k * 1024 * scale))}"
+            tests.append(("prob-checkptr", fixed + ["-l4", n(1024), "-s8", "-S64", "-duniform"]))
+            tests.append(("prob-checkptr", fixed + ["-l1", n(1024), "-s8", "-S256", "-duniform"]))
+            tests.append(("prob-checkptr", fixed + ["-l4", n(1024), "-s8", "-S256", "-dexpequal"]))
+            tests.append(("prob-checkptr", fixed + ["-l1", n(64), "-s8", "-S3072", "-duniform"]))
+            tests.append(("prob-checkptr", fixed + ["-l4", n(128), "-s8", "-S3072", "-dexpequal"]))
+    return tests
+
+
+def defaults_for(test: str) -> CommonOpts:
+    """Per-test defaults the reference's mains set before parsing."""
+    o = CommonOpts()
+    if test == "freeslabs":
+        o.ntries = 4
+    return o
+
+
+def run_all(quick: bool = False, memory: int = 512 * 1024 * 1024, out=print) -> tuple[int, int]:
+    ntests = nok = 0
+    for test, argv in run_all_sweep(memory=memory, quick=quick):
+        o = defaults_for(test)
+        o.parse(argv)
+        ok = run_test(test, o, out=lambda m: None)
+        out(f"{'ok  ' if ok else 'FAIL'} {test} {' '.join(argv)}")
+        ntests += 1
+        nok += ok
+    out(f"tests: {ntests} TOTAL, {nok} SUCCEEDED, {ntests - nok} FAILED")
+    return ntests, nok
+
+
+def main(argv=None) -> int:
+    argv = list(sys.argv[1:] if argv is None else argv)
+    if not argv or argv[0] in ("-h", "--help"):
+        print(__doc__)
+        return 0
+    test, args = argv[0], argv[1:]
+    if test == "run-all":
+        n, ok = run_all(quick="--quick" in args)
+        return 0 if ok == n else 1
+    if test not in TESTS and test != "freeslabs":
+        print(f"unknown test {test}", file=sys.stderr)
+        return 2
+    o = defaults_for(test)
+    try:
+        o.parse(args)
+    except OptionError as e:
+        print(e, file=sys.stderr)
+        return 255
+    return 0 if run_test(test, o) else 255
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -217,6 +217,35 @@ typedef struct ha_serve_result {
 } ha_serve_result;
 int ha_run_serve(const ha_serve_io *io, ha_serve_result *res);
 
+/* The reference test programs' kernels (tst/corr/checkptr, tst/corr/prob-checkptr,
+   tst/perf/throughput, tst/perf/priv-throughput) for the option set of
+   tst/common/common.h CommonOpts, ntries times:
+     test 0 checkptr: allocate nallocs blocks per active thread (sizes stamped
+       into the blocks), check_alloc, free;
+     test 1 prob-checkptr: niters steps of the alloc/free chain with
+       probabs[phase][allocated], check_alloc of the blocks held; the rest is
+       freed at the end;
+     test 2 throughput: allocate, check that no pointer is NULL, free (timed);
+     test 3 priv-throughput: niters rounds of allocate-then-free (timed).
+   Active threads: tid < nthreads and not (tid >> group_sh) & period_mask.
+   Sizes: alloc_sz when max_alloc_sz == alloc_sz, else distr 1 uniform,
+   2 exponential-uniform, 3 exponential-equal over [alloc_sz, max_alloc_sz]
+   with max_alloc_sh = the largest sh with alloc_sz << sh <= max_alloc_sz.
+   Draws hash (seed, tid, draw index).  check_alloc: every held pointer is
+   non-NULL and, sorted, leaves room for its size; checks_failed counts the
+   violations (NULLs and overlaps), failed_try is the first try with one. */
+typedef struct ha_test_opts {
+  int32_t test, nthreads, bs, ntries, nallocs, niters;
+  uint32_t alloc_sz, max_alloc_sz, max_alloc_sh, distr, period_mask, group_sh, seed, pad;
+  float probabs[2][2];
+} ha_test_opts;
+typedef struct ha_test_result {
+  uint64_t mallocs, frees, nulls, checked, checks_failed;
+  double malloc_ms, free_ms;
+  int32_t tries_run, failed_try;
+} ha_test_result;
+int ha_run_test(const ha_test_opts *o, ha_test_result *r);
+
 /* Copy probe: `reps` launches of a streaming copy of `bytes` (a multiple of 16,
    1 MiB..64 GiB) between two device buffers; *gbps = 2 * bytes / best launch
    time.  The known-size kernel for calibrating HBM byte counters. */
