@@ -29,7 +29,7 @@ EXPORTS = ("ha_default_opts", "ha_init_c", "ha_shutdown_c", "ha_extfrag_c", "ha_
            "ha_last_error", "ha_geometry_c", "ha_replay", "ha_dump_words", "ha_dump",
            "ha_run_plain", "ha_malloc_sizes", "ha_free_ptrs", "ha_read_words", "ha_probe_atomics",
            "ha_run_lists", "ha_run_phase", "ha_replay_batch", "ha_run_graphs", "ha_run_serve",
-           "ha_probe_copy", "ha_run_test")
+           "ha_probe_copy", "ha_run_test", "ha_run_sample")
 
 
 class HallocError(RuntimeError):
@@ -92,6 +92,17 @@ class _TestResult(ctypes.Structure):
     _fields_ = [("mallocs", ctypes.c_uint64), ("frees", ctypes.c_uint64), ("nulls", ctypes.c_uint64),
                 ("checked", ctypes.c_uint64), ("checks_failed", ctypes.c_uint64), ("malloc_ms", ctypes.c_double),
                 ("free_ms", ctypes.c_double), ("tries_run", ctypes.c_int32), ("failed_try", ctypes.c_int32)]
+
+
+class _SampleArgs(ctypes.Structure):
+    _fields_ = [("sample", ctypes.c_int32), ("n", ctypes.c_int32), ("bs", ctypes.c_int32), ("ncells", ctypes.c_int32),
+                ("min_len", ctypes.c_int32), ("max_len", ctypes.c_int32), ("seed", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32)]
+
+
+class _SampleResult(ctypes.Structure):
+    _fields_ = [("nulls", ctypes.c_uint64), ("errors", ctypes.c_uint64), ("alloc_ms", ctypes.c_double),
+                ("work_ms", ctypes.c_double), ("free_ms", ctypes.c_double)]
 
 
 class _GraphArgs(ctypes.Structure):
@@ -180,6 +191,7 @@ def lib() -> ctypes.CDLL:
         "ha_run_serve": (i32, [P(_ServeIO), P(_ServeResult)]),
         "ha_probe_copy": (i32, [u64, i32, P(ctypes.c_double)]),
         "ha_run_test": (i32, [P(_TestOpts), P(_TestResult)]),
+        "ha_run_sample": (i32, [P(_SampleArgs), P(_SampleResult)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -345,6 +357,15 @@ class Halloc:
         r = _TestResult()
         _check(lib().ha_run_test(ctypes.byref(a), ctypes.byref(r)))
         return {k: getattr(r, k) for k, _ in _TestResult._fields_}
+
+    @staticmethod
+    def run_sample(sample, n, bs=128, ncells=32, min_len=31, max_len=31, seed=0):
+        """A reference sample as an application check: "grid-points" or
+        "add-strings" (see include/halloc_amd.h ha_run_sample)."""
+        a = _SampleArgs({"grid-points": 0, "add-strings": 1}[sample], n, bs, ncells, min_len, max_len, seed, 0)
+        r = _SampleResult()
+        _check(lib().ha_run_sample(ctypes.byref(a), ctypes.byref(r)))
+        return {k: getattr(r, k) for k, _ in _SampleResult._fields_}
 
     @staticmethod
     def run_graphs(nnodes, batch=8, waves=1024, bs=256, plain_chunk_nodes=1 << 23, repeats=1,

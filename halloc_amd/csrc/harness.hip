@@ -1191,6 +1191,135 @@ __global__ void t_sizes_k(void* const* ptrs, size_t n, u32* out) {
     out[k] = ptrs[k] ? *(const u32*)ptrs[k] : 0u;
 }
 
+// ---------------------------------------------------------------- samples
+// samples/grid-points/grid-points.cu:61-133: every point gets a 16 B list
+// node from hamalloc, pushed onto its grid cell's list with an atomic
+// exchange; the lists are then walked and freed.  Points are uniform in
+// [0, 1)^3 in steps of 2^-16 (the sample's random() % 65536 / 65536), from a
+// hash.
+struct PointNode {
+  u32 ip, pad;
+  PointNode* next;
+};
+__device__ __forceinline__ u32 point_coord(u32 seed, u32 ip, u32 axis) { return hash3(seed, ip, axis) & 0xFFFFu; }
+__device__ __forceinline__ u32 point_cell(u32 seed, u32 ip, u32 nc) {
+  u32 c[3];
+  for (u32 a = 0; a < 3; ++a) {
+    const u32 k = (u32)floorf((float)point_coord(seed, ip, a) * (1.0f / 65536.0f) * (float)nc);
+    c[a] = k < nc ? k : nc - 1;
+  }
+  return c[0] + nc * (c[1] + nc * c[2]);
+}
+__global__ void grid_insert_k(PointNode** grid, u32 nc, u32 n, u32 seed, unsigned long long* st) {
+  const u32 ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= n) return;
+  PointNode* q = (PointNode*)hamalloc(sizeof(PointNode));
+  if (!q) {
+    atomicAdd(&st[0], 1ull);
+    return;
+  }
+  q->ip = ip;
+  q->next = (PointNode*)atomicExch((unsigned long long*)&grid[point_cell(seed, ip, nc)], (unsigned long long)q);
+}
+// a node pointer is followed only inside the arena (at most n steps per list)
+__device__ __forceinline__ bool in_arena(const void* q) {
+  return (u64)((const char*)q - ha::ha_p.arena) + sizeof(PointNode) <= ha::ha_p.arena_bytes;
+}
+// every point is in the list of its own cell, exactly once
+__global__ void grid_check_k(PointNode* const* grid, u32 nc, u32 n, u32 seed, u32* seen, unsigned long long* st) {
+  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc * nc * nc) return;
+  u32 cnt = 0, bad = 0;
+  for (const PointNode* q = grid[c]; q && cnt <= n; ++cnt) {
+    if (!in_arena(q)) {
+      ++bad;
+      break;
+    }
+    const u32 ip = q->ip;
+    if (ip >= n || point_cell(seed, ip, nc) != c || atomicAdd(&seen[ip], 1u) != 0) ++bad;
+    q = q->next;
+  }
+  atomicAdd(&st[1], (unsigned long long)bad);
+  atomicAdd(&st[2], (unsigned long long)cnt);
+}
+__global__ void grid_free_k(PointNode** grid, u32 nc, u32 n) {
+  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc * nc * nc) return;
+  PointNode* q = grid[c];
+  for (u32 k = 0; q && k <= n && in_arena(q); ++k) {
+    PointNode* nx = q->next;
+    hafree(q);
+    q = nx;
+  }
+  grid[c] = nullptr;
+}
+
+// samples/add-strings/add-strings.cu:71-248: two arrays of n strings are
+// allocated and filled, concatenated elementwise into a third (its length
+// measured, its block from hamalloc), and all three freed.  Lengths are
+// uniform in [min_len, max_len] (31 in the sample); the characters follow
+// the string and position, so the concatenation can be checked.
+__device__ __forceinline__ u32 str_len(u32 seed, u32 which, u32 i, u32 lo, u32 hi) {
+  return lo + hash3(seed ^ which, i, 77u) % (hi - lo + 1);
+}
+__device__ __forceinline__ char str_char(u32 which, u32 i, u32 j) { return (char)('a' + (i * 7u + j + which * 13u) % 26u); }
+__global__ void str_alloc_k(char** strs, u32 which, u32 n, u32 lo, u32 hi, u32 seed, unsigned long long* st) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u32 l = str_len(seed, which, i, lo, hi);
+  u64* w = (u64*)hamalloc(l + 1);
+  strs[i] = (char*)w;
+  if (!w) {
+    atomicAdd(&st[0], 1ull);
+    return;
+  }
+  // 8 bytes at a time (blocks are 8-byte aligned and a multiple of 8 long)
+  for (u32 j0 = 0; j0 <= l; j0 += 8) {
+    u64 v = 0;
+    for (u32 b = 0; b < 8; ++b) {
+      const u32 j = j0 + b;
+      const u64 ch = j < l ? (u64)(unsigned char)str_char(which, i, j) : 0ull;
+      v |= ch << (8 * b);
+    }
+    w[j0 / 8] = v;
+  }
+}
+__global__ void str_add_k(char** c, char* const* a, char* const* b, u32 n, u32 max_len, unsigned long long* st) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !a[i] || !b[i]) {
+    if (i < n) c[i] = nullptr;
+    return;
+  }
+  u32 la = 0, lb = 0;
+  while (la < max_len && a[i][la]) ++la;
+  while (lb < max_len && b[i][lb]) ++lb;
+  char* s = (char*)hamalloc(la + lb + 1);
+  c[i] = s;
+  if (!s) {
+    atomicAdd(&st[0], 1ull);
+    return;
+  }
+  for (u32 j = 0; j < la; ++j) s[j] = a[i][j];
+  for (u32 j = 0; j < lb; ++j) s[la + j] = b[i][j];
+  s[la + lb] = 0;
+}
+__global__ void str_check_k(char* const* c, u32 n, u32 lo, u32 hi, u32 seed, unsigned long long* st) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !c[i]) return;
+  const u32 la = str_len(seed, 0, i, lo, hi), lb = str_len(seed, 1, i, lo, hi);
+  u32 bad = 0;
+  for (u32 j = 0; j < la; ++j) bad += c[i][j] != str_char(0, i, j);
+  for (u32 j = 0; j < lb; ++j) bad += c[i][la + j] != str_char(1, i, j);
+  bad += c[i][la + lb] != 0;
+  if (bad) atomicAdd(&st[1], 1ull);
+}
+__global__ void str_free_k(char** s, u32 n) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  hafree(s[i]);
+  s[i] = nullptr;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1496,6 +1625,78 @@ int ha_run_test(const ha_test_opts* a, ha_test_result* r) {
   r->nulls = h[2];
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  return HA_OK;
+}
+
+
+int ha_run_sample(const ha_sample_args* a, ha_sample_result* r) {
+  if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
+  if (!a || !r || a->sample < 0 || a->sample > 1 || a->n < 1 || a->n > (1 << 26) || a->bs < 64 || a->bs > 1024 ||
+      (a->sample == 0 && (a->ncells < 1 || a->ncells > 512)) ||
+      (a->sample == 1 && (a->min_len < 0 || a->max_len < a->min_len || a->max_len > 4096)))
+    return hfail(HA_E_INVALID, "bad sample arguments");
+  const u32 n = (u32)a->n;
+  Stream s;
+  HH(hipStreamCreate(&s.s));
+  hipEvent_t e[5];  // alloc, work, free: e0-e1, e1-e2, e3-e4
+  for (auto& x : e) HH(hipEventCreate(&x));
+  DBuf dst;
+  HH(dst.alloc(4 * 8));
+  HH(hipMemset(dst.p, 0, 4 * 8));
+  unsigned long long* st = (unsigned long long*)dst.p;
+  const unsigned grid = (n + (u32)a->bs - 1) / (u32)a->bs;
+  std::memset(r, 0, sizeof *r);
+  float t[3] = {0, 0, 0};
+  if (a->sample == 0) {
+    const u32 nc = (u32)a->ncells, ncells3 = nc * nc * nc;
+    DBuf dg, dseen;
+    HH(dg.alloc((size_t)ncells3 * 8));
+    HH(dseen.alloc((size_t)n * 4));
+    HH(hipMemset(dg.p, 0, (size_t)ncells3 * 8));
+    HH(hipMemset(dseen.p, 0, (size_t)n * 4));
+    const unsigned cgrid = (ncells3 + (u32)a->bs - 1) / (u32)a->bs;
+    HH(hipEventRecord(e[0], s.s));
+    grid_insert_k<<<grid, a->bs, 0, s.s>>>((PointNode**)dg.p, nc, n, a->seed, st);
+    HH(hipEventRecord(e[1], s.s));
+    grid_check_k<<<cgrid, a->bs, 0, s.s>>>((PointNode* const*)dg.p, nc, n, a->seed, (u32*)dseen.p, st);
+    HH(hipEventRecord(e[2], s.s));
+    HH(hipEventRecord(e[3], s.s));
+    grid_free_k<<<cgrid, a->bs, 0, s.s>>>((PointNode**)dg.p, nc, n);
+    HH(hipEventRecord(e[4], s.s));
+    HH(hipGetLastError());
+    HH(hipStreamSynchronize(s.s));
+  } else {
+    DBuf da, db, dc;
+    HH(da.alloc((size_t)n * 8));
+    HH(db.alloc((size_t)n * 8));
+    HH(dc.alloc((size_t)n * 8));
+    const u32 lo = (u32)a->min_len, hi = (u32)a->max_len;
+    HH(hipEventRecord(e[0], s.s));
+    str_alloc_k<<<grid, a->bs, 0, s.s>>>((char**)da.p, 0, n, lo, hi, a->seed, st);
+    str_alloc_k<<<grid, a->bs, 0, s.s>>>((char**)db.p, 1, n, lo, hi, a->seed, st);
+    HH(hipEventRecord(e[1], s.s));
+    str_add_k<<<grid, a->bs, 0, s.s>>>((char**)dc.p, (char* const*)da.p, (char* const*)db.p, n, hi, st);
+    HH(hipEventRecord(e[2], s.s));
+    str_check_k<<<grid, a->bs, 0, s.s>>>((char* const*)dc.p, n, lo, hi, a->seed, st);  // not timed
+    HH(hipEventRecord(e[3], s.s));
+    str_free_k<<<grid, a->bs, 0, s.s>>>((char**)da.p, n);
+    str_free_k<<<grid, a->bs, 0, s.s>>>((char**)db.p, n);
+    str_free_k<<<grid, a->bs, 0, s.s>>>((char**)dc.p, n);
+    HH(hipEventRecord(e[4], s.s));
+    HH(hipGetLastError());
+    HH(hipStreamSynchronize(s.s));
+  }
+  HH(hipEventElapsedTime(&t[0], e[0], e[1]));
+  HH(hipEventElapsedTime(&t[1], e[1], e[2]));
+  HH(hipEventElapsedTime(&t[2], e[3], e[4]));
+  unsigned long long h[4];
+  HH(hipMemcpy(h, dst.p, sizeof h, hipMemcpyDeviceToHost));
+  r->nulls = h[0];
+  r->errors = h[1] + (a->sample == 0 && h[2] + h[0] != n ? 1 : 0);  // grid: every point listed once
+  r->alloc_ms = t[0];
+  r->work_ms = t[1];
+  r->free_ms = t[2];
+  for (auto& x : e) (void)hipEventDestroy(x);
   return HA_OK;
 }
 

@@ -12,6 +12,8 @@ one-for-one; the kernels are the library's (``ha_run_test``):
     python -m halloc_amd.harness priv-throughput -n1048576 -T256 -s16 -l1
     python -m halloc_amd.harness freeslabs
     python -m halloc_amd.harness run-all [--quick]
+    python -m halloc_amd.harness grid-points      (samples/grid-points)
+    python -m halloc_amd.harness add-strings      (samples/add-strings)
 
 Options (same letters and ranges as the reference): -a allocator (halloc),
 -m memory, -C halloc fraction, -B busy fraction, -R roomy fraction, -b slab
@@ -344,6 +346,39 @@ def run_all(quick: bool = False, memory: int = 512 * 1024 * 1024, out=print) -> 
     return ntests, nok
 
 
+def run_sample(sample: str, out=print) -> bool:
+    """A reference sample's main: a warm-up run, then the measured one
+    (samples/grid-points/grid-points.cu:203-216: 10 000 points in 8^3 cells,
+    then 10^6 in 32^3, 512 MiB; samples/add-strings/add-strings.cu:410-423:
+    strings of 31 characters).  Returns whether every check passed."""
+    import halloc_amd as H
+    H.Halloc.init(H.HallocOpts(memory=512 << 20))
+    try:
+        if sample == "grid-points":
+            H.Halloc.run_sample(sample, 10000, ncells=8)
+            n = 1000000
+            r = H.Halloc.run_sample(sample, n, ncells=32)
+            out(f"allocation time {r['alloc_ms']:.2f} ms")
+            out(f"free time {r['free_ms']:.2f} ms")
+            out(f"allocation performance {n / r['alloc_ms'] * 1e-3:.2f} Mpoints/s")
+            out(f"free performance {n / r['free_ms'] * 1e-3:.2f} Mpoints/s")
+        else:
+            H.Halloc.run_sample(sample, 10000)
+            n = 1000000
+            r = H.Halloc.run_sample(sample, n)
+            out(f"GPU allocation time: {r['alloc_ms'] / 2:.2f} ms")
+            out(f"GPU allocation performance: {n / (r['alloc_ms'] / 2) * 1e-3:.2f} Mstrings/s")
+            out(f"GPU concatenation time: {r['work_ms']:.2f} ms")
+            out(f"GPU concatenation performance: {n / r['work_ms'] * 1e-3:.2f} Mstrings/s")
+            out(f"GPU freeing time: {r['free_ms'] / 3:.2f} ms")
+            out(f"GPU freeing performance: {n / (r['free_ms'] / 3) * 1e-3:.2f} Mstrings/s")
+    finally:
+        H.Halloc.shutdown()
+    if r["nulls"] or r["errors"]:
+        out(f"{r['nulls']} NULLs, {r['errors']} failed checks")
+    return not r["nulls"] and not r["errors"]
+
+
 def main(argv=None) -> int:
     argv = list(sys.argv[1:] if argv is None else argv)
     if not argv or argv[0] in ("-h", "--help"):
@@ -353,6 +388,8 @@ def main(argv=None) -> int:
     if test == "run-all":
         n, ok = run_all(quick="--quick" in args)
         return 0 if ok == n else 1
+    if test in ("grid-points", "add-strings"):
+        return 0 if run_sample(test) else 255
     if test not in TESTS and test != "freeslabs":
         print(f"unknown test {test}", file=sys.stderr)
         return 2

@@ -246,6 +246,27 @@ typedef struct ha_test_result {
 } ha_test_result;
 int ha_run_test(const ha_test_opts *o, ha_test_result *r);
 
+/* The reference samples (samples/*) as application checks.
+     sample 0 grid-points: n points uniform in [0,1)^3 are sorted into an
+       ncells^3 grid, a 16 B list node from hamalloc per point pushed onto its
+       cell's list; the lists are checked (every point once, in its own cell)
+       and freed;
+     sample 1 add-strings: two arrays of n strings (lengths uniform in
+       [min_len, max_len]) allocated and filled, concatenated elementwise into
+       a third, checked, and all three freed.
+   alloc_ms / work_ms / free_ms: inserting or allocating / the grid check or
+   the concatenation / freeing; errors: points or strings that fail the
+   check. */
+typedef struct ha_sample_args {
+  int32_t sample, n, bs, ncells, min_len, max_len;
+  uint32_t seed, pad;
+} ha_sample_args;
+typedef struct ha_sample_result {
+  uint64_t nulls, errors;
+  double alloc_ms, work_ms, free_ms;
+} ha_sample_result;
+int ha_run_sample(const ha_sample_args *a, ha_sample_result *r);
+
 /* Copy probe: `reps` launches of a streaming copy of `bytes` (a multiple of 16,
    1 MiB..64 GiB) between two device buffers; *gbps = 2 * bytes / best launch
    time.  The known-size kernel for calibrating HBM byte counters. */

@@ -46,3 +46,28 @@ def test_exhaustion_is_reported():
     lines = []
     assert not harness.run_test("checkptr", o, out=lines.append)
     assert "cannot allocate enough memory" in lines[0]
+
+
+@pytest.mark.parametrize("sample", ["grid-points", "add-strings"])
+def test_samples(sample):
+    # the reference samples' mains: every point in its own cell once, every
+    # concatenation equal to its two strings, everything freed
+    say(f"sample {sample}")
+    lines = []
+    assert harness.run_sample(sample, out=lines.append)
+    say("\n".join(lines))
+
+
+def test_sample_string_lengths_vary():
+    # strings of 1..600 characters reach 14 size classes: with 16 head sets
+    # each, the pool needs room for that many heads (4 GiB: 768 slabs)
+    say("sample add-strings 1..600")
+    H.Halloc.init(H.HallocOpts(memory=4 << 30))
+    try:
+        r = H.Halloc.run_sample("add-strings", 100000, min_len=1, max_len=600, seed=3)
+        assert r["nulls"] == 0 and r["errors"] == 0, r
+        r = H.Halloc.run_sample("grid-points", 200000, ncells=5, seed=4)
+        assert r["nulls"] == 0 and r["errors"] == 0, r
+        assert H.Halloc.free_bytes() == H.Halloc.geometry()["arena_bytes"]
+    finally:
+        H.Halloc.shutdown()
