@@ -349,7 +349,7 @@ class Halloc:
     def run_test(test, nthreads, bs, ntries, nallocs, niters, alloc_sz, max_alloc_sz, max_alloc_sh, distr,
                  period_mask, group_sh, probabs, seed=0):
         """One of the reference test programs' kernels (test 0 checkptr, 1
-        prob-checkptr, 2 throughput, 3 priv-throughput) over a CommonOpts option
+        prob-checkptr, 2 throughput, 3 priv-throughput, 4 phase-throughput) over a CommonOpts option
         set; see halloc_amd.harness for the command line."""
         pr = ((ctypes.c_float * 2) * 2)(*[(ctypes.c_float * 2)(*row) for row in probabs])
         a = _TestOpts(test, nthreads, bs, ntries, nallocs, niters, alloc_sz, max_alloc_sz, max_alloc_sh, distr,
@@ -360,9 +360,11 @@ class Halloc:
 
     @staticmethod
     def run_sample(sample, n, bs=128, ncells=32, min_len=31, max_len=31, seed=0):
-        """A reference sample as an application check: "grid-points" or
-        "add-strings" (see include/halloc_amd.h ha_run_sample)."""
-        a = _SampleArgs({"grid-points": 0, "add-strings": 1}[sample], n, bs, ncells, min_len, max_len, seed, 0)
+        """A reference sample as an application check: "grid-points",
+        "add-strings" or "random-graph" (max_len: the largest degree); see
+        include/halloc_amd.h ha_run_sample."""
+        a = _SampleArgs({"grid-points": 0, "add-strings": 1, "random-graph": 2}[sample], n, bs, ncells, min_len,
+                        max_len, seed, 0)
         r = _SampleResult()
         _check(lib().ha_run_sample(ctypes.byref(a), ctypes.byref(r)))
         return {k: getattr(r, k) for k, _ in _SampleResult._fields_}

@@ -1137,7 +1137,7 @@ __global__ void t_priv_k(TOpts o, void** ptrs, u32 try_id, unsigned long long* s
 // prob_corr_k (tst/corr/prob-checkptr/prob-checkptr.cu:13-45): per iteration a
 // thread in state allocated/free acts with p[phase][state], phase alternating
 // with (try * niters + iter); ctrs[i] = blocks the thread holds
-__global__ void t_prob_k(TOpts o, void** ptrs, u32* ctrs, u32 try_id, unsigned long long* st) {
+__global__ void t_prob_k(TOpts o, void** ptrs, u32* ctrs, u32 try_id, int stamp, unsigned long long* st) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
   u32 m = 0, f = 0, nn = 0;
   if (!t_inactive(o, i)) {
@@ -1151,7 +1151,7 @@ __global__ void t_prob_k(TOpts o, void** ptrs, u32* ctrs, u32 try_id, unsigned l
           const u32 sz = t_size(o, i, step * o.nallocs + a);
           void* q = hamalloc(sz);
           ptrs[i + (size_t)o.n * a] = q;
-          if (q) *(u32*)q = sz;
+          if (q && stamp) *(u32*)q = sz;
           m += q != nullptr;
           nn += q == nullptr;
         }
@@ -1252,6 +1252,58 @@ __global__ void grid_free_k(PointNode** grid, u32 nc, u32 n) {
     q = nx;
   }
   grid[c] = nullptr;
+}
+
+// samples/random-graph/random-graph.cu:56-181: vertex v gets U{1..max_degree}
+// edges, each a 16 B list node {target, next} from hamalloc pushed onto its
+// own list, targets uniform over the vertices; the lists are then checked
+// (length = degree, targets in range) and freed (the sample leaves them).
+struct EdgeNode {
+  const void* target;
+  EdgeNode* next;
+};
+struct Vertex {
+  u32 id, nedges;
+  EdgeNode* edges;
+};
+__global__ void rgraph_build_k(Vertex* vs, u32 nv, u32 max_degree, u32 seed, unsigned long long* st) {
+  const u32 v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  const u32 ne = 1 + hash3(seed, v, 0xE0u) % max_degree;
+  Vertex x{v, 0u, nullptr};
+  for (u32 e = 0; e < ne; ++e) {
+    EdgeNode* q = (EdgeNode*)hamalloc(sizeof(EdgeNode));
+    if (!q) {
+      atomicAdd(&st[0], 1ull);
+      continue;
+    }
+    q->target = vs + hash3(seed, v, e) % nv;
+    q->next = x.edges;
+    x.edges = q;
+    x.nedges++;
+  }
+  vs[v] = x;
+}
+__global__ void rgraph_check_free_k(Vertex* vs, u32 nv, u32 max_degree, u32 seed, int check, unsigned long long* st) {
+  const u32 v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  const u32 ne = 1 + hash3(seed, v, 0xE0u) % max_degree;
+  u32 cnt = 0, bad = 0;
+  EdgeNode* q = vs[v].edges;
+  // pushed last to first: node k of the walk is edge (nedges - 1 - k); a
+  // node pointer is followed only inside the arena
+  for (; q && cnt < ne; ++cnt) {
+    if ((u64)((char*)q - ha::ha_p.arena) + sizeof(EdgeNode) > ha::ha_p.arena_bytes) {
+      ++bad;
+      break;
+    }
+    EdgeNode* nx = q->next;
+    if (check && q->target != vs + hash3(seed, v, vs[v].nedges - 1 - cnt) % nv) ++bad;
+    if (!check) hafree(q);
+    q = nx;
+  }
+  if (check && (q || cnt != vs[v].nedges || vs[v].id != v)) ++bad;
+  if (bad) atomicAdd(&st[1], 1ull);
 }
 
 // samples/add-strings/add-strings.cu:71-248: two arrays of n strings are
@@ -1500,7 +1552,7 @@ int ha_probe_atomics(double* out) {
 
 int ha_run_test(const ha_test_opts* a, ha_test_result* r) {
   if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
-  if (!a || !r || a->test < 0 || a->test > 3 || a->nthreads < 1 || a->nthreads > (1 << 26) || a->bs < 1 ||
+  if (!a || !r || a->test < 0 || a->test > 4 || a->nthreads < 1 || a->nthreads > (1 << 26) || a->bs < 1 ||
       a->bs > 1024 || a->ntries < 1 || a->nallocs < 1 || a->nallocs > 64 || a->niters < 1 ||
       a->max_alloc_sz < a->alloc_sz || a->distr < 1 || a->distr > 3 || a->group_sh > 31 || a->max_alloc_sh > 24 ||
       ((uint64_t)a->alloc_sz << (a->max_alloc_sh + 1)) > 0xFFFFFFFFull ||
@@ -1589,11 +1641,23 @@ int ha_run_test(const ha_test_opts* a, ha_test_result* r) {
         if (int rc = timed([&] { t_free_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, st); }, r->free_ms)) return rc;
         break;
       case 1:  // prob-checkptr
-        if (int rc = timed([&] { t_prob_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32*)dc.p, (u32)t, st); },
+        if (int rc = timed([&] { t_prob_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32*)dc.p, (u32)t, 1, st); },
                            r->malloc_ms))
           return rc;
         if (int rc = check(true)) return rc;
         break;
+      case 4: {  // phase-throughput: even tries count as malloc time, odd as free time; check_nz only
+        if (int rc = timed([&] { t_prob_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32*)dc.p, (u32)t, 0, st); },
+                           t % 2 ? r->free_ms : r->malloc_ms))
+          return rc;
+        HH(hipMemcpy(hp.data(), dp.p, nptrs * 8, hipMemcpyDeviceToHost));
+        HH(hipMemcpy(hc.data(), dc.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+        for (u32 i = 0; i < n; ++i) {
+          if ((i >> o.group_sh) & o.period_mask) continue;
+          for (u32 k = 0; k < hc[i]; ++k) r->checks_failed += hp[i + (size_t)n * k] == nullptr;
+        }
+        break;
+      }
       case 2: {  // throughput: check_nz only
         if (int rc = timed([&] { t_malloc_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32)t, 0, st); }, r->malloc_ms))
           return rc;
@@ -1616,6 +1680,9 @@ int ha_run_test(const ha_test_opts* a, ha_test_result* r) {
   if (a->test == 1) {
     t_free_rest_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32*)dc.p, st);
     HH(hipGetLastError());
+  } else if (a->test == 4) {  // the rest counts as free time
+    if (int rc = timed([&] { t_free_rest_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32*)dc.p, st); }, r->free_ms))
+      return rc;
   }
   HH(hipStreamSynchronize(s.s));
   unsigned long long h[3];
@@ -1631,9 +1698,10 @@ int ha_run_test(const ha_test_opts* a, ha_test_result* r) {
 
 int ha_run_sample(const ha_sample_args* a, ha_sample_result* r) {
   if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
-  if (!a || !r || a->sample < 0 || a->sample > 1 || a->n < 1 || a->n > (1 << 26) || a->bs < 64 || a->bs > 1024 ||
+  if (!a || !r || a->sample < 0 || a->sample > 2 || a->n < 1 || a->n > (1 << 26) || a->bs < 64 || a->bs > 1024 ||
       (a->sample == 0 && (a->ncells < 1 || a->ncells > 512)) ||
-      (a->sample == 1 && (a->min_len < 0 || a->max_len < a->min_len || a->max_len > 4096)))
+      (a->sample == 1 && (a->min_len < 0 || a->max_len < a->min_len || a->max_len > 4096)) ||
+      (a->sample == 2 && (a->max_len < 1 || a->max_len > 64)))
     return hfail(HA_E_INVALID, "bad sample arguments");
   const u32 n = (u32)a->n;
   Stream s;
@@ -1662,6 +1730,19 @@ int ha_run_sample(const ha_sample_args* a, ha_sample_result* r) {
     HH(hipEventRecord(e[2], s.s));
     HH(hipEventRecord(e[3], s.s));
     grid_free_k<<<cgrid, a->bs, 0, s.s>>>((PointNode**)dg.p, nc, n);
+    HH(hipEventRecord(e[4], s.s));
+    HH(hipGetLastError());
+    HH(hipStreamSynchronize(s.s));
+  } else if (a->sample == 2) {  // random-graph: max_len is the largest degree
+    DBuf dv;
+    HH(dv.alloc((size_t)n * sizeof(Vertex)));
+    HH(hipEventRecord(e[0], s.s));
+    rgraph_build_k<<<grid, a->bs, 0, s.s>>>((Vertex*)dv.p, n, (u32)a->max_len, a->seed, st);
+    HH(hipEventRecord(e[1], s.s));
+    rgraph_check_free_k<<<grid, a->bs, 0, s.s>>>((Vertex*)dv.p, n, (u32)a->max_len, a->seed, 1, st);
+    HH(hipEventRecord(e[2], s.s));
+    HH(hipEventRecord(e[3], s.s));
+    rgraph_check_free_k<<<grid, a->bs, 0, s.s>>>((Vertex*)dv.p, n, (u32)a->max_len, a->seed, 0, st);
     HH(hipEventRecord(e[4], s.s));
     HH(hipGetLastError());
     HH(hipStreamSynchronize(s.s));

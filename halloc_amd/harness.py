@@ -10,10 +10,12 @@ one-for-one; the kernels are the library's (``ha_run_test``):
     python -m halloc_amd.harness prob-checkptr -i5 -t8 -f0.5 -F0.5 -e0.75 -g5 -l4 -s8 -S64 -duniform
     python -m halloc_amd.harness throughput -s16 -S3072 -dexpequal -n262144 -l4
     python -m halloc_amd.harness priv-throughput -n1048576 -T256 -s16 -l1
+    python -m halloc_amd.harness phase-throughput -f0.95 -F0.05 -e0.91 -s16 -S256 -l1 -i1 -t10
     python -m halloc_amd.harness freeslabs
     python -m halloc_amd.harness run-all [--quick]
     python -m halloc_amd.harness grid-points      (samples/grid-points)
     python -m halloc_amd.harness add-strings      (samples/add-strings)
+    python -m halloc_amd.harness random-graph     (samples/random-graph)
 
 Options (same letters and ranges as the reference): -a allocator (halloc),
 -m memory, -C halloc fraction, -B busy fraction, -R roomy fraction, -b slab
@@ -32,7 +34,7 @@ import math
 import sys
 
 DISTRS = {"uniform": 1, "expuniform": 2, "expequal": 3}
-TESTS = {"checkptr": 0, "prob-checkptr": 1, "throughput": 2, "priv-throughput": 3}
+TESTS = {"checkptr": 0, "prob-checkptr": 1, "throughput": 2, "priv-throughput": 3, "phase-throughput": 4}
 NBYTES_IN_GIB = 1024.0 ** 3
 
 
@@ -240,10 +242,10 @@ def run_test(test: str, o: CommonOpts, out=print) -> bool:
     try:
         if test == "freeslabs":
             return _freeslabs(o, out)
-        if test in ("throughput", "priv-throughput"):
+        if test in ("throughput", "priv-throughput", "phase-throughput"):
             # the reference's warm-up: min(4 * bs, n) threads, one try
             w = dataclasses.replace(o, nthreads=min(4 * o.bs, o.nthreads), ntries=1)
-            if test == "throughput":
+            if test != "priv-throughput":
                 w.niters = 1
             _kernel(test, w)
         if test in ("checkptr", "throughput"):
@@ -265,6 +267,19 @@ def run_test(test: str, o: CommonOpts, out=print) -> bool:
         out(f"pair throughput {n / (tm + tf) * 1e-6:.2f} Mpairs/s")
         out(f"malloc speed {o.total_sz() / tm / NBYTES_IN_GIB:.2f} GiB/s")
         out(f"pair speed {o.total_sz() / (tm + tf) / NBYTES_IN_GIB:.2f} GiB/s")
+        return r["checks_failed"] == 0
+    if test == "phase-throughput":
+        # tst/perf/phase-throughput/phase-throughput.cu:110-135
+        npairs = 0.5 * n * o.exec_fraction
+        nm = 0.25 * n * (o.exec_fraction + o.alloc_fraction - o.free_fraction)
+        tp = tm + tf
+        if o.niters == 1:
+            out(f"malloc throughput {nm / tm * 1e-6:.2f} Mmallocs/s")
+            out(f"free throughput {nm / tf * 1e-6:.2f} Mfrees/s")
+        out(f"pair throughput {npairs / tp * 1e-6:.2f} Mpairs/s")
+        if o.niters == 1:
+            out(f"malloc speed {nm * o.expected_sz() / tm / NBYTES_IN_GIB:.2f} GiB/s")
+        out(f"pair speed {npairs * o.expected_sz() / tp / NBYTES_IN_GIB:.2f} GiB/s")
         return r["checks_failed"] == 0
     out(f"pair throughput {n / tm * 1e-6:.2f} Mpairs/s")
     out(f"pair speed {o.total_sz() / tm / NBYTES_IN_GIB:.2f} GiB/s")
@@ -350,7 +365,8 @@ def run_sample(sample: str, out=print) -> bool:
     """A reference sample's main: a warm-up run, then the measured one
     (samples/grid-points/grid-points.cu:203-216: 10 000 points in 8^3 cells,
     then 10^6 in 32^3, 512 MiB; samples/add-strings/add-strings.cu:410-423:
-    strings of 31 characters).  Returns whether every check passed."""
+    strings of 31 characters; samples/random-graph).  Returns whether every
+    check passed."""
     import halloc_amd as H
     H.Halloc.init(H.HallocOpts(memory=512 << 20))
     try:
@@ -362,6 +378,14 @@ def run_sample(sample: str, out=print) -> bool:
             out(f"free time {r['free_ms']:.2f} ms")
             out(f"allocation performance {n / r['alloc_ms'] * 1e-3:.2f} Mpoints/s")
             out(f"free performance {n / r['free_ms'] * 1e-3:.2f} Mpoints/s")
+        elif sample == "random-graph":
+            # samples/random-graph/random-graph.cu:238-246: 10^4 vertices of
+            # degree <= 4, then 10^6 of degree <= 8
+            H.Halloc.run_sample(sample, 10000, max_len=4)
+            n, d = 1000000, 8
+            r = H.Halloc.run_sample(sample, n, max_len=d)
+            out(f"GPU time: {r['alloc_ms']:.3f} ms")
+            out(f"GPU performance: {0.5 * (d + 1) * n / r['alloc_ms'] * 1e-3:.3f} Medges/s")
         else:
             H.Halloc.run_sample(sample, 10000)
             n = 1000000
@@ -388,7 +412,7 @@ def main(argv=None) -> int:
     if test == "run-all":
         n, ok = run_all(quick="--quick" in args)
         return 0 if ok == n else 1
-    if test in ("grid-points", "add-strings"):
+    if test in ("grid-points", "add-strings", "random-graph"):
         return 0 if run_sample(test) else 255
     if test not in TESTS and test != "freeslabs":
         print(f"unknown test {test}", file=sys.stderr)

@@ -226,7 +226,11 @@ int ha_run_serve(const ha_serve_io *io, ha_serve_result *res);
        probabs[phase][allocated], check_alloc of the blocks held; the rest is
        freed at the end;
      test 2 throughput: allocate, check that no pointer is NULL, free (timed);
-     test 3 priv-throughput: niters rounds of allocate-then-free (timed).
+     test 3 priv-throughput: niters rounds of allocate-then-free (timed);
+     test 4 phase-throughput (tst/perf/phase-throughput): the chain of test 1
+       without stamps, timed per try (even tries as malloc time, odd as free
+       time) with a NULL check of the blocks held; freeing the rest counts as
+       free time.
    Active threads: tid < nthreads and not (tid >> group_sh) & period_mask.
    Sizes: alloc_sz when max_alloc_sz == alloc_sz, else distr 1 uniform,
    2 exponential-uniform, 3 exponential-equal over [alloc_sz, max_alloc_sz]
@@ -253,7 +257,10 @@ int ha_run_test(const ha_test_opts *o, ha_test_result *r);
        and freed;
      sample 1 add-strings: two arrays of n strings (lengths uniform in
        [min_len, max_len]) allocated and filled, concatenated elementwise into
-       a third, checked, and all three freed.
+       a third, checked, and all three freed;
+     sample 2 random-graph: n vertices with U{1..max_len} edges each, every
+       edge a 16 B list node from hamalloc; the lists are checked (length,
+       targets) and freed.
    alloc_ms / work_ms / free_ms: inserting or allocating / the grid check or
    the concatenation / freeing; errors: points or strings that fail the
    check. */

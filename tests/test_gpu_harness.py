@@ -27,6 +27,7 @@ def test_run_all_sweep_quick():
 @pytest.mark.parametrize("test,argv", [
     ("throughput", ["-n65536", "-s16", "-S3072", "-dexpequal", "-l4", "-t2", "-m268435456"]),
     ("priv-throughput", ["-n262144", "-T256", "-s16", "-l1", "-i4", "-t2", "-m268435456"]),
+    ("phase-throughput", ["-n262144", "-f0.95", "-F0.05", "-e0.91", "-s16", "-S256", "-l1", "-t10"]),
 ])
 def test_perf_tests(test, argv):
     say(f"harness {test}")
@@ -48,7 +49,7 @@ def test_exhaustion_is_reported():
     assert "cannot allocate enough memory" in lines[0]
 
 
-@pytest.mark.parametrize("sample", ["grid-points", "add-strings"])
+@pytest.mark.parametrize("sample", ["grid-points", "add-strings", "random-graph"])
 def test_samples(sample):
     # the reference samples' mains: every point in its own cell once, every
     # concatenation equal to its two strings, everything freed
