@@ -85,13 +85,16 @@ class _TestOpts(ctypes.Structure):
                 ("ntries", ctypes.c_int32), ("nallocs", ctypes.c_int32), ("niters", ctypes.c_int32),
                 ("alloc_sz", ctypes.c_uint32), ("max_alloc_sz", ctypes.c_uint32), ("max_alloc_sh", ctypes.c_uint32),
                 ("distr", ctypes.c_uint32), ("period_mask", ctypes.c_uint32), ("group_sh", ctypes.c_uint32),
-                ("seed", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("probabs", (ctypes.c_float * 2) * 2)]
+                ("seed", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("probabs", (ctypes.c_float * 2) * 2),
+                ("extfrag", ctypes.POINTER(ctypes.c_double))]
 
 
 class _TestResult(ctypes.Structure):
     _fields_ = [("mallocs", ctypes.c_uint64), ("frees", ctypes.c_uint64), ("nulls", ctypes.c_uint64),
                 ("checked", ctypes.c_uint64), ("checks_failed", ctypes.c_uint64), ("malloc_ms", ctypes.c_double),
-                ("free_ms", ctypes.c_double), ("tries_run", ctypes.c_int32), ("failed_try", ctypes.c_int32)]
+                ("free_ms", ctypes.c_double), ("tries_run", ctypes.c_int32), ("failed_try", ctypes.c_int32),
+                ("lat_n", ctypes.c_uint64 * 2), ("lat_avg", ctypes.c_double * 2), ("lat_min", ctypes.c_double * 2),
+                ("lat_max", ctypes.c_double * 2)]
 
 
 class _SampleArgs(ctypes.Structure):
@@ -349,14 +352,20 @@ class Halloc:
     def run_test(test, nthreads, bs, ntries, nallocs, niters, alloc_sz, max_alloc_sz, max_alloc_sh, distr,
                  period_mask, group_sh, probabs, seed=0):
         """One of the reference test programs' kernels (test 0 checkptr, 1
-        prob-checkptr, 2 throughput, 3 priv-throughput, 4 phase-throughput) over a CommonOpts option
-        set; see halloc_amd.harness for the command line."""
+        prob-checkptr, 2 throughput, 3 priv-throughput, 4 phase-throughput, 5 latency,
+        6 phase-latency, 7 phase-alloc-write, 8 phase-extfrag) over a CommonOpts
+        option set; see halloc_amd.harness for the command line.  Test 8 adds
+        "extfrag": per try, [ha_extfrag(false), ha_extfrag(true)]."""
         pr = ((ctypes.c_float * 2) * 2)(*[(ctypes.c_float * 2)(*row) for row in probabs])
+        ef = (ctypes.c_double * (2 * max(ntries, 1)))()
         a = _TestOpts(test, nthreads, bs, ntries, nallocs, niters, alloc_sz, max_alloc_sz, max_alloc_sh, distr,
-                      period_mask, group_sh, seed, 0, pr)
+                      period_mask, group_sh, seed, 0, pr, ef)
         r = _TestResult()
         _check(lib().ha_run_test(ctypes.byref(a), ctypes.byref(r)))
-        return {k: getattr(r, k) for k, _ in _TestResult._fields_}
+        out = {k: (list(getattr(r, k)) if k.startswith("lat_") else getattr(r, k)) for k, _ in _TestResult._fields_}
+        if test == 8:
+            out["extfrag"] = [(ef[2 * t], ef[2 * t + 1]) for t in range(r.tries_run)]
+        return out
 
     @staticmethod
     def run_sample(sample, n, bs=128, ncells=32, min_len=31, max_len=31, seed=0):

@@ -1136,7 +1136,8 @@ __global__ void t_priv_k(TOpts o, void** ptrs, u32 try_id, unsigned long long* s
 }
 // prob_corr_k (tst/corr/prob-checkptr/prob-checkptr.cu:13-45): per iteration a
 // thread in state allocated/free acts with p[phase][state], phase alternating
-// with (try * niters + iter); ctrs[i] = blocks the thread holds
+// with (try * niters + iter); ctrs[i] = blocks the thread holds.  stamp: 0
+// none, 1 the size into the first word, 2 the fill of phase-alloc-write
 __global__ void t_prob_k(TOpts o, void** ptrs, u32* ctrs, u32 try_id, int stamp, unsigned long long* st) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
   u32 m = 0, f = 0, nn = 0;
@@ -1151,7 +1152,9 @@ __global__ void t_prob_k(TOpts o, void** ptrs, u32* ctrs, u32 try_id, int stamp,
           const u32 sz = t_size(o, i, step * o.nallocs + a);
           void* q = hamalloc(sz);
           ptrs[i + (size_t)o.n * a] = q;
-          if (q && stamp) *(u32*)q = sz;
+          if (q && stamp == 1) *(u32*)q = sz;
+          if (q && stamp == 2)  // phase-alloc-write.cu:33-37: 123 into every whole 8 B word
+            for (u32 w = 0; w < sz / 8; ++w) ((unsigned long long*)q)[w] = 123ull;
           m += q != nullptr;
           nn += q == nullptr;
         }
@@ -1185,6 +1188,101 @@ __global__ void t_free_rest_k(TOpts o, void** ptrs, u32* ctrs, unsigned long lon
   }
   t_stats(st, 0, f, 0);
 }
+// Latencies in shader-clock cycles (tst/perf/latency/latency.cu:14-42,
+// tst/perf/phase-latency/phase-latency.cu:18-63), summed per wave before one
+// lane adds them: L = {count, sum, min, max} for mallocs, then for frees.
+__device__ __forceinline__ unsigned long long wave_min64(unsigned long long v) {
+  for (int o = 32; o; o >>= 1) {
+    const unsigned long long w = __shfl_xor(v, o);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_max64(unsigned long long v) {
+  for (int o = 32; o; o >>= 1) {
+    const unsigned long long w = __shfl_xor(v, o);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+// every lane of the wave calls it; has: this lane timed an operation of kind k
+__device__ __forceinline__ void lat_add(unsigned long long* L, u32 k, bool has, unsigned long long cyc) {
+  const unsigned long long c = wave_sum64(has), sm = wave_sum64(has ? cyc : 0ull);
+  const unsigned long long mn = wave_min64(has ? cyc : ~0ull), mx = wave_max64(has ? cyc : 0ull);
+  if ((threadIdx.x & 63) == 0 && c) {
+    atomicAdd(&L[4 * k], c);
+    atomicAdd(&L[4 * k + 1], sm);
+    atomicMin(&L[4 * k + 2], mn);
+    atomicMax(&L[4 * k + 3], mx);
+  }
+}
+// latency test: every active thread's mallocs, then (second launch) its frees
+__global__ void t_lat_k(TOpts o, void** ptrs, u32 try_id, int do_free, unsigned long long* L,
+                        unsigned long long* st) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = !t_inactive(o, i);
+  u32 m = 0, f = 0, nn = 0;
+  for (u32 a = 0; a < o.nallocs; ++a) {
+    unsigned long long cyc = 0;
+    if (act) {
+      void*& q = ptrs[i + (size_t)o.n * a];
+      if (do_free) {
+        f += q != nullptr;
+        const unsigned long long t0 = clock64();
+        hafree(q);
+        cyc = clock64() - t0;
+        q = nullptr;
+      } else {
+        const u32 sz = t_size(o, i, try_id * o.nallocs + a);
+        const unsigned long long t0 = clock64();
+        q = hamalloc(sz);
+        cyc = clock64() - t0;
+        m += q != nullptr;
+        nn += q == nullptr;
+      }
+    }
+    lat_add(L, do_free ? 1u : 0u, act, cyc);
+  }
+  t_stats(st, m, f, nn);
+}
+// phase-latency: the chain of t_prob_k with every malloc and free timed
+__global__ void t_phase_lat_k(TOpts o, void** ptrs, u32* ctrs, u32 try_id, unsigned long long* L,
+                              unsigned long long* st) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = !t_inactive(o, i);
+  u32 ctr = act ? ctrs[i] : 0u, m = 0, f = 0, nn = 0;
+  for (u32 it = 0; it < o.niters; ++it) {
+    const u32 step = try_id * o.niters + it, phase = step % 2;
+    const bool go = act && t_bernoulli(o.p[phase][ctr > 0], t_draw(o, i, 0x80000000u | step));
+    const bool alloc = go && ctr == 0, release = go && ctr > 0;  // both imply an active thread
+    for (u32 a = 0; a < o.nallocs; ++a) {
+      unsigned long long cyc = 0;
+      if (alloc) {
+        const u32 sz = t_size(o, i, step * o.nallocs + a);
+        const unsigned long long t0 = clock64();
+        void* q = hamalloc(sz);
+        cyc = clock64() - t0;
+        ptrs[i + (size_t)o.n * a] = q;
+        m += q != nullptr;
+        nn += q == nullptr;
+      } else if (release) {
+        void*& q = ptrs[i + (size_t)o.n * a];
+        f += q != nullptr;
+        const unsigned long long t0 = clock64();
+        hafree(q);
+        cyc = clock64() - t0;
+        q = nullptr;
+      }
+      lat_add(L, 0u, alloc, cyc);
+      lat_add(L, 1u, release, cyc);
+    }
+    if (alloc) ctr = o.nallocs;
+    if (release) ctr = 0;
+  }
+  if (act) ctrs[i] = ctr;
+  t_stats(st, m, f, nn);
+}
+
 // the stamped size of every non-NULL pointer (0 for NULL)
 __global__ void t_sizes_k(void* const* ptrs, size_t n, u32* out) {
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
@@ -1552,7 +1650,7 @@ int ha_probe_atomics(double* out) {
 
 int ha_run_test(const ha_test_opts* a, ha_test_result* r) {
   if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
-  if (!a || !r || a->test < 0 || a->test > 4 || a->nthreads < 1 || a->nthreads > (1 << 26) || a->bs < 1 ||
+  if (!a || !r || a->test < 0 || a->test > 8 || a->nthreads < 1 || a->nthreads > (1 << 26) || a->bs < 1 ||
       a->bs > 1024 || a->ntries < 1 || a->nallocs < 1 || a->nallocs > 64 || a->niters < 1 ||
       a->max_alloc_sz < a->alloc_sz || a->distr < 1 || a->distr > 3 || a->group_sh > 31 || a->max_alloc_sh > 24 ||
       ((uint64_t)a->alloc_sz << (a->max_alloc_sh + 1)) > 0xFFFFFFFFull ||
@@ -1566,7 +1664,13 @@ int ha_run_test(const ha_test_opts* a, ha_test_result* r) {
   o.distr = a->distr; o.period_mask = a->period_mask; o.group_sh = a->group_sh; o.seed = a->seed;
   for (int x = 0; x < 2; ++x)
     for (int y = 0; y < 2; ++y) o.p[x][y] = a->probabs[x][y];
-  DBuf dp, dc, ds, dsz;
+  DBuf dp, dc, ds, dsz, dl;
+  HH(dl.alloc(8 * 8));
+  {
+    const unsigned long long l0[8] = {0, 0, ~0ull, 0, 0, 0, ~0ull, 0};
+    HH(hipMemcpy(dl.p, l0, sizeof l0, hipMemcpyHostToDevice));
+  }
+  unsigned long long* L = (unsigned long long*)dl.p;
   HH(dp.alloc(nptrs * 8));
   HH(dc.alloc((size_t)n * 4));
   HH(ds.alloc(3 * 8));
@@ -1646,10 +1750,17 @@ int ha_run_test(const ha_test_opts* a, ha_test_result* r) {
           return rc;
         if (int rc = check(true)) return rc;
         break;
-      case 4: {  // phase-throughput: even tries count as malloc time, odd as free time; check_nz only
-        if (int rc = timed([&] { t_prob_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32*)dc.p, (u32)t, 0, st); },
+      case 4:    // phase-throughput: even tries count as malloc time, odd as free time; check_nz only
+      case 7:    // phase-alloc-write: the same, each block filled
+      case 8: {  // phase-extfrag: the same, ha_extfrag after each try
+        const int fill = a->test == 7 ? 2 : 0;
+        if (int rc = timed([&] { t_prob_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32*)dc.p, (u32)t, fill, st); },
                            t % 2 ? r->free_ms : r->malloc_ms))
           return rc;
+        if (a->test == 8 && a->extfrag) {
+          if (int rc = ha_extfrag_c(0, &a->extfrag[2 * t])) return rc;
+          if (int rc = ha_extfrag_c(1, &a->extfrag[2 * t + 1])) return rc;
+        }
         HH(hipMemcpy(hp.data(), dp.p, nptrs * 8, hipMemcpyDeviceToHost));
         HH(hipMemcpy(hc.data(), dc.p, (size_t)n * 4, hipMemcpyDeviceToHost));
         for (u32 i = 0; i < n; ++i) {
@@ -1669,6 +1780,28 @@ int ha_run_test(const ha_test_opts* a, ha_test_result* r) {
         if (int rc = timed([&] { t_free_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, st); }, r->free_ms)) return rc;
         break;
       }
+      case 5:  // latency: mallocs timed, NULL check, frees timed
+        if (int rc = timed([&] { t_lat_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32)t, 0, L, st); }, r->malloc_ms))
+          return rc;
+        HH(hipMemcpy(hp.data(), dp.p, nptrs * 8, hipMemcpyDeviceToHost));
+        for (u32 i = 0; i < n; ++i) {
+          if ((i >> o.group_sh) & o.period_mask) continue;
+          for (u32 k = 0; k < o.nallocs; ++k) r->checks_failed += hp[i + (size_t)n * k] == nullptr;
+        }
+        if (int rc = timed([&] { t_lat_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32)t, 1, L, st); }, r->free_ms))
+          return rc;
+        break;
+      case 6:  // phase-latency, check_nz of the blocks held
+        if (int rc = timed([&] { t_phase_lat_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32*)dc.p, (u32)t, L, st); },
+                           r->malloc_ms))
+          return rc;
+        HH(hipMemcpy(hp.data(), dp.p, nptrs * 8, hipMemcpyDeviceToHost));
+        HH(hipMemcpy(hc.data(), dc.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+        for (u32 i = 0; i < n; ++i) {
+          if ((i >> o.group_sh) & o.period_mask) continue;
+          for (u32 k = 0; k < hc[i]; ++k) r->checks_failed += hp[i + (size_t)n * k] == nullptr;
+        }
+        break;
       default:  // priv-throughput
         if (int rc = timed([&] { t_priv_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32)t, st); }, r->malloc_ms))
           return rc;
@@ -1677,10 +1810,10 @@ int ha_run_test(const ha_test_opts* a, ha_test_result* r) {
     r->tries_run = t + 1;
     if (r->checks_failed != before && r->failed_try < 0) r->failed_try = t;
   }
-  if (a->test == 1) {
+  if (a->test == 1 || a->test == 6) {
     t_free_rest_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32*)dc.p, st);
     HH(hipGetLastError());
-  } else if (a->test == 4) {  // the rest counts as free time
+  } else if (a->test == 4 || a->test == 7 || a->test == 8) {  // the rest counts as free time
     if (int rc = timed([&] { t_free_rest_k<<<grid, a->bs, 0, s.s>>>(o, (void**)dp.p, (u32*)dc.p, st); }, r->free_ms))
       return rc;
   }
@@ -1690,6 +1823,14 @@ int ha_run_test(const ha_test_opts* a, ha_test_result* r) {
   r->mallocs = h[0];
   r->frees = h[1];
   r->nulls = h[2];
+  unsigned long long l[8];
+  HH(hipMemcpy(l, dl.p, sizeof l, hipMemcpyDeviceToHost));
+  for (int k = 0; k < 2; ++k) {
+    r->lat_n[k] = l[4 * k];
+    r->lat_avg[k] = l[4 * k] ? (double)l[4 * k + 1] / (double)l[4 * k] : 0.0;
+    r->lat_min[k] = l[4 * k] ? (double)l[4 * k + 2] : 0.0;
+    r->lat_max[k] = (double)l[4 * k + 3];
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return HA_OK;

@@ -11,6 +11,10 @@ one-for-one; the kernels are the library's (``ha_run_test``):
     python -m halloc_amd.harness throughput -s16 -S3072 -dexpequal -n262144 -l4
     python -m halloc_amd.harness priv-throughput -n1048576 -T256 -s16 -l1
     python -m halloc_amd.harness phase-throughput -f0.95 -F0.05 -e0.91 -s16 -S256 -l1 -i1 -t10
+    python -m halloc_amd.harness latency -n65536 -s16 -l1 -t4
+    python -m halloc_amd.harness phase-latency -f0.95 -F0.05 -e0.91 -s16 -S256 -l1 -i4 -t4
+    python -m halloc_amd.harness phase-alloc-write -f0.95 -F0.05 -e0.91 -s16 -S256 -l1 -i1 -t10
+    python -m halloc_amd.harness phase-extfrag -f0.95 -F0.05 -e0.91 -s16 -S256 -l1 -i1 -t10
     python -m halloc_amd.harness freeslabs
     python -m halloc_amd.harness run-all [--quick]
     python -m halloc_amd.harness grid-points      (samples/grid-points)
@@ -34,7 +38,10 @@ import math
 import sys
 
 DISTRS = {"uniform": 1, "expuniform": 2, "expequal": 3}
-TESTS = {"checkptr": 0, "prob-checkptr": 1, "throughput": 2, "priv-throughput": 3, "phase-throughput": 4}
+TESTS = {"checkptr": 0, "prob-checkptr": 1, "throughput": 2, "priv-throughput": 3, "phase-throughput": 4,
+         "latency": 5, "phase-latency": 6, "phase-alloc-write": 7, "phase-extfrag": 8}
+_WARMED = ("throughput", "priv-throughput", "phase-throughput", "latency", "phase-latency", "phase-alloc-write",
+           "phase-extfrag")
 NBYTES_IN_GIB = 1024.0 ** 3
 
 
@@ -233,8 +240,10 @@ def _kernel(test: str, o: CommonOpts, seed: int = 0) -> dict:
 def run_test(test: str, o: CommonOpts, out=print) -> bool:
     """One reference test on a fresh heap: the test body of
     tst/corr/checkptr, tst/corr/prob-checkptr, tst/corr/freeslabs,
-    tst/perf/throughput or tst/perf/priv-throughput.  Returns whether it
-    passed (the reference exits with -1 where this returns False)."""
+    tst/perf/throughput, tst/perf/priv-throughput, tst/perf/phase-throughput,
+    tst/perf/latency, tst/perf/phase-latency, tst/perf/phase-alloc-write or
+    tst/perf/phase-extfrag.  Returns whether it passed (the reference exits
+    with -1 where this returns False)."""
     import halloc_amd as H
     if o.nthreads == 0:
         return True
@@ -242,13 +251,13 @@ def run_test(test: str, o: CommonOpts, out=print) -> bool:
     try:
         if test == "freeslabs":
             return _freeslabs(o, out)
-        if test in ("throughput", "priv-throughput", "phase-throughput"):
+        if test in _WARMED:
             # the reference's warm-up: min(4 * bs, n) threads, one try
             w = dataclasses.replace(o, nthreads=min(4 * o.bs, o.nthreads), ntries=1)
             if test != "priv-throughput":
                 w.niters = 1
             _kernel(test, w)
-        if test in ("checkptr", "throughput"):
+        if test in ("checkptr", "throughput", "latency"):
             o = dataclasses.replace(o, niters=1)
         r = _kernel(test, o)
     finally:
@@ -259,6 +268,26 @@ def run_test(test: str, o: CommonOpts, out=print) -> bool:
             out(f"cannot allocate enough memory or pointers overlap: {r['checks_failed']} of "
                 f"{r['checked'] + r['checks_failed']} pointers, first at try {r['failed_try']}")
         return ok
+    if test == "phase-extfrag":
+        # tst/perf/phase-extfrag/phase-extfrag.cu:84-93
+        if r["checks_failed"]:
+            out("cannot allocate enough memory")
+            return False
+        for t, (ef, efi) in enumerate(r["extfrag"]):
+            out(f"external fragmentation {t} {ef:.2f} {efi:.2f}")
+        return True
+    if test in ("latency", "phase-latency"):
+        # tst/perf/latency/latency.cu:117-128, tst/perf/phase-latency/phase-latency.cu:172-183
+        if r["checks_failed"]:
+            out("cannot allocate enough memory")
+            return False
+        names = ("malloc", "free")
+        for k in range(2):
+            out(f"min {names[k]} latency {r['lat_min'][k]:.2f} cycles")
+            out(f"avg {names[k]} latency {r['lat_avg'][k]:.2f} cycles")
+            out(f"max {names[k]} latency {r['lat_max'][k]:.2f} cycles")
+        out(f"avg pair latency {r['lat_avg'][0] + r['lat_avg'][1]:.2f} cycles")
+        return True
     n = o.total_nallocs()
     tm, tf = r["malloc_ms"] / 1e3, r["free_ms"] / 1e3
     if test == "throughput":
@@ -268,8 +297,8 @@ def run_test(test: str, o: CommonOpts, out=print) -> bool:
         out(f"malloc speed {o.total_sz() / tm / NBYTES_IN_GIB:.2f} GiB/s")
         out(f"pair speed {o.total_sz() / (tm + tf) / NBYTES_IN_GIB:.2f} GiB/s")
         return r["checks_failed"] == 0
-    if test == "phase-throughput":
-        # tst/perf/phase-throughput/phase-throughput.cu:110-135
+    if test in ("phase-throughput", "phase-alloc-write"):
+        # tst/perf/phase-throughput/phase-throughput.cu:110-135, phase-alloc-write.cu:116-144
         npairs = 0.5 * n * o.exec_fraction
         nm = 0.25 * n * (o.exec_fraction + o.alloc_fraction - o.free_fraction)
         tp = tm + tf

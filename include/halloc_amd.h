@@ -230,7 +230,17 @@ int ha_run_serve(const ha_serve_io *io, ha_serve_result *res);
      test 4 phase-throughput (tst/perf/phase-throughput): the chain of test 1
        without stamps, timed per try (even tries as malloc time, odd as free
        time) with a NULL check of the blocks held; freeing the rest counts as
-       free time.
+       free time;
+     test 5 latency (tst/perf/latency): every malloc, then every free, timed
+       in shader-clock cycles, with a NULL check between;
+     test 6 phase-latency (tst/perf/phase-latency): the chain of test 1 with
+       every malloc and free timed;
+     test 7 phase-alloc-write (tst/perf/phase-alloc-write): test 4 with 123
+       written into every whole 8 B word of each block allocated;
+     test 8 phase-extfrag (tst/perf/phase-extfrag): test 4 with
+       extfrag[2 t] = ha_extfrag(false), extfrag[2 t + 1] = ha_extfrag(true)
+       after try t (extfrag: ntries * 2 doubles, or NULL).
+   lat_n / lat_avg / lat_min / lat_max [0] mallocs, [1] frees (cycles).
    Active threads: tid < nthreads and not (tid >> group_sh) & period_mask.
    Sizes: alloc_sz when max_alloc_sz == alloc_sz, else distr 1 uniform,
    2 exponential-uniform, 3 exponential-equal over [alloc_sz, max_alloc_sz]
@@ -242,11 +252,14 @@ typedef struct ha_test_opts {
   int32_t test, nthreads, bs, ntries, nallocs, niters;
   uint32_t alloc_sz, max_alloc_sz, max_alloc_sh, distr, period_mask, group_sh, seed, pad;
   float probabs[2][2];
+  double *extfrag;
 } ha_test_opts;
 typedef struct ha_test_result {
   uint64_t mallocs, frees, nulls, checked, checks_failed;
   double malloc_ms, free_ms;
   int32_t tries_run, failed_try;
+  uint64_t lat_n[2];
+  double lat_avg[2], lat_min[2], lat_max[2];
 } ha_test_result;
 int ha_run_test(const ha_test_opts *o, ha_test_result *r);
 

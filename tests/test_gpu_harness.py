@@ -28,6 +28,7 @@ def test_run_all_sweep_quick():
     ("throughput", ["-n65536", "-s16", "-S3072", "-dexpequal", "-l4", "-t2", "-m268435456"]),
     ("priv-throughput", ["-n262144", "-T256", "-s16", "-l1", "-i4", "-t2", "-m268435456"]),
     ("phase-throughput", ["-n262144", "-f0.95", "-F0.05", "-e0.91", "-s16", "-S256", "-l1", "-t10"]),
+    ("phase-alloc-write", ["-n262144", "-f0.95", "-F0.05", "-e0.91", "-s16", "-S256", "-l1", "-t10"]),
 ])
 def test_perf_tests(test, argv):
     say(f"harness {test}")
@@ -37,6 +38,55 @@ def test_perf_tests(test, argv):
     assert harness.run_test(test, o, out=lines.append)
     say("\n".join(lines))
     assert any("pair throughput" in l for l in lines)
+
+
+@pytest.mark.parametrize("test,argv", [
+    ("latency", ["-n65536", "-s16", "-S256", "-duniform", "-l2", "-t3", "-m268435456"]),
+    ("phase-latency", ["-n65536", "-f0.95", "-F0.05", "-e0.91", "-s16", "-S256", "-l1", "-i4", "-t4"]),
+])
+def test_latency_tests(test, argv):
+    # tst/perf/latency and tst/perf/phase-latency: the reference's output
+    # lines, and every timed operation counted once
+    say(f"harness {test}")
+    o = harness.defaults_for(test)
+    o.parse(argv)
+    lines = []
+    assert harness.run_test(test, o, out=lines.append)
+    say("\n".join(lines))
+    assert any(l.startswith("avg pair latency") for l in lines) and len(lines) == 7
+    H.Halloc.init(o.halloc_opts())
+    try:
+        r = H.Halloc.run_test(harness.TESTS[test], o.nthreads, o.bs, o.ntries, o.nallocs,
+                              1 if test == "latency" else o.niters, o.alloc_sz, o.max_alloc_sz,
+                              o.max_alloc_sh, harness.DISTRS[o.distr], o.period_mask, o.group_sh,
+                              o.probabs, seed=5)
+        assert H.Halloc.free_bytes() == H.Halloc.geometry()["arena_bytes"]
+    finally:
+        H.Halloc.shutdown()
+    assert r["nulls"] == 0 and r["checks_failed"] == 0, r
+    # every malloc timed; in the chain every free of a block held is timed
+    assert r["lat_n"][0] == r["mallocs"] > 0
+    if test == "latency":
+        assert r["lat_n"][0] == r["lat_n"][1] == r["frees"] == o.total_nallocs()
+    else:
+        assert 0 < r["lat_n"][1] <= r["frees"]
+    for k in range(2):
+        assert 0 < r["lat_min"][k] <= r["lat_avg"][k] <= r["lat_max"][k] < 1e9, r
+
+
+def test_phase_extfrag():
+    # tst/perf/phase-extfrag: one line per try; the ideal measure never
+    # exceeds the plain one, and both stay in [0, 1]
+    say("harness phase-extfrag")
+    o = harness.defaults_for("phase-extfrag")
+    o.parse(["-n262144", "-f0.95", "-F0.05", "-e0.91", "-s16", "-S256", "-l1", "-t6"])
+    lines = []
+    assert harness.run_test("phase-extfrag", o, out=lines.append)
+    say("\n".join(lines))
+    assert [l.split()[2] for l in lines] == [str(t) for t in range(6)]
+    for l in lines:
+        ef, efi = float(l.split()[3]), float(l.split()[4])
+        assert 0.0 <= efi <= ef + 0.005 and ef <= 1.0, l
 
 
 def test_exhaustion_is_reported():
