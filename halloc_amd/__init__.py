@@ -370,10 +370,11 @@ class Halloc:
     @staticmethod
     def run_sample(sample, n, bs=128, ncells=32, min_len=31, max_len=31, seed=0):
         """A reference sample as an application check: "grid-points",
-        "add-strings" or "random-graph" (max_len: the largest degree); see
+        "add-strings", "random-graph" (max_len: the largest degree) or "objects"
+        (typed new (halloc) / hadelete; max_len: the largest array in words); see
         include/halloc_amd.h ha_run_sample."""
-        a = _SampleArgs({"grid-points": 0, "add-strings": 1, "random-graph": 2}[sample], n, bs, ncells, min_len,
-                        max_len, seed, 0)
+        kind = {"grid-points": 0, "add-strings": 1, "random-graph": 2, "objects": 3}[sample]
+        a = _SampleArgs(kind, n, bs, ncells, min_len, max_len, seed, 0)
         r = _SampleResult()
         _check(lib().ha_run_sample(ctypes.byref(a), ctypes.byref(r)))
         return {k: getattr(r, k) for k, _ in _SampleResult._fields_}

@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#define HALLOC_CPP  // the typed forms, for the objects sample
 #include "../../include/halloc.h"
 #include "../../include/halloc_amd.h"
 #include "ha_internal.h"
@@ -1470,6 +1471,55 @@ __global__ void str_free_k(char** s, u32 n) {
   s[i] = nullptr;
 }
 
+// Typed objects through new (halloc) / hadelete: an object whose constructor
+// and destructor count themselves, and an array of words, per thread.
+struct CountedObj {
+  u32 id;
+  unsigned long long key;
+  unsigned long long* census;  // [0] constructed, [1] destroyed
+  __device__ CountedObj(u32 i, u32 seed, unsigned long long* c) : id(i), key(0), census(c) {
+    key = ((unsigned long long)hash3(seed, i, 7) << 32) | hash3(seed, i, 8);
+    atomicAdd(&census[0], 1ull);
+  }
+  __device__ ~CountedObj() { atomicAdd(&census[1], 1ull); }
+};
+__device__ __forceinline__ u32 obj_words(u32 seed, u32 i, u32 hi) { return 1 + hash3(seed, i, 9) % hi; }
+__global__ void obj_new_k(CountedObj** objs, u32** arrs, u32 n, u32 hi, u32 seed, unsigned long long* st) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  CountedObj* o = new (halloc) CountedObj(i, seed, st + 2);
+  const u32 k = obj_words(seed, i, hi);
+  u32* w = new (halloc) u32[k];
+  objs[i] = o;
+  arrs[i] = w;
+  if (!o) atomicAdd(&st[0], 1ull);
+  if (!w) {
+    atomicAdd(&st[0], 1ull);
+    return;
+  }
+  for (u32 j = 0; j < k; ++j) w[j] = i * 131u + j;
+}
+__global__ void obj_check_delete_k(CountedObj** objs, u32** arrs, u32 n, u32 hi, u32 seed, unsigned long long* st) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  CountedObj* o = objs[i];
+  u32* w = arrs[i];
+  const u32 k = obj_words(seed, i, hi);
+  u32 bad = 0;
+  if (o) {
+    bad += o->id != i || o->key != (((unsigned long long)hash3(seed, i, 7) << 32) | hash3(seed, i, 8)) ||
+           o->census != st + 2 || ((uintptr_t)o & 7);
+  }
+  if (w)
+    for (u32 j = 0; j < k; ++j) bad += w[j] != i * 131u + j;
+  if (o && w && (uintptr_t)o < (uintptr_t)(w + k) && (uintptr_t)w < (uintptr_t)(o + 1)) ++bad;
+  if (bad) atomicAdd(&st[1], 1ull);
+  hadelete(o);
+  hadelete_array(w);
+  objs[i] = nullptr;
+  arrs[i] = nullptr;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1839,10 +1889,10 @@ int ha_run_test(const ha_test_opts* a, ha_test_result* r) {
 
 int ha_run_sample(const ha_sample_args* a, ha_sample_result* r) {
   if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
-  if (!a || !r || a->sample < 0 || a->sample > 2 || a->n < 1 || a->n > (1 << 26) || a->bs < 64 || a->bs > 1024 ||
+  if (!a || !r || a->sample < 0 || a->sample > 3 || a->n < 1 || a->n > (1 << 26) || a->bs < 64 || a->bs > 1024 ||
       (a->sample == 0 && (a->ncells < 1 || a->ncells > 512)) ||
       (a->sample == 1 && (a->min_len < 0 || a->max_len < a->min_len || a->max_len > 4096)) ||
-      (a->sample == 2 && (a->max_len < 1 || a->max_len > 64)))
+      (a->sample == 2 && (a->max_len < 1 || a->max_len > 64)) || (a->sample == 3 && (a->max_len < 1 || a->max_len > 768)))
     return hfail(HA_E_INVALID, "bad sample arguments");
   const u32 n = (u32)a->n;
   Stream s;
@@ -1871,6 +1921,20 @@ int ha_run_sample(const ha_sample_args* a, ha_sample_result* r) {
     HH(hipEventRecord(e[2], s.s));
     HH(hipEventRecord(e[3], s.s));
     grid_free_k<<<cgrid, a->bs, 0, s.s>>>((PointNode**)dg.p, nc, n);
+    HH(hipEventRecord(e[4], s.s));
+    HH(hipGetLastError());
+    HH(hipStreamSynchronize(s.s));
+  } else if (a->sample == 3) {  // objects: max_len is the largest array in words
+    DBuf dobj, darr;
+    HH(dobj.alloc((size_t)n * 8));
+    HH(darr.alloc((size_t)n * 8));
+    HH(hipEventRecord(e[0], s.s));
+    obj_new_k<<<grid, a->bs, 0, s.s>>>((CountedObj**)dobj.p, (u32**)darr.p, n, (u32)a->max_len, a->seed, st);
+    HH(hipEventRecord(e[1], s.s));
+    HH(hipEventRecord(e[2], s.s));
+    HH(hipEventRecord(e[3], s.s));
+    obj_check_delete_k<<<grid, a->bs, 0, s.s>>>((CountedObj**)dobj.p, (u32**)darr.p, n, (u32)a->max_len, a->seed,
+                                                st);
     HH(hipEventRecord(e[4], s.s));
     HH(hipGetLastError());
     HH(hipStreamSynchronize(s.s));
@@ -1915,6 +1979,8 @@ int ha_run_sample(const ha_sample_args* a, ha_sample_result* r) {
   HH(hipMemcpy(h, dst.p, sizeof h, hipMemcpyDeviceToHost));
   r->nulls = h[0];
   r->errors = h[1] + (a->sample == 0 && h[2] + h[0] != n ? 1 : 0);  // grid: every point listed once
+  // objects: every object constructed was destroyed once
+  if (a->sample == 3 && h[2] != h[3]) r->errors += 1;
   r->alloc_ms = t[0];
   r->work_ms = t[1];
   r->free_ms = t[2];

@@ -41,6 +41,43 @@ struct halloc_opts_t {
 __device__ void *hamalloc(size_t nbytes);
 /** frees memory from hamalloc; NULL is ignored (replaces src/halloc.h:41) */
 __device__ void hafree(void *p);
+
+#if defined(HALLOC_CPP)
+/* Typed device allocation, the counterpart of the reference's HALLOC_CPP
+   operator new / delete (src/halloc.h:67-80, src/halloc.cu:201-208,258-265).
+   HIP already defines the plain device operator new / delete (over the
+   device heap's malloc / free), so the halloc forms take a tag, as the
+   reference's commented-out halloc_tag_t sketches:
+     T *p = new (halloc) T(args);    a hamalloc block; when the heap is
+                                     exhausted, NULL and no constructor runs
+     U *a = new (halloc) U[n];       U trivially destructible
+     hadelete(p);  hadelete_array(a);   destructor, then hafree; NULL ignored
+   p must point at the object allocated (its most-derived type); blocks are
+   8-byte aligned at least (16 for classes that are not a multiple of 24). */
+#include <new>
+#include <type_traits>
+struct halloc_tag_t {};
+inline constexpr halloc_tag_t halloc{};
+__device__ inline void *operator new(size_t nbytes, halloc_tag_t) noexcept { return hamalloc(nbytes); }
+__device__ inline void *operator new[](size_t nbytes, halloc_tag_t) noexcept { return hamalloc(nbytes); }
+/* called only when a constructor throws, which device code does not */
+__device__ inline void operator delete(void *p, halloc_tag_t) noexcept { hafree(p); }
+__device__ inline void operator delete[](void *p, halloc_tag_t) noexcept { hafree(p); }
+template <class T>
+__device__ inline void hadelete(T *p) {
+  if (p) {
+    p->~T();
+    hafree((void *)p);
+  }
+}
+/* an array of a trivially destructible type carries no element count ahead
+   of its first element, so the block starts at a */
+template <class T>
+__device__ inline void hadelete_array(T *a) {
+  static_assert(std::is_trivially_destructible<T>::value, "hadelete_array: trivially destructible types only");
+  hafree((void *)a);
+}
+#endif /* HALLOC_CPP */
 #endif
 
 /** initializes the allocator on the current device (replaces src/halloc.h:46).

@@ -263,7 +263,7 @@ typedef struct ha_test_result {
 } ha_test_result;
 int ha_run_test(const ha_test_opts *o, ha_test_result *r);
 
-/* The reference samples (samples/*) as application checks.
+/* The reference samples (the samples directory) as application checks.
      sample 0 grid-points: n points uniform in [0,1)^3 are sorted into an
        ncells^3 grid, a 16 B list node from hamalloc per point pushed onto its
        cell's list; the lists are checked (every point once, in its own cell)
@@ -273,7 +273,13 @@ int ha_run_test(const ha_test_opts *o, ha_test_result *r);
        a third, checked, and all three freed;
      sample 2 random-graph: n vertices with U{1..max_len} edges each, every
        edge a 16 B list node from hamalloc; the lists are checked (length,
-       targets) and freed.
+       targets) and freed;
+     sample 3 objects (the typed forms of halloc.h under HALLOC_CPP, the
+       counterpart of src/halloc.h:67-80): per thread one object by
+       new (halloc) with a counting constructor and destructor and one array
+       of U{1..max_len} words by new (halloc) u32[k], checked, and released
+       by hadelete / hadelete_array; an error if constructions and
+       destructions differ.
    alloc_ms / work_ms / free_ms: inserting or allocating / the grid check or
    the concatenation / freeing; errors: points or strings that fail the
    check. */
