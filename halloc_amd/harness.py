@@ -17,6 +17,7 @@ one-for-one; the kernels are the library's (``ha_run_test``):
     python -m halloc_amd.harness phase-extfrag -f0.95 -F0.05 -e0.91 -s16 -S256 -l1 -i1 -t10
     python -m halloc_amd.harness freeslabs
     python -m halloc_amd.harness run-all [--quick]
+    python -m halloc_amd.harness throughput-all [--quick]   (tst/perf/throughput/throughput-all.sh)
     python -m halloc_amd.harness grid-points      (samples/grid-points)
     python -m halloc_amd.harness add-strings      (samples/add-strings)
     python -m halloc_amd.harness random-graph     (samples/random-graph)
@@ -377,6 +378,31 @@ def defaults_for(test: str) -> CommonOpts:
     return o
 
 
+# tst/perf/throughput/throughput-all.sh:4-19: (size, allocations per thread, threads or None for the default)
+THROUGHPUT_ALL = ((16, 4, None), (24, 4, None), (32, 4, None), (48, 4, None), (64, 4, None), (96, 2, None),
+                  (128, 2, None), (192, 1, None), (256, 1, None), (384, 1, 512 * 1024), (512, 1, 512 * 1024),
+                  (768, 1, 256 * 1024), (1024, 1, 256 * 1024), (1536, 1, 128 * 1024), (2048, 1, 128 * 1024),
+                  (3072, 1, 64 * 1024))
+
+
+def throughput_all(quick: bool = False, out=print) -> tuple[int, int]:
+    """The throughput test at every size class, as throughput-all.sh runs it
+    (quick: two tries instead of eight).  Prints one line per size and
+    returns (runs, runs without NULLs)."""
+    nruns = nok = 0
+    for sz, nallocs, nthreads in THROUGHPUT_ALL:
+        argv = [f"-s{sz}", f"-l{nallocs}"] + ([f"-n{nthreads}"] if nthreads else [])
+        o = defaults_for("throughput")
+        o.parse(argv + (["-t2"] if quick else []))
+        lines = []
+        ok = run_test("throughput", o, out=lines.append)
+        pair = next((l for l in lines if l.startswith("pair throughput")), "no result")
+        out(f"{'ok  ' if ok else 'FAIL'} throughput {' '.join(argv)}: {pair}")
+        nruns += 1
+        nok += ok
+    return nruns, nok
+
+
 def run_all(quick: bool = False, memory: int = 512 * 1024 * 1024, out=print) -> tuple[int, int]:
     ntests = nok = 0
     for test, argv in run_all_sweep(memory=memory, quick=quick):
@@ -440,6 +466,9 @@ def main(argv=None) -> int:
     test, args = argv[0], argv[1:]
     if test == "run-all":
         n, ok = run_all(quick="--quick" in args)
+        return 0 if ok == n else 1
+    if test == "throughput-all":
+        n, ok = throughput_all(quick="--quick" in args)
         return 0 if ok == n else 1
     if test in ("grid-points", "add-strings", "random-graph"):
         return 0 if run_sample(test) else 255

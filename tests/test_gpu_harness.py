@@ -24,6 +24,16 @@ def test_run_all_sweep_quick():
     assert n == len(harness.run_all_sweep(quick=True)) and ok == n
 
 
+def test_throughput_all_quick():
+    # tst/perf/throughput/throughput-all.sh with two tries per size: every
+    # size class from 16 to 3072 B at the script's thread counts, no NULLs
+    say("harness throughput-all quick")
+    lines = []
+    n, ok = harness.throughput_all(quick=True, out=lines.append)
+    say("\n".join(lines))
+    assert n == len(harness.THROUGHPUT_ALL) == 16 and ok == n
+
+
 @pytest.mark.parametrize("test,argv", [
     ("throughput", ["-n65536", "-s16", "-S3072", "-dexpequal", "-l4", "-t2", "-m268435456"]),
     ("priv-throughput", ["-n262144", "-T256", "-s16", "-l1", "-i4", "-t2", "-m268435456"]),

@@ -6,6 +6,7 @@ import math
 import numpy as np
 import pytest
 
+from halloc_amd import harness
 from halloc_amd.harness import CommonOpts, OptionError, run_all_sweep
 
 
@@ -89,3 +90,13 @@ def test_run_all_sweep_list():
     prob = [a for t, a in full if t == "prob-checkptr"]
     assert len(prob) == 3 * 2 * 5
     assert {a[1] for a in prob} == {"-t16384", "-t3277"}
+
+
+def test_throughput_all_list():
+    # throughput-all.sh: the 16 size classes in order, 4 / 2 / 1 allocations
+    # per thread, threads halved every two classes above 256 B
+    sizes = [sz for sz, _, _ in harness.THROUGHPUT_ALL]
+    assert sizes == [16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072]
+    for sz, nallocs, n in harness.THROUGHPUT_ALL:
+        assert nallocs == (4 if sz <= 64 else 2 if sz <= 128 else 1)
+        assert n is None if sz <= 256 else n == (1 << 20) >> (1 + (sizes.index(sz) - 9) // 2)
