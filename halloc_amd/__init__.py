@@ -94,7 +94,8 @@ class _TestResult(ctypes.Structure):
                 ("checked", ctypes.c_uint64), ("checks_failed", ctypes.c_uint64), ("malloc_ms", ctypes.c_double),
                 ("free_ms", ctypes.c_double), ("tries_run", ctypes.c_int32), ("failed_try", ctypes.c_int32),
                 ("lat_n", ctypes.c_uint64 * 2), ("lat_avg", ctypes.c_double * 2), ("lat_min", ctypes.c_double * 2),
-                ("lat_max", ctypes.c_double * 2)]
+                ("lat_max", ctypes.c_double * 2), ("malloc_wall_ms", ctypes.c_double),
+                ("free_wall_ms", ctypes.c_double)]
 
 
 class _SampleArgs(ctypes.Structure):

@@ -18,6 +18,7 @@ one-for-one; the kernels are the library's (``ha_run_test``):
     python -m halloc_amd.harness freeslabs
     python -m halloc_amd.harness run-all [--quick]
     python -m halloc_amd.harness throughput-all [--quick]   (tst/perf/throughput/throughput-all.sh)
+    python -m halloc_amd.harness exp scaling|speed|all [--quick] [--out DIR]   (tst/exp/scaling, tst/exp/speed)
     python -m halloc_amd.harness grid-points      (samples/grid-points)
     python -m halloc_amd.harness add-strings      (samples/add-strings)
     python -m halloc_amd.harness random-graph     (samples/random-graph)
@@ -238,13 +239,15 @@ def _kernel(test: str, o: CommonOpts, seed: int = 0) -> dict:
                              o.probabs, seed)
 
 
-def run_test(test: str, o: CommonOpts, out=print) -> bool:
+def run_test(test: str, o: CommonOpts, out=print, clock: str = "wall") -> bool:
     """One reference test on a fresh heap: the test body of
     tst/corr/checkptr, tst/corr/prob-checkptr, tst/corr/freeslabs,
     tst/perf/throughput, tst/perf/priv-throughput, tst/perf/phase-throughput,
     tst/perf/latency, tst/perf/phase-latency, tst/perf/phase-alloc-write or
     tst/perf/phase-extfrag.  Returns whether it passed (the reference exits
-    with -1 where this returns False)."""
+    with -1 where this returns False).  clock "wall" times launch plus
+    synchronization on the host, as the reference does (omp_get_wtime);
+    "kernel" uses the kernels' own HIP-event time."""
     import halloc_amd as H
     if o.nthreads == 0:
         return True
@@ -290,7 +293,10 @@ def run_test(test: str, o: CommonOpts, out=print) -> bool:
         out(f"avg pair latency {r['lat_avg'][0] + r['lat_avg'][1]:.2f} cycles")
         return True
     n = o.total_nallocs()
-    tm, tf = r["malloc_ms"] / 1e3, r["free_ms"] / 1e3
+    if clock == "wall":
+        tm, tf = r["malloc_wall_ms"] / 1e3, r["free_wall_ms"] / 1e3
+    else:
+        tm, tf = r["malloc_ms"] / 1e3, r["free_ms"] / 1e3
     if test == "throughput":
         out(f"malloc throughput {n / tm * 1e-6:.2f} Mmallocs/s")
         out(f"free throughput {n / tf * 1e-6:.2f} Mfrees/s")
@@ -403,6 +409,115 @@ def throughput_all(quick: bool = False, out=print) -> tuple[int, int]:
     return nruns, nok
 
 
+# the flags every experiment of tst/exp/scaling/exp-run.pl and
+# tst/exp/speed/exp-run.pl shares
+EXP_COMMON = ["-f0.95", "-F0.05", "-e0.91", f"-m{512 * 1024 * 1024}", "-g5"]
+EXP_FILES = {"thru": "exp-log-thru.csv", "lat": "exp-log-lat.csv", "single": "exp-log-single.csv",
+             "combi": "exp-log-combi.csv"}
+
+
+def _exp_fields(lines) -> dict:
+    """The figures tst/exp/common.pl:runtest reads from a test's output lines
+    (-1 where a line is absent)."""
+    v = {}
+    for line in lines:
+        w = line.split()
+        if "throughput" in w:
+            v["thru_" + ("malloc" if "malloc" in w else "free" if "free" in w else "pair")] = float(w[2])
+        elif "speed" in w:
+            v["speed_" + ("malloc" if "malloc" in w else "pair")] = float(w[2])
+        elif "latency" in w and w[0] in ("min", "avg", "max") and w[1] in ("malloc", "free"):
+            v[f"lat_{w[1]}_{w[0]}"] = float(w[3])
+    return v
+
+
+def _exp_run(test: str, args: list[str]):
+    """runtest of tst/exp/common.pl: phase-<test> with the shared flags;
+    returns a lookup of the extracted figures."""
+    test = "phase-" + test
+    o = defaults_for(test)
+    o.parse(EXP_COMMON + args)
+    lines = []
+    if not run_test(test, o, out=lines.append):
+        raise RuntimeError(f"{test} {' '.join(args)} failed: {'; '.join(lines)}")
+    v = _exp_fields(lines)
+    return lambda k: v.get(k, -1.0)
+
+
+def exp_scaling(quick: bool = False, out=print) -> dict:
+    """tst/exp/scaling/exp-run.pl: pair throughput per thread count, spree
+    (one iteration per launch, 512 launches) and priv (32 iterations per
+    launch, 16 launches), then latency over 8 launches of 16 iterations.
+    quick: four thread counts and a sixteenth of the launches."""
+    total = 32 if quick else 512
+    thru = [["alloc_sz", "nallocs", "nthreads", "priv_pair", "spree_pair", "spree_malloc", "spree_free"]]
+    nts = [32, 1024, 32768, 1 << 20] if quick else [32 << k for k in range(16)]
+    for n in nts:
+        for sz in (16, 256):
+            for nallocs in (1, 4):
+                if nallocs == 4 and sz > 64:
+                    continue
+                args = [f"-n{n}", f"-l{nallocs}", f"-s{sz}"]
+                spree = _exp_run("throughput", args + ["-i1", f"-t{total}"])
+                priv = _exp_run("throughput", args + ["-i32", f"-t{total // 32}"])
+                thru.append([sz, nallocs, n, priv("thru_pair"), spree("thru_pair"), spree("thru_malloc"),
+                             spree("thru_free")])
+                out(" ".join(str(x) for x in thru[-1]))
+    lat = [["alloc_sz", "nallocs", "nthreads", "malloc_min", "malloc_avg", "malloc_max", "free_min", "free_avg",
+            "free_max"]]
+    total = 16 if quick else 128
+    for n in ([1, 1024, 1 << 20] if quick else [1 << k for k in range(21)]):
+        for sz in (16, 256):
+            g = _exp_run("latency", [f"-n{n}", "-l1", f"-s{sz}", "-i16", f"-t{total // 16}"])
+            lat.append([sz, 1, n] + [g(f"lat_{a}_{b}") for a in ("malloc", "free") for b in ("min", "avg", "max")])
+            out(" ".join(str(x) for x in lat[-1]))
+    return {"thru": thru, "lat": lat}
+
+
+def exp_speed(quick: bool = False, out=print) -> dict:
+    """tst/exp/speed/exp-run.pl: every size class alone, then the mixed
+    ranges 8..32, 8..64, 8..256 and 8..3072 B; threads sized to 40% of the
+    512 MiB pool (at most 2^20); spree and priv as in exp_scaling."""
+    total = 32 if quick else 512
+    cols = ["block", "priv_speed", "priv_thru", "spree_speed", "spree_speed_malloc", "speed_thru",
+            "speed_thru_malloc"]
+
+    def speed_row(label, args):
+        spree = _exp_run("throughput", args + ["-i1", f"-t{total}"])
+        priv = _exp_run("throughput", args + ["-i32", f"-t{total // 32}"])
+        row = [label, priv("speed_pair"), priv("thru_pair"), spree("speed_pair"), spree("speed_malloc"),
+               spree("thru_pair"), spree("thru_malloc")]
+        out(" ".join(str(x) for x in row))
+        return row
+
+    memory, max_n = 512 * 1024 * 1024, 1 << 20
+    single = [cols]
+    for sz in (16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072):
+        nallocs = 4 if sz <= 64 else 2 if sz <= 128 else 1
+        n = min(int(math.floor(0.4 * memory / (sz * nallocs))), max_n)
+        single.append(speed_row(sz, [f"-n{n}", f"-l{nallocs}", f"-s{sz}"]))
+    combi = [cols]
+    for lo, hi in ((8, 32), (8, 64), (8, 256), (8, 3072)):
+        nallocs = 4 if hi <= 64 else 1
+        n = min(int(math.floor(0.4 * memory / ((lo + hi) / 2 * nallocs))), max_n)
+        combi.append(speed_row(f"{lo}..{hi}", [f"-n{n}", f"-l{nallocs}", f"-s{lo}", f"-S{hi}"]))
+    return {"single": single, "combi": combi}
+
+
+def write_exp_csvs(tables: dict, outdir: str) -> list[str]:
+    """The experiment logs under the reference's file names, space separated."""
+    import os
+    os.makedirs(outdir, exist_ok=True)
+    paths = []
+    for key, rows in tables.items():
+        path = os.path.join(outdir, EXP_FILES[key])
+        with open(path, "w") as f:
+            for row in rows:
+                f.write(" ".join(f"{x:.2f}" if isinstance(x, float) else str(x) for x in row) + "\n")
+        paths.append(path)
+    return paths
+
+
 def run_all(quick: bool = False, memory: int = 512 * 1024 * 1024, out=print) -> tuple[int, int]:
     ntests = nok = 0
     for test, argv in run_all_sweep(memory=memory, quick=quick):
@@ -467,6 +582,19 @@ def main(argv=None) -> int:
     if test == "run-all":
         n, ok = run_all(quick="--quick" in args)
         return 0 if ok == n else 1
+    if test == "exp":
+        # exp scaling|speed|all [--quick] [--out DIR]
+        which = next((x for x in args if x in ("scaling", "speed", "all")), "all")
+        outdir = args[args.index("--out") + 1] if "--out" in args else "."
+        quick = "--quick" in args
+        tables = {}
+        if which in ("scaling", "all"):
+            tables.update(exp_scaling(quick))
+        if which in ("speed", "all"):
+            tables.update(exp_speed(quick))
+        for path in write_exp_csvs(tables, outdir):
+            print(f"wrote {path}")
+        return 0
     if test == "throughput-all":
         n, ok = throughput_all(quick="--quick" in args)
         return 0 if ok == n else 1

@@ -241,6 +241,9 @@ int ha_run_serve(const ha_serve_io *io, ha_serve_result *res);
        extfrag[2 t] = ha_extfrag(false), extfrag[2 t + 1] = ha_extfrag(true)
        after try t (extfrag: ntries * 2 doubles, or NULL).
    lat_n / lat_avg / lat_min / lat_max [0] mallocs, [1] frees (cycles).
+   malloc_ms / free_ms: kernel time (HIP events); malloc_wall_ms /
+   free_wall_ms: host wall-clock around launch and synchronization, as the
+   reference's tests time (omp_get_wtime).
    Active threads: tid < nthreads and not (tid >> group_sh) & period_mask.
    Sizes: alloc_sz when max_alloc_sz == alloc_sz, else distr 1 uniform,
    2 exponential-uniform, 3 exponential-equal over [alloc_sz, max_alloc_sz]
@@ -260,6 +263,7 @@ typedef struct ha_test_result {
   int32_t tries_run, failed_try;
   uint64_t lat_n[2];
   double lat_avg[2], lat_min[2], lat_max[2];
+  double malloc_wall_ms, free_wall_ms;
 } ha_test_result;
 int ha_run_test(const ha_test_opts *o, ha_test_result *r);
 

@@ -34,6 +34,22 @@ def test_throughput_all_quick():
     assert n == len(harness.THROUGHPUT_ALL) == 16 and ok == n
 
 
+def test_exp_replay_quick(tmp_path):
+    # tst/exp/scaling and tst/exp/speed at four thread counts and a sixteenth
+    # of the launches: every run passes its NULL check and reports a rate
+    say("harness exp quick")
+    t = harness.exp_scaling(quick=True, out=say)
+    t.update(harness.exp_speed(quick=True, out=say))
+    assert len(t["thru"]) == 1 + 4 * 3 and len(t["lat"]) == 1 + 3 * 2
+    assert len(t["single"]) == 17 and len(t["combi"]) == 5
+    for key in ("thru", "single", "combi"):
+        for row in t[key][1:]:
+            assert all(isinstance(x, float) and x > 0 for x in row[-4:]), (key, row)
+    for row in t["lat"][1:]:
+        assert 0 < row[3] <= row[4] <= row[5] and 0 < row[6] <= row[7] <= row[8], row
+    assert len(harness.write_exp_csvs(t, str(tmp_path))) == 4
+
+
 @pytest.mark.parametrize("test,argv", [
     ("throughput", ["-n65536", "-s16", "-S3072", "-dexpequal", "-l4", "-t2", "-m268435456"]),
     ("priv-throughput", ["-n262144", "-T256", "-s16", "-l1", "-i4", "-t2", "-m268435456"]),

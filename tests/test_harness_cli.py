@@ -100,3 +100,41 @@ def test_throughput_all_list():
     for sz, nallocs, n in harness.THROUGHPUT_ALL:
         assert nallocs == (4 if sz <= 64 else 2 if sz <= 128 else 1)
         assert n is None if sz <= 256 else n == (1 << 20) >> (1 + (sizes.index(sz) - 9) // 2)
+
+
+def test_exp_tables_follow_the_experiment_scripts(monkeypatch, tmp_path):
+    # tst/exp/scaling/exp-run.pl and tst/exp/speed/exp-run.pl: which runs, with
+    # which flags, into which rows (the device runs are replaced by a stub)
+    calls = []
+
+    def stub(test, args):
+        calls.append((test, args))
+        return lambda k: 1.5
+    monkeypatch.setattr(harness, "_exp_run", stub)
+    t = harness.exp_scaling(out=lambda m: None)
+    assert len(t["thru"]) == 1 + 16 * 3 and len(t["lat"]) == 1 + 21 * 2
+    assert t["thru"][1][:3] == [16, 1, 32] and t["thru"][-1][:3] == [256, 1, 1 << 20]
+    assert ("throughput", ["-n1048576", "-l4", "-s16", "-i32", "-t16"]) in calls
+    assert ("throughput", ["-n32", "-l1", "-s256", "-i1", "-t512"]) in calls
+    assert ("latency", ["-n1", "-l1", "-s16", "-i16", "-t8"]) in calls
+    calls.clear()
+    t.update(harness.exp_speed(out=lambda m: None))
+    assert [r[0] for r in t["single"][1:]][:3] == [16, 24, 32] and len(t["single"]) == 17
+    assert [r[0] for r in t["combi"][1:]] == ["8..32", "8..64", "8..256", "8..3072"]
+    # threads: 40% of 512 MiB over the bytes per thread, at most 2^20
+    assert ("throughput", ["-n69905", "-l1", "-s3072", "-i1", "-t512"]) in calls
+    assert ("throughput", ["-n1048576", "-l4", "-s8", "-S32", "-i1", "-t512"]) in calls
+    assert ("throughput", ["-n139446", "-l1", "-s8", "-S3072", "-i32", "-t16"]) in calls
+    paths = harness.write_exp_csvs(t, str(tmp_path))
+    assert sorted(p.rsplit("/", 1)[1] for p in paths) == sorted(harness.EXP_FILES.values())
+    lines = open(tmp_path / "exp-log-lat.csv").read().splitlines()
+    assert lines[0].startswith("alloc_sz nallocs nthreads malloc_min") and lines[1] == "16 1 1 " + " ".join(["1.50"] * 6)
+
+
+def test_exp_fields_match_common_pl():
+    v = harness._exp_fields(["malloc throughput 1.5 Mmallocs/s", "free throughput 2.5 Mfrees/s",
+                             "pair throughput 3.00 Mpairs/s", "malloc speed 4.00 GiB/s", "pair speed 5.00 GiB/s",
+                             "min malloc latency 6.00 cycles", "max free latency 7.00 cycles",
+                             "avg pair latency 9.00 cycles"])
+    assert v == {"thru_malloc": 1.5, "thru_free": 2.5, "thru_pair": 3.0, "speed_malloc": 4.0, "speed_pair": 5.0,
+                 "lat_malloc_min": 6.0, "lat_free_max": 7.0}
