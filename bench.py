@@ -165,8 +165,10 @@ def bench_sweep(H, memory=1 << 30, seed=0, rounds=8):
     import numpy as np
     from halloc_amd.workloads import sweep_sizes
     rng = np.random.default_rng(seed)
-    H.Halloc.init(H.HallocOpts(memory=memory))
+    # sizes over all 16 classes: head sets sized to the heap (4 at 1 GiB)
+    H.Halloc.init(H.HallocOpts(memory=memory, nheads=0))
     arena = H.Halloc.geometry()["arena_bytes"]
+    nheads = H.Halloc.geometry()["nheads"]
     nreq, ms, viol, nulls = 0, 0.0, 0, 0
     nbig, ms_big, nulls_big = 0, 0.0, 0
     for _ in range(rounds):
@@ -186,7 +188,7 @@ def bench_sweep(H, memory=1 << 30, seed=0, rounds=8):
         nulls_big += b["nulls"]
     out = {"workload": f"2^19 lanes in {rounds} rounds, U{{1..8}} requests log-uniform on [16 B, 3 KiB] + 1% "
                        f"on (3 KiB, 64 KiB] (device heap, timed apart), random-order frees, "
-                       f"{memory >> 30} GiB pool",
+                       f"{memory >> 30} GiB pool, {nheads} head sets (nheads 0)",
            "requests": nreq, "pairs_per_s": nreq / (ms / 1e3), "nulls": nulls,
            "large_requests": nbig, "large_path_pairs_per_s": nbig / (ms_big / 1e3), "large_path_nulls": nulls_big,
            "violations": viol + nulls, "per_class_pairs_per_s": {}}

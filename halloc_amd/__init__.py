@@ -139,7 +139,8 @@ class _ServeResult(ctypes.Structure):
 
 @dataclass
 class HallocOpts:
-    """Fields and defaults of halloc_opts_t (src/halloc.h:15-34), plus nheads,
+    """Fields and defaults of halloc_opts_t (src/halloc.h:15-34), plus nheads
+    (0: sized to the heap for live blocks of many size classes),
     hash_step_frac (0 keeps the reference's probe stride) and cached_arena
     (0: uncached arena, coherent across XCDs; 1: L2-cached arena with a
     release in every hafree)."""

@@ -1354,7 +1354,7 @@ static int init_locked(const ha_opts_c& o) {
   }
   if (o.sb_sz_sh < (int)MIN_SB_SZ_SH || o.sb_sz_sh > (int)MAX_SB_SZ_SH)
     return fail(HA_E_INVALID, "sb_sz_sh must be in 21..26");
-  if (o.nheads < 1 || o.nheads > (int)MAX_NHEADS) return fail(HA_E_INVALID, "nheads must be in 1..64");
+  if (o.nheads < 0 || o.nheads > (int)MAX_NHEADS) return fail(HA_E_INVALID, "nheads must be in 0..64");
   if (!(o.halloc_fraction > 0 && o.halloc_fraction <= 1) || !(o.sparse_fraction >= 0) ||
       !(o.sparse_fraction <= o.roomy_fraction) || !(o.roomy_fraction <= o.busy_fraction) ||
       !(o.busy_fraction <= 1))
@@ -1374,7 +1374,13 @@ static int init_locked(const ha_opts_c& o) {
   P.sb_sz = 1u << P.sb_sz_sh;
   P.nsbs = (u32)(halloc_memory / P.sb_sz);
   if (P.nsbs == 0 || P.nsbs > MAX_NSBS) return fail(HA_E_INVALID, "memory gives no slab or too many slabs");
+  // nheads 0: as many sets (a power of two, 1..16) as keep one head of every
+  // size class in every set within a third of the slabs
   P.nheads = (u32)o.nheads;
+  if (P.nheads == 0) {
+    P.nheads = 1;
+    while (2 * P.nheads <= 16 && 2 * P.nheads * NSIZES * 3 <= P.nsbs) P.nheads *= 2;
+  }
   P.nh_m = mod_magic(P.nheads);
   P.nh8_m = mod_magic(P.nheads / 8 ? P.nheads / 8 : 1);
   P.cached_arena = (u32)o.cached_arena;

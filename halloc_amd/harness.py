@@ -225,11 +225,30 @@ class CommonOpts:
     def total_sz(self) -> float:
         return self.expected_sz() * self.total_nallocs()
 
+    def size_classes(self) -> int:
+        """How many of the 16 size classes (16..3072 B, src/halloc.cu:367-393)
+        the drawn sizes can fall in; sizes above 3072 B use none."""
+        blocks = [16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072]
+        lo, hi = max(self.alloc_sz, 1), min(self.max_alloc_sz, 3072)
+        return sum(1 for i, b in enumerate(blocks) if b >= lo and (i == 0 or blocks[i - 1] < hi))
+
+    def nheads(self) -> int:
+        """Head sets: every (set, class) pair pins a partly filled head slab,
+        so the largest power of two up to 16 with 3 * sets * classes <= slabs
+        (the classes the sizes span; 16 sets for one class from 48 slabs)."""
+        nsbs = int(self.memory * self.halloc_fraction) >> self.sb_sz_sh
+        ncls = max(1, self.size_classes())
+        n = 1
+        while 2 * n <= 16 and 3 * 2 * n * ncls <= nsbs:
+            n *= 2
+        return n
+
     def halloc_opts(self):
+        """The allocator options, with the head-set count of nheads()."""
         import halloc_amd as H
         return H.HallocOpts(memory=self.memory, halloc_fraction=self.halloc_fraction,
                             busy_fraction=self.busy_fraction, roomy_fraction=self.roomy_fraction,
-                            sparse_fraction=self.sparse_fraction, sb_sz_sh=self.sb_sz_sh)
+                            sparse_fraction=self.sparse_fraction, sb_sz_sh=self.sb_sz_sh, nheads=self.nheads())
 
 
 def _kernel(test: str, o: CommonOpts, seed: int = 0) -> dict:

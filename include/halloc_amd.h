@@ -36,7 +36,12 @@ typedef struct ha_opts_c {
   int32_t sb_sz_sh;          /* default 22 (4 MiB slabs), 21..26 */
   int32_t nheads;            /* head slabs per size class (1..64), default 16: a wave uses its XCD's
                                 set (xcc_id % nheads); above 8 (a multiple of 8), one of its XCD's
-                                nheads / 8 sets, picked by the wave's CU */
+                                nheads / 8 sets, picked by the wave's CU.  0: the largest power of
+                                two up to 16 with nheads * 16 classes * 3 <= slabs (2 at 512 MiB,
+                                4 at 1 GiB, 16 from 4 GiB), for workloads whose live blocks span
+                                many size classes: each (set, class) pair pins a partly filled head
+                                slab, and with 16 sets and 16 live classes at 512 MiB the heads
+                                take the slabs the largest classes need, so requests fail early */
   double hash_step_frac;     /* 0 (default): the probe stride's prime is the largest below
                                 nblocks/256 + nblocks/64 (src/halloc.cu:378-381); else below
                                 hash_step_frac * nblocks, 0 < hash_step_frac < 1 */
