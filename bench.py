@@ -217,7 +217,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--nheads", type=int, default=16)
+    # head sets of the plain 16 B kernel: 32 (the plain form's tuned count; one
+    # size class on 96 slabs leaves room for them), against the library's 16
+    ap.add_argument("--nheads", type=int, default=32)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds for the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-probe", action="store_true")
@@ -267,7 +269,8 @@ def main():
         "dtype": "u64",
         "data": "synthetic",
         "config": {"workload": "plain 16 B: 2^20 lanes (256/block), hamalloc(16) + 16 B write/verify + "
-                               "hafree per lane per step; 512 MiB pool, default halloc_opts_t",
+                               f"hafree per lane per step; 512 MiB pool, default halloc_opts_t, "
+                               f"{args.nheads} head sets",
                    "pool_bytes": 512 << 20, "lanes": NTHREADS, "block": BS, "nheads": args.nheads},
         "kernel_ms_per_step": r["total_ms"] / args.steps,
         "launch_ms": [round(float(x), 4) for x in r["ms"]],

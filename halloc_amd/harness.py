@@ -234,12 +234,12 @@ class CommonOpts:
 
     def nheads(self) -> int:
         """Head sets: every (set, class) pair pins a partly filled head slab,
-        so the largest power of two up to 16 with 3 * sets * classes <= slabs
-        (the classes the sizes span; 16 sets for one class from 48 slabs)."""
+        so the largest power of two up to 32 with 3 * sets * classes <= slabs
+        (the classes the sizes span; 32 sets for one class from 96 slabs)."""
         nsbs = int(self.memory * self.halloc_fraction) >> self.sb_sz_sh
         ncls = max(1, self.size_classes())
         n = 1
-        while 2 * n <= 16 and 3 * 2 * n * ncls <= nsbs:
+        while 2 * n <= 32 and 3 * 2 * n * ncls <= nsbs:
             n *= 2
         return n
 

@@ -142,11 +142,12 @@ def test_exp_fields_match_common_pl():
 
 def test_head_sets_follow_the_classes_spanned():
     # 512 MiB with 4 MiB slabs: 96 slabs; 3 * sets * classes stays within them
-    assert parsed(["-s16"]).size_classes() == 1 and parsed(["-s16"]).nheads() == 16
+    assert parsed(["-s16"]).size_classes() == 1 and parsed(["-s16"]).nheads() == 32
     assert parsed(["-s8", "-S32"]).size_classes() == 3 and parsed(["-s8", "-S32"]).nheads() == 8
     assert parsed(["-s8", "-S64"]).size_classes() == 5 and parsed(["-s8", "-S64"]).nheads() == 4
     assert parsed(["-s8", "-S256"]).size_classes() == 9 and parsed(["-s8", "-S256"]).nheads() == 2
     assert parsed(["-s8", "-S3072"]).size_classes() == 16 and parsed(["-s8", "-S3072"]).nheads() == 2
-    assert parsed(["-s3072"]).nheads() == 16 and parsed(["-s100", "-S200"]).size_classes() == 3
+    assert parsed(["-s3072"]).nheads() == 32 and parsed(["-s100", "-S200"]).size_classes() == 3
     assert parsed(["-s8", "-S3072", f"-m{1 << 30}"]).nheads() == 4
     assert parsed(["-s8", "-S3072", f"-m{4 << 30}"]).nheads() == 16
+    assert parsed(["-s16", f"-m{256 << 20}"]).nheads() == 16  # 48 slabs
