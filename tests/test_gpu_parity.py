@@ -6,6 +6,8 @@ model's bit for bit.  Concurrent mode: full-grid workloads must satisfy the
 reference test suite's invariants (tst/common/common.cu:412-529,
 tst/corr/freeslabs/freeslabs.cu:35-71), checked on the dumped state.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -256,6 +258,28 @@ def test_size_sweep_lists_random_free_order(halloc):
     d = H.Halloc.dump()
     assert ((d[:g["nsbs"]] >> np.uint64(32)) == 0).all()
     assert M.check_dump(model_opts(opts), d)[0] == 0
+
+
+def test_mixed_classes_served_with_heap_sized_head_sets(halloc):
+    # 132 000 requests uniform on 8..3072 B (all 16 classes, about half the
+    # slab bytes of a 512 MiB heap): with the head sets sized to the heap
+    # (nheads 0 -> 2 sets) every request is served, blocks are disjoint, and
+    # freeing them returns the whole heap
+    say("mixed classes 8..3072 B at 512 MiB, nheads 0")
+    opts = halloc(H.HallocOpts(memory=512 << 20, nheads=0))
+    g = H.Halloc.geometry()
+    assert g["nsbs"] == 96 and g["nheads"] == 2
+    rng = np.random.default_rng(0)
+    sizes = rng.integers(8, 3073, 132000).astype(np.uint32)
+    ptrs = H.Halloc.malloc_sizes(sizes, bs=128)
+    assert (ptrs != 0).all(), int((ptrs == 0).sum())
+    o = np.argsort(ptrs)
+    p, s = ptrs[o], sizes[o].astype(np.uint64)
+    assert (p[:-1] + s[:-1] <= p[1:]).all()
+    H.Halloc.free_ptrs(ptrs)
+    assert H.Halloc.free_bytes() == g["arena_bytes"]
+    d = H.Halloc.dump()
+    assert M.check_dump(model_opts(dataclasses.replace(opts, nheads=g["nheads"])), d)[0] == 0
 
 
 def test_phase_workload_invariants(halloc):
