@@ -1283,7 +1283,8 @@ static u32 chunk_val(u32 chunk_sz) {
   for (; (1u << sh) < chunk_sz; sh++) {}
   return div3 << 16 | sh;
 }
-// size table (src/halloc.cu:363-394) with blocks per slab rounded to RUN
+// size table (src/halloc.cu:363-394) with blocks per slab rounded to RUN, and
+// to a run count THREAD_FREQ does not divide
 static bool build_size_infos(u32 sb_sz, const ha_opts_c& o, SizeInfo* out) {
   for (u32 sid = 0; sid < NSIZES; ++sid) {
     SizeInfo& s = out[sid];
@@ -1292,6 +1293,10 @@ static bool build_size_infos(u32 sb_sz, const ha_opts_c& o, SizeInfo* out) {
     s.block_sz = (sid % 2 ? 3 : 2) * unit;
     s.tmod = RUN;
     s.nblocks = sb_sz / s.block_sz / RUN * RUN;
+    // a lane group's first probe starts at run (ticket / RUN) * THREAD_FREQ:
+    // with THREAD_FREQ dividing the number of runs, first probes would reach
+    // only every THREAD_FREQ-th run (31 of 341 for 192 B at 4 MiB slabs)
+    if ((s.nblocks / RUN) % THREAD_FREQ == 0) s.nblocks -= RUN;
     s.chunk_id = sid % 2 + (sid < NSIZES / 2 ? 0 : 2);
     s.chunk_bytes = (s.chunk_id % 2 ? 3 : 2) * (s.chunk_id / 2 ? 128 : 8);
     s.chunk_sz = chunk_val(s.chunk_bytes);

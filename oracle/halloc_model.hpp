@@ -170,6 +170,7 @@ struct SizeInfo {
 };
 
 // the size table (src/halloc.cu:363-394); hash_mode 1 rounds nblocks to a multiple of RUN
+// whose run count THREAD_FREQ does not divide
 inline bool build_size_infos(u32 sb_sz, const Opts& o, SizeInfo out[NSIZES]) {
   for (u32 s = 0; s < NSIZES; ++s) {
     SizeInfo& si = out[s];
@@ -178,6 +179,9 @@ inline bool build_size_infos(u32 sb_sz, const Opts& o, SizeInfo out[NSIZES]) {
     u32 nblocks = sb_sz / si.block_sz;
     si.tmod = o.hash_mode == 0 ? tmod_ref(s) : RUN;
     nblocks = nblocks / si.tmod * si.tmod;
+    // runs advance THREAD_FREQ at a time: one run fewer when that would
+    // divide their number, so that first probes reach every run
+    if (o.hash_mode != 0 && (nblocks / RUN) % THREAD_FREQ == 0) nblocks -= RUN;
     si.nblocks = nblocks;
     si.chunk_id = s % 2 + (s < NSIZES / 2 ? 0 : 2);
     si.chunk_bytes = (si.chunk_id % 2 ? 3 : 2) * (si.chunk_id / 2 ? 128 : 8);

@@ -109,9 +109,17 @@ def test_size_table_properties(hash_mode, sh):
         assert r["ncib"] in (1, 2, 4, 8)
         tmod = r["tmod"]
         assert tmod == (64 if hash_mode else 1 << max(2 - sid // 2, 0))
-        # nblocks: the largest multiple of tmod that fits
+        # nblocks: the largest multiple of tmod that fits; with 64-block runs,
+        # one run fewer where 11 (the run stride) would divide the run count
         assert r["nblocks"] % tmod == 0
-        assert r["nblocks"] * r["block_sz"] <= sb < (r["nblocks"] + tmod) * r["block_sz"]
+        fit = sb // r["block_sz"] // tmod * tmod
+        if hash_mode and (fit // 64) % 11 == 0:
+            fit -= 64
+        assert r["nblocks"] == fit and r["nblocks"] * r["block_sz"] <= sb
+        if hash_mode:
+            # first probes start at run (ticket / 64) * 11 mod runs: every run is reached
+            runs = r["nblocks"] // 64
+            assert len({(j * 11) % runs for j in range(runs)}) == runs
         assert r["nchunks"] == r["nblocks"] * r["ncib"]
         # hash step = ncib * the largest admissible prime
         n, nb = r["nblocks"] // 256 + r["nblocks"] // 64, r["nblocks"]
