@@ -376,8 +376,10 @@ again:
 // wave that takes the head lock replaces a missing or full head, and a wave
 // that finds the lock taken waits for the head to change or the lock to be
 // released.  A waiter whose wait saw a slab search fail gets SB_NONE too
-// rather than repeating the search; otherwise it looks again.
-HA_DEV u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen) {
+// rather than repeating the search; otherwise it looks again.  force: the
+// caller's probes ran out of tries in `seen`, which is then replaced even
+// below its limit (as the reference replaces a head a thread failed in).
+HA_DEV u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen, bool force = false) {
   const SizeInfo& s = ha_p.si[size_id];
   u32* hp = hot(ha_p.head, ihead, size_id);
   u32* lp = hot(ha_p.head_lock, ihead, size_id);
@@ -390,7 +392,9 @@ HA_DEV u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen) {
     if (try_lock(lp)) {
       cur = ld32(hp);
       u32 nh;
-      if (cur == SB_NONE || ctr_count(ld64(ctr_of(cur & ~HEAD_EXT))) >= head_limit(cur, s)) {
+      // below: a forced replacement of a head that still has room
+      const bool below = cur != SB_NONE && ctr_count(ld64(ctr_of(cur & ~HEAD_EXT))) < head_limit(cur, s);
+      if (cur == SB_NONE || !below || (force && cur == seen)) {
         nh = ld32(cp);
         st32(cp, SB_NONE);
         if (nh == SB_NONE) {
@@ -401,9 +405,11 @@ HA_DEV u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen) {
           if (ha_p.nheads > 1) lock(cl);
           nh = find_sb_for_size(size_id, s.chunk_id, ihead, true);
           if (ha_p.nheads > 1) unlock(cl);
-          st32(ep, (((ld32(ep) >> 1) + 1) << 1) | (nh == SB_NONE ? 1u : 0u));
+          st32(ep, (((ld32(ep) >> 1) + 1) << 1) | (nh == SB_NONE && !below ? 1u : 0u));
         }
-        if (nh != SB_NONE) {
+        if (nh == SB_NONE && below) {
+          nh = cur;  // no other slab: keep the head, which still has room
+        } else if (nh != SB_NONE) {
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
           st32(hp, nh);
           if (cur == SB_NONE) a_add32(ha_p.active, 1u);
@@ -450,7 +456,7 @@ HA_DEV u32 mod_chunks(const SizeInfo& s, u64 x) {
 // the whole group; off0 is the probe offset of the first attempt (a run that
 // resumes after a first attempt made elsewhere starts at itry 1).
 HA_DEV u64 probe_run(const SizeInfo& s, u32 hsb, u32 limit, u32 old, u64 P, u32 me, u32& itry, u32& my_ich,
-                     u64 off0 = 0) {
+                     u64 off0 = 0, bool* spent = nullptr) {
   const u32 k = s.ncib, nchunks = s.nchunks, nwb = nchunks >> 6, nwn = nchunks >> 4;
   const u32 slot0 = old & (RUN - 1), split = RUN - slot0;
   const u32 runA = mod_by(((old & ~(RUN - 1)) * THREAD_FREQ) * k, nchunks, s.nchunks_m);
@@ -504,7 +510,10 @@ HA_DEV u64 probe_run(const SizeInfo& s, u32 hsb, u32 limit, u32 old, u64 P, u32 
     }
     const u32 step = mod_by((itry + 1) * STEP_FREQ * k, s.hash_step, s.hash_step_m);
     off = (u64)(itry + 1) * step;
-    if (++itry % MAX_NTRIES >= MAX_NTRIES - 1) break;
+    if (++itry % MAX_NTRIES >= MAX_NTRIES - 1) {
+      if (spent) *spent = true;
+      break;
+    }
   }
   return reserved;
 }
@@ -533,13 +542,13 @@ HA_DEV void* malloc_group(u32 sid, u64 g, u32 ihead, u32 me) {
   void* p = nullptr;
   u32 my_ich = 0;
   while (want) {
-    bool need = false;
+    bool need = false, spent = false;
     const u32 hsb = hword == SB_NONE ? SB_NONE : (hword & ~HEAD_EXT);
     const u32 limit = head_limit(hword, s);
     if (hsb == SB_NONE) {
       need = true;
     } else {
-      const u64 reserved = probe_run(s, hsb, limit, old, want, me, itry, my_ich);
+      const u64 reserved = probe_run(s, hsb, limit, old, want, me, itry, my_ich, 0, &spent);
       if (reserved != want) need = true;
       if (reserved) {
         // sb_ctr_inc (src/slab.cuh:122-155): one atomic for the group
@@ -566,7 +575,7 @@ HA_DEV void* malloc_group(u32 sid, u64 g, u32 ihead, u32 me) {
     }
     if (need) {
       u32 nh = 0;
-      if (me == lead) nh = new_sb_for_size(sid, ihead, hword);
+      if (me == lead) nh = new_sb_for_size(sid, ihead, hword, spent && want != 0);
       hword = rdl(nh, lead);
       if (hword == SB_NONE) want = 0;
     }

@@ -552,7 +552,9 @@ class Heap {
   // caller last used: a head replaced since then is returned as it is; a
   // thread that finds the head lock taken waits for a change, and gets
   // SB_NONE if a slab search failed meanwhile, else looks again.
-  u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen) {
+  // force: the caller's probes ran out of tries in `seen`, which is then
+  // replaced even below its limit (halloc.hip new_sb_for_size)
+  u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen, bool force = false) {
     const SizeInfo& s = si[size_id];
     size_t hi = (size_t)ihead * NSIZES + size_id;
     for (;;) {
@@ -562,7 +564,9 @@ class Heap {
       if (try_lock(head_lock[hi])) {
         cur = head[hi].load();
         u32 nh;
-        if (cur == SB_NONE || ctr_count(ctr[cur & ~HEAD_EXT].load()) >= head_limit(cur, s)) {
+        // below: a forced replacement of a head that still has room
+        const bool below = cur != SB_NONE && ctr_count(ctr[cur & ~HEAD_EXT].load()) < head_limit(cur, s);
+        if (cur == SB_NONE || !below || (force && cur == seen)) {
           nh = cached[hi].load();
           cached[hi].store(SB_NONE);
           if (nh == SB_NONE) {
@@ -571,9 +575,11 @@ class Heap {
             if (nheads > 1) lock(class_lock[size_id]);
             nh = find_sb_for_size(size_id, s.chunk_id, ihead, true);
             if (nheads > 1) unlock(class_lock[size_id]);
-            search_ep[hi].store((((search_ep[hi].load() >> 1) + 1) << 1) | (nh == SB_NONE ? 1u : 0u));
+            search_ep[hi].store((((search_ep[hi].load() >> 1) + 1) << 1) | (nh == SB_NONE && !below ? 1u : 0u));
           }
-          if (nh != SB_NONE) {
+          if (nh == SB_NONE && below) {
+            nh = cur;  // no other slab: keep the head, which still has room
+          } else if (nh != SB_NONE) {
             head[hi].store(nh);
             if (cur == SB_NONE) active.fetch_add(1);
             if (cur != SB_NONE) {
@@ -603,7 +609,7 @@ class Heap {
   // skip_first: the lanes P already failed a first attempt at ich0 (made and
   // rolled back by the caller); go on from the second probe.
   u64 probe_in(const SizeInfo& s, u32 hsb, u32 limit, u64 P, const u32* ich0, u32* ich, u32& itry,
-               bool skip_first = false) {
+               bool skip_first = false, bool* spent = nullptr) {
     const u64 mask1 = (1ull << s.ncib) - 1;
     std::atomic<u64>* bw = &bits[(size_t)hsb * bit_words];
     u64 probing = P, reserved = 0;
@@ -631,7 +637,10 @@ class Heap {
         if (ctr_count(ctr[hsb].load()) >= limit) break;
       }
       for (u32 l = 0; l < WAVE; ++l) if (probing >> l & 1) ich[l] = probe_next(ich0[l], itry, s);
-      if (++itry % MAX_NTRIES >= MAX_NTRIES - 1) break;
+      if (++itry % MAX_NTRIES >= MAX_NTRIES - 1) {
+        if (spent) *spent = true;
+        break;
+      }
     }
     return reserved;
   }
@@ -670,13 +679,13 @@ class Heap {
     u32 itry = 0;
     u64 want = g;
     while (want) {
-      bool need = false;
+      bool need = false, spent = false;
       const u32 hsb = hword == SB_NONE ? SB_NONE : (hword & ~HEAD_EXT);
       const u32 limit = head_limit(hword, s);
       if (hsb == SB_NONE) {
         need = true;
       } else {
-        u64 reserved = probe_in(s, hsb, limit, want, ich0, ich, itry);
+        u64 reserved = probe_in(s, hsb, limit, want, ich0, ich, itry, false, &spent);
         if (reserved != want) need = true;
         if (reserved) {
           // sb_ctr_inc (src/slab.cuh:122-155), one group per (slab, ncib)
@@ -695,7 +704,7 @@ class Heap {
         }
       }
       if (need) {
-        u32 nh = new_sb_for_size(size_id, ihead, hword);
+        u32 nh = new_sb_for_size(size_id, ihead, hword, spent && want != 0);
         hword = nh;
         if (nh == SB_NONE) {
           for (u32 l = 0; l < WAVE; ++l) if (want >> l & 1) {
