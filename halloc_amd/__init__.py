@@ -29,7 +29,7 @@ EXPORTS = ("ha_default_opts", "ha_init_c", "ha_shutdown_c", "ha_extfrag_c", "ha_
            "ha_last_error", "ha_geometry_c", "ha_replay", "ha_dump_words", "ha_dump",
            "ha_run_plain", "ha_malloc_sizes", "ha_free_ptrs", "ha_read_words", "ha_probe_atomics",
            "ha_run_lists", "ha_run_phase", "ha_replay_batch", "ha_run_graphs", "ha_run_serve",
-           "ha_probe_copy", "ha_run_test", "ha_run_sample")
+           "ha_probe_copy", "ha_run_test", "ha_run_sample", "ha_run_corr_test0", "ha_probe_chain")
 
 
 class HallocError(RuntimeError):
@@ -54,12 +54,19 @@ class _Geometry(ctypes.Structure):
                 ("nheads", ctypes.c_uint32), ("nsw", ctypes.c_uint32), ("bit_words", ctypes.c_uint32),
                 ("nib_words", ctypes.c_uint32), ("pad", ctypes.c_uint32),
                 ("arena_bytes", ctypes.c_uint64), ("large_bytes", ctypes.c_uint64),
-                ("size_table", ctypes.c_uint32 * (16 * 12))]
+                ("size_table", ctypes.c_uint32 * (16 * 12)),
+                ("arena_base", ctypes.c_uint64), ("large_units", ctypes.c_uint32),
+                ("large_classes", ctypes.c_uint32)]
 
 
 class _PlainResult(ctypes.Structure):
     _fields_ = [("errors", ctypes.c_uint64), ("nulls", ctypes.c_uint64), ("total_ms", ctypes.c_double),
                 ("wall_ms", ctypes.c_double)]
+
+
+class _Corr0Result(ctypes.Structure):
+    _fields_ = [("blocks", ctypes.c_uint64), ("nulls", ctypes.c_uint64), ("bad_contents", ctypes.c_uint64),
+                ("bad_gaps", ctypes.c_uint64), ("free_ms", ctypes.c_double)]
 
 
 class _ListResult(ctypes.Structure):
@@ -186,6 +193,8 @@ def lib() -> ctypes.CDLL:
         "ha_dump": (i32, [vp, ctypes.c_int64]),
         "ha_run_plain": (i32, [i32, i32, i32, i32, i32, i32, vp, P(_PlainResult)]),
         "ha_malloc_sizes": (i32, [vp, i32, i32, vp]),
+        "ha_run_corr_test0": (i32, [i32, i32, i32, P(_Corr0Result)]),
+        "ha_probe_chain": (i32, [vp, i32, i32, vp]),
         "ha_free_ptrs": (i32, [vp, i32, i32]),
         "ha_read_words": (i32, [vp, i32, vp]),
         "ha_probe_atomics": (i32, [vp]),
@@ -243,7 +252,8 @@ class Halloc:
         table = np.frombuffer(bytes(g.size_table), dtype=np.uint32).reshape(NSIZES, 12).copy()
         return {"nsbs": g.nsbs, "sb_sz": g.sb_sz, "sb_sz_sh": g.sb_sz_sh, "nheads": g.nheads,
                 "nsw": g.nsw, "bit_words": g.bit_words, "nib_words": g.nib_words,
-                "arena_bytes": g.arena_bytes, "large_bytes": g.large_bytes, "size_table": table}
+                "arena_bytes": g.arena_bytes, "large_bytes": g.large_bytes, "size_table": table,
+                "arena_base": g.arena_base, "large_units": g.large_units, "large_classes": g.large_classes}
 
     # ---------------------------------------------------------- workloads
     @staticmethod
@@ -293,6 +303,24 @@ class Halloc:
         ptrs = np.zeros(len(s), dtype=np.uint64)
         _check(lib().ha_malloc_sizes(s.ctypes.data, len(s), bs, ptrs.ctypes.data))
         return ptrs
+
+    @staticmethod
+    def probe_chain(rounds=(0, 1, 2, 4, 7, 8, 16), nthreads=1 << 20) -> dict:
+        """Launch time (ms, best of 5) of a grid whose waves each make r
+        dependent atomic round trips, for every r in rounds."""
+        rr = np.ascontiguousarray(rounds, dtype=np.int32)
+        ms = np.zeros(len(rr), dtype=np.float64)
+        _check(lib().ha_probe_chain(rr.ctypes.data, len(rr), nthreads, ms.ctypes.data))
+        return {int(r): float(m) for r, m in zip(rr, ms)}
+
+    @staticmethod
+    def run_corr_test0(nthreads=(2 << 20) // 8, nmallocs=8, bs=256) -> dict:
+        """tst/corr/test/test.cu test 0: nthreads x nmallocs 16 B blocks, each
+        written with its own address and read back, sorted gaps >= 16 B, freed."""
+        r = _Corr0Result()
+        _check(lib().ha_run_corr_test0(nthreads, nmallocs, bs, ctypes.byref(r)))
+        return {"blocks": r.blocks, "nulls": r.nulls, "bad_contents": r.bad_contents, "bad_gaps": r.bad_gaps,
+                "free_ms": r.free_ms}
 
     @staticmethod
     def free_ptrs(ptrs, bs=256) -> None:

@@ -86,6 +86,77 @@ HA_HD inline void run_word_nibs(u64 B, u32 me, u32 s, u32 k, u64 val, u32& word,
   issuer = sub ? lo + (u32)__builtin_ctzll(sub) : 64u;
 }
 
+// ----------------------------------------------------------- unit fields
+// The large-block region keeps one bit per 4 KiB unit in 64-bit words; a block
+// of k = 2^c units (k <= 64) is a k-bit field aligned to k inside one word.
+
+// k one-bits (k in 1..64)
+HA_HD inline u64 low_mask(u32 k) { return k >= 64 ? ~0ull : ((1ull << k) - 1); }
+// one bit at every multiple of k: the start bits of the aligned k-bit fields
+HA_HD inline u64 field_starts(u32 k) { return k >= 64 ? 1ull : ~0ull / low_mask(k); }
+// start bits of the aligned k-bit fields of v that are entirely zero
+HA_HD inline u64 free_fields(u64 v, u32 k) {
+  u64 f = ~v;
+  for (u32 s = 1; s < k; s <<= 1) f &= f >> s;
+  return f & field_starts(k);
+}
+// start bits of the aligned k-bit fields of v that hold at least one one-bit
+HA_HD inline u64 used_fields(u64 v, u32 k) {
+  u64 f = v;
+  for (u32 s = 1; s < k; s <<= 1) f |= f >> s;
+  return f & field_starts(k);
+}
+// every start bit of b (aligned to k) widened to its k-bit field
+HA_HD inline u64 widen_fields(u64 b, u32 k) { return b * low_mask(k); }
+// position of the r-th lowest one-bit of x (r < popcount(x))
+HA_HD inline u32 select_bit(u64 x, u32 r) {
+  u32 pos = 0;
+  for (u32 w = 32; w; w >>= 1) {
+    const u32 c = (u32)__builtin_popcountll(x & low_mask(w));
+    if (r >= c) {
+      r -= c;
+      x >>= w;
+      pos += w;
+    }
+  }
+  return pos;
+}
+// the n lowest one-bits of x
+HA_HD inline u64 lowest_bits(u64 x, u32 n) {
+  if (n == 0) return 0;
+  if (n >= (u32)__builtin_popcountll(x)) return x;
+  return x & low_mask(select_bit(x, n - 1) + 1);
+}
+// The first word a group of large requests of class c looks at: its ticket
+// spread over the nw words by a multiplicative (Fibonacci) hash.
+HA_HD inline u32 large_start(u32 ticket, u32 c, u32 nw) {
+  const u32 h = (ticket * 0x9E3779B1u) ^ (c * 0x85EBCA77u);
+  return (u32)(((u64)h * nw) >> 32);
+}
+
+// The largest block (in units) the large region's bit words lb[0..nw) can
+// hand out now, over classes 0..ncl-1: a free aligned field inside a word
+// for up to 64 units, an aligned run of zero words above.
+inline u64 large_max_units(const u64* lb, u32 nw, u32 ncl) {
+  u64 best = 0;
+  for (u32 c = 0; c < ncl; ++c) {
+    bool found = false;
+    if (c <= 6) {
+      for (u32 w = 0; w < nw && !found; ++w) found = free_fields(lb[w], 1u << c) != 0;
+    } else {
+      const u32 m = 1u << (c - 6);
+      for (u32 g = 0; g + 1 <= nw / m && !found; ++g) {
+        bool z = true;
+        for (u32 j = 0; j < m && z; ++j) z = lb[(u64)g * m + j] == 0;
+        found = z;
+      }
+    }
+    if (!found) break;  // no class above fits either
+    best = 1ull << c;
+  }
+  return best;
+}
+
 // x mod n without a division, for x < 2^32 and 1 <= n, given m = mod_magic(n)
 // = floor(2^32 / n) (0 for n = 1): q = floor(x * m / 2^32) is floor(x / n) or
 // one less, since x * m / 2^32 lies in (x / n - 1, x / n]; one correction step

@@ -10,6 +10,9 @@
 //   sb_lock     nsbs u32                                                (src/slab.cuh:18)
 //   sets        NSETS * nsw u64 slab bitmaps: free, roomy[16], busy[16], sparse[4]
 //   head/cached/head_lock/ticket: nheads * 16 u32 each, one per 128 B line
+//   lbits       ceil(lunits/64) u64: one bit per 4 KiB unit of the large region
+//   lstart      lunits u32: class + 1 of the live large block starting at the unit
+//   lticket     LARGE_NCLASSES u32, one per 128 B line
 // Read-only parameters (pointers, sizes, the size table) sit in __constant__ memory.
 #pragma once
 
@@ -48,10 +51,11 @@ constexpr u32 MAX_SB_SZ_SH = 26;     // tst/common/common.cu:169 allows -b 20..2
 // a head word is a slab id, plus HEAD_EXT when the slab came from the busy set:
 // such a head is filled up to (busy + nchunks) / 2 instead of busy
 constexpr u32 HEAD_EXT = 0x80000000u;
-// large blocks (above MAX_BLOCK_SZ): power-of-two multiples of LARGE_UNIT
+// large blocks (above MAX_BLOCK_SZ): class c is 2^c units of LARGE_UNIT bytes,
+// aligned to its size inside the large region; one bit per unit marks it used
 constexpr u32 LARGE_UNIT_SH = 12;     // 4 KiB
 constexpr u32 LARGE_NCLASSES = 40;    // classes 4 KiB << c
-constexpr u32 LSTACK_STRIDE = 16;     // u64 words: one 128 B line per class stack head
+constexpr u32 LARGE_SB = 0xfffffffeu; // the slab field of a replay record of a large block
 
 constexpr u32 CTR_CHUNK_SH = 8;
 constexpr u64 CTR_CHUNK_NONE = 0xF;
@@ -84,16 +88,18 @@ struct Params {
   u32* ticket;
   u32* class_lock;  // one per size class, serializes head searches of its pairs
   u32* search_ep;   // per pair: searches done (bits 31..1) and whether the last failed (bit 0)
+  u32* forced;      // per pair: the head whose forced replacement found no slab, and the free epoch then
   u32* active;      // pairs that hold a head
-  // large-block region: per class a free stack (tag << 32 | unit + 1), a bump
-  // pointer in units, and per 4 KiB unit the next link and the block's class
+  u32* free_ep;     // slabs that became free so far
+  // large-block region: one bit per 4 KiB unit (lbits, the bits past lunits
+  // set), per unit the class + 1 of the live block that starts there (lstart,
+  // 0 elsewhere), per class a ticket that spreads the first word probed
   char* large;
   u64 large_bytes;
-  u64* lstack;
-  u64* lbump;
-  u32* llink;
-  u32* lclass;
-  u32 lunits, lnclasses;
+  u64* lbits;
+  u32* lstart;
+  u32* lticket;
+  u32 lunits, lwords, lnclasses, lpad;
   u32 nsbs, sb_sz, sb_sz_sh, nheads;
   u32 nsw, bit_words, nib_words, initialized;
   u32 cached_arena;       // hafree releases before publishing

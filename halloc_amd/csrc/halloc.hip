@@ -128,6 +128,7 @@ HA_DEV void try_mark_free(u32 sb, u32 size_id, u32 chunk_id, bool from_head) {
       st32(&d->chunk_id, SZ_NONE);
       st32(&d->chunk_sz, 0);
       set_add(0, sb);
+      a_add32(ha_p.free_ep, 1u);
       set_remove(set_sparse(chunk_id), sb);
       if (size_id != SZ_NONE) {
         set_remove(set_roomy(size_id), sb);
@@ -378,13 +379,19 @@ again:
 // released.  A waiter whose wait saw a slab search fail gets SB_NONE too
 // rather than repeating the search; otherwise it looks again.  force: the
 // caller's probes ran out of tries in `seen`, which is then replaced even
-// below its limit (as the reference replaces a head a thread failed in).
+// below its limit -- a deliberate difference: the reference replaces a head
+// only when it is missing or at its busy threshold (src/slab.cuh:385-387).
+// When such a forced search finds no other slab the head is kept, and the
+// pair records the head and the free-slab epoch: a later forced replacement
+// of the same head before any slab has become free keeps it without
+// searching again.
 HA_DEV u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen, bool force = false) {
   const SizeInfo& s = ha_p.si[size_id];
   u32* hp = hot(ha_p.head, ihead, size_id);
   u32* lp = hot(ha_p.head_lock, ihead, size_id);
   u32* cp = hot(ha_p.cached, ihead, size_id);
   u32* ep = hot(ha_p.search_ep, ihead, size_id);
+  u32* fp = hot(ha_p.forced, ihead, size_id);  // [0] head word, [1] free epoch
   for (;;) {
     u32 cur = ld32(hp);
     if (cur != seen && cur != SB_NONE) return cur;
@@ -394,7 +401,10 @@ HA_DEV u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen, bool force = false)
       u32 nh;
       // below: a forced replacement of a head that still has room
       const bool below = cur != SB_NONE && ctr_count(ld64(ctr_of(cur & ~HEAD_EXT))) < head_limit(cur, s);
-      if (cur == SB_NONE || !below || (force && cur == seen)) {
+      const u32 fe = ld32(ha_p.free_ep);
+      if (below && ld32(fp) == cur && ld32(fp + 1) == fe) {
+        nh = cur;  // this head's forced search failed, and no slab became free since
+      } else if (cur == SB_NONE || !below || (force && cur == seen)) {
         nh = ld32(cp);
         st32(cp, SB_NONE);
         if (nh == SB_NONE) {
@@ -409,6 +419,8 @@ HA_DEV u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen, bool force = false)
         }
         if (nh == SB_NONE && below) {
           nh = cur;  // no other slab: keep the head, which still has room
+          st32(fp + 1, fe);
+          st32(fp, cur);
         } else if (nh != SB_NONE) {
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
           st32(hp, nh);
@@ -1009,53 +1021,139 @@ HA_DEV void free_small(u64 off) {
 }
 
 // ------------------------------------------------------------ large blocks
-// Requests above MAX_BLOCK_SZ come from the large region (the share of the
-// pool outside the slabs, src/halloc.cu:285-289): class c holds blocks of
-// 4 KiB << c.  A free block goes on its class's lock-free stack (an ABA tag in
-// the head's high word); an empty stack is refilled from a bump pointer.
-// Blocks keep their class for the life of the heap.
+// Requests above MAX_BLOCK_SZ come from the large region, the share of the
+// pool outside the slabs (src/halloc.cu:285-289, which the reference hands to
+// the CUDA device heap).  Class c is a block of 2^c units of 4 KiB aligned to
+// its size.  One bit per unit marks it used, so the units of a freed block
+// serve any class again: blocks split and merge without lists.  lstart[u]
+// holds c + 1 while a block of class c starts at unit u; a free claims it
+// with one exchange, so a second free, an interior pointer or a unit never
+// handed out reads 0 and changes nothing.
 HA_DEV u32 large_class(u64 nbytes) {
-  const u64 units = (nbytes + (1ull << LARGE_UNIT_SH) - 1) >> LARGE_UNIT_SH;
+  const u64 units = (nbytes >> LARGE_UNIT_SH) + ((nbytes & ((1ull << LARGE_UNIT_SH) - 1)) != 0);
   return units <= 1 ? 0u : 64u - (u32)__builtin_clzll(units - 1);
 }
 
-HA_DEV void* large_alloc(u64 nbytes) {
-  const u32 c = large_class(nbytes);
-  if (c >= ha_p.lnclasses) return nullptr;
-  u64* hp = ha_p.lstack + (u64)c * LSTACK_STRIDE;
-  for (;;) {
-    const u64 h = ld64(hp);
-    const u32 idx = (u32)h;
-    if (!idx) break;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the link written before idx was pushed
-    const u32 nxt = ld32(ha_p.llink + (idx - 1));
-    if (atomicCAS((unsigned long long*)hp, h, (((h >> 32) + 1) << 32) | nxt) == h)
-      return ha_p.large + ((u64)(idx - 1) << LARGE_UNIT_SH);
-  }
-  const u64 nu = 1ull << c;
-  for (;;) {
-    const u64 b = ld64(ha_p.lbump);
-    if (b + nu > ha_p.lunits) return nullptr;
-    if (atomicCAS((unsigned long long*)ha_p.lbump, b, b + nu) == b) {
-      st32(ha_p.lclass + b, c);
-      return ha_p.large + (b << LARGE_UNIT_SH);
+HA_DEV u64 a_cas(u64* p, u64 expected, u64 desired) {
+  __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED, HA_SCOPE);
+  return expected;  // the value found: `expected` as passed when the exchange took place
+}
+
+constexpr u64 UNIT_NONE = ~0ull;
+
+// One class group of large requests of one wave: the lanes g (wave-uniform)
+// want a block of class c.  The group takes one ticket (one atomic).  Blocks
+// of up to 64 units: the group starts at word large_start(ticket) and walks
+// the words in order; in each word the pending lanes take its free fields
+// lowest first (the lane of rank r the r-th field), reserved by ONE atomic OR
+// for the group, and the fields another wave took meanwhile are given back by
+// one AND.  The group stays on a word while the value its last atomic
+// returned shows a free field (every field it loses is one another lane
+// won), so in a kernel that only allocates, NULL means every word was full
+// when the group looked at it.  Larger blocks take an
+// aligned run of whole words, one lane at a time in lane order, from the run
+// large_start(ticket + rank) on, each word claimed by a compare-and-swap from
+// zero.  Returns this lane's unit (UNIT_NONE: nothing free was found).
+HA_DEV u64 large_group(u32 c, u64 g, u32 me) {
+  const u32 lead = (u32)__builtin_ctzll(g);
+  const u32 nw = ha_p.lwords;
+  u64* lb = ha_p.lbits;
+  u32 old = 0;
+  if (me == lead) old = a_add32(ha_p.lticket + c * HOT_STRIDE, (u32)__builtin_popcountll(g));
+  old = rdl(old, lead);
+  u64 unit = UNIT_NONE;
+  if (c <= 6) {
+    const u32 k = 1u << c;
+    u32 w = large_start(old, c, nw), moved = 0;
+    u64 v = rdl64(ld64(lb + w), lead);
+    u64 pend = g;
+    while (pend) {
+      const u64 cand = free_fields(v, k);
+      const u32 n = (u32)__builtin_popcountll(cand);
+      if (n == 0) {
+        if (++moved >= nw) break;  // every word looked at: nothing free
+        w = w + 1 == nw ? 0u : w + 1;
+        v = rdl64(ld64(lb + w), lead);
+        continue;
+      }
+      const u64 pick = lowest_bits(cand, (u32)__builtin_popcountll(pend));
+      const u64 U = widen_fields(pick, k);
+      const u32 r = (u32)__builtin_popcountll(pend & ((1ull << me) - 1));
+      const bool mine = ((pend >> me) & 1) && r < n;
+      const u32 pos = mine ? select_bit(pick, r) : 0u;
+      u64 o = 0;
+      if (me == lead) o = a_or(lb + w, U);
+      o = rdl64(o, lead);
+      const u64 won = pick & ~used_fields(o, k);
+      const u64 back = U & ~o & ~widen_fields(won, k);  // bits set in fields found taken
+      if (me == lead && back) a_and(lb + w, ~back);
+      const bool got = mine && ((won >> pos) & 1);
+      if (got) unit = (u64)w * 64 + pos;
+      pend &= ~ballot(got);
+      v = o | widen_fields(won, k);
+    }
+  } else {
+    const u32 m = 1u << (c - 6);  // words per block
+    const u32 ngr = nw / m;        // aligned runs of m words
+    for (u64 G = g; G; G &= G - 1) {
+      const u32 l = (u32)__builtin_ctzll(G);
+      if (me != l || ngr == 0) continue;
+      const u32 r = (u32)__builtin_popcountll(g & ((1ull << l) - 1));
+      const u32 g0 = large_start(old + r, c, ngr);
+      for (u32 i = 0; i < ngr && unit == UNIT_NONE; ++i) {
+        const u32 grp = g0 + i >= ngr ? g0 + i - ngr : g0 + i;
+        u64* base = lb + (u64)grp * m;
+        u32 j = 0;
+        while (j < m && ld64(base + j) == 0) ++j;
+        if (j < m) continue;
+        for (j = 0; j < m; ++j)
+          if (a_cas(base + j, 0ull, ~0ull) != 0ull) break;
+        if (j == m) {
+          unit = (u64)grp * m * 64;
+        } else {
+          while (j--) a_and(base + j, 0ull);  // give back the words taken
+        }
+      }
     }
   }
+  if (unit != UNIT_NONE) st32(ha_p.lstart + unit, c + 1);
+  return unit;
+}
+
+// hamalloc for the active lanes whose requests exceed MAX_BLOCK_SZ: one
+// class group at a time, groups ordered by their lowest lane
+HA_DEV void* large_alloc(u64 nbytes) {
+  const u32 me = lane_id();
+  const u32 c = large_class(nbytes);
+  const bool fits = c < ha_p.lnclasses;
+  void* res = nullptr;
+  u64 rem = ballot(fits);
+  while (rem) {
+    const u32 l0 = (u32)__builtin_ctzll(rem);
+    const u32 gc = rdl(c, l0);
+    const u64 g = ballot(fits && c == gc) & rem;
+    if ((g >> me) & 1) {
+      const u64 u = large_group(gc, g, me);
+      if (u != UNIT_NONE) res = ha_p.large + (u << LARGE_UNIT_SH);
+    }
+    rem &= ~g;
+  }
+  return res;
 }
 
 HA_DEV void large_free(u64 off) {
-  if (off & ((1ull << LARGE_UNIT_SH) - 1)) return;  // not a block start
-  const u32 unit = (u32)(off >> LARGE_UNIT_SH);
-  const u32 c = ld32(ha_p.lclass + unit);
-  if (c >= ha_p.lnclasses) return;
-  u64* hp = ha_p.lstack + (u64)c * LSTACK_STRIDE;
-  for (;;) {
-    const u64 h = ld64(hp);
-    st32(ha_p.llink + unit, (u32)h);
-    // the link, and with a cached arena the caller's stores to the block,
-    // before the block is visible on the stack
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (atomicCAS((unsigned long long*)hp, h, (((h >> 32) + 1) << 32) | (unit + 1)) == h) return;
+  if (off & ((1ull << LARGE_UNIT_SH) - 1)) return;  // not a unit start: not a block
+  const u64 unit = off >> LARGE_UNIT_SH;
+  const u32 s = a_xchg32(ha_p.lstart + unit, 0u);
+  if (s == 0) return;  // freed already, inside a block, or never handed out
+  // with a cached arena the caller's stores to the block reach memory before
+  // its units can be handed out again (see free_rounds)
+  if (ha_p.cached_arena) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  const u32 c = s - 1;
+  if (c <= 6) {
+    a_and(ha_p.lbits + unit / 64, ~(low_mask(1u << c) << (unit % 64)));
+  } else {
+    for (u64 j = 0; j < (1ull << (c - 6)); ++j) a_and(ha_p.lbits + unit / 64 + j, 0ull);
   }
 }
 
@@ -1337,11 +1435,14 @@ __global__ void init_state_k(Params P) {
     if (lo < P.nsbs) w = P.nsbs - lo >= 64 ? ~0ull : ((1ull << (P.nsbs - lo)) - 1);
     P.sets[i] = w;  // the free set
   }
+  // the units past the end of the large region are never free
+  if (i == 0 && P.lwords && P.lunits % 64) P.lbits[P.lwords - 1] = ~low_mask(P.lunits % 64);
   if (i < P.nheads * NSIZES) {
     P.head[i * HOT_STRIDE] = SB_NONE;
     P.cached[i * HOT_STRIDE] = SB_NONE;
     P.head_lock[i * HOT_STRIDE] = 0;
     P.ticket[i * HOT_STRIDE] = TICKET_INIT;
+    P.forced[i * HOT_STRIDE] = SB_NONE;
   }
 }
 
@@ -1407,6 +1508,7 @@ static int init_locked(const ha_opts_c& o) {
   // the large-block region follows the slabs in the same allocation
   P.lunits = (u32)std::min<u64>(cuda_memory >> LARGE_UNIT_SH, 0xfffffff0ull);
   P.large_bytes = (u64)P.lunits << LARGE_UNIT_SH;
+  P.lwords = (P.lunits + 63) / 64;
   P.lnclasses = 0;
   while (P.lnclasses < LARGE_NCLASSES && (1ull << P.lnclasses) <= P.lunits) ++P.lnclasses;
   // metadata offsets, each section 256 B aligned
@@ -1424,11 +1526,12 @@ static int init_locked(const ha_opts_c& o) {
   u64 o_ticket = off; off = align_up(off + hot_bytes, 256);
   u64 o_clock = off; off = align_up(off + (u64)NSIZES * HOT_STRIDE * 4, 256);
   u64 o_sep = off; off = align_up(off + hot_bytes, 256);
+  u64 o_forced = off; off = align_up(off + hot_bytes, 256);
   u64 o_active = off; off = align_up(off + 4, 256);
-  u64 o_lstack = off; off = align_up(off + (u64)LARGE_NCLASSES * LSTACK_STRIDE * 8, 256);
-  u64 o_lbump = off; off = align_up(off + 8, 256);
-  u64 o_llink = off; off = align_up(off + (u64)P.lunits * 4, 256);
-  u64 o_lclass = off; off = align_up(off + (u64)P.lunits * 4, 256);
+  u64 o_free_ep = off; off = align_up(off + 4, 256);
+  u64 o_lbits = off; off = align_up(off + (u64)P.lwords * 8, 256);
+  u64 o_lstart = off; off = align_up(off + (u64)P.lunits * 4, 256);
+  u64 o_ltick = off; off = align_up(off + (u64)LARGE_NCLASSES * HOT_STRIDE * 4, 256);
   void* arena = nullptr;
   int rc = arena_get(dev, P.arena_bytes + P.large_bytes, o.cached_arena != 0, &arena);
   if (rc) return rc;
@@ -1451,12 +1554,13 @@ static int init_locked(const ha_opts_c& o) {
   P.ticket = (u32*)(m + o_ticket);
   P.class_lock = (u32*)(m + o_clock);
   P.search_ep = (u32*)(m + o_sep);
+  P.forced = (u32*)(m + o_forced);
   P.active = (u32*)(m + o_active);
+  P.free_ep = (u32*)(m + o_free_ep);
   P.large = P.arena + P.arena_bytes;
-  P.lstack = (u64*)(m + o_lstack);
-  P.lbump = (u64*)(m + o_lbump);
-  P.llink = (u32*)(m + o_llink);
-  P.lclass = (u32*)(m + o_lclass);
+  P.lbits = (u64*)(m + o_lbits);
+  P.lstart = (u32*)(m + o_lstart);
+  P.lticket = (u32*)(m + o_ltick);
   P.initialized = 1;
   HA_HIP(hipMemset(meta, 0, off));
   u32 n = std::max(std::max(P.nsbs, P.nsw), P.nheads * NSIZES);
@@ -1543,18 +1647,13 @@ int ha_extfrag_c(int ideal, double* out) {
   const ha::Params& P = ha::g_host;
   ha::u64 max_alloc = 0;
   if (P.lunits) {
-    std::vector<ha::u64> heads((size_t)ha::LARGE_NCLASSES * ha::LSTACK_STRIDE);
-    ha::u64 bump = 0;
+    std::vector<ha::u64> lb(P.lwords);
     HA_HIP(hipDeviceSynchronize());
-    HA_HIP(hipMemcpy(heads.data(), P.lstack, heads.size() * 8, hipMemcpyDeviceToHost));
-    HA_HIP(hipMemcpy(&bump, P.lbump, 8, hipMemcpyDeviceToHost));
-    max_alloc = bump < P.lunits ? (ha::u64)(P.lunits - bump) << ha::LARGE_UNIT_SH : 0;
-    for (ha::u32 c = 0; c < P.lnclasses; ++c)
-      if ((ha::u32)heads[(size_t)c * ha::LSTACK_STRIDE] != 0)
-        max_alloc = std::max<ha::u64>(max_alloc, 1ull << (c + ha::LARGE_UNIT_SH));
+    HA_HIP(hipMemcpy(lb.data(), P.lbits, lb.size() * 8, hipMemcpyDeviceToHost));
+    max_alloc = (ha::u64)ha::large_max_units(lb.data(), P.lwords, P.lnclasses) << ha::LARGE_UNIT_SH;
   }
   if (ideal) max_alloc += free_slabs;
-  free_mem += P.large_bytes;
+  free_mem += P.large_bytes;  // the whole non-slab share, as the reference adds cuda_mem_g
   *out = free_mem ? 1.0 - (double)max_alloc / (double)free_mem : 0.0;
   return HA_OK;
 }
@@ -1569,6 +1668,9 @@ int ha_geometry_c(ha_geometry* g) {
   g->nsw = P.nsw; g->bit_words = P.bit_words; g->nib_words = P.nib_words;
   g->arena_bytes = P.arena_bytes; g->large_bytes = P.large_bytes;
   for (ha::u32 s = 0; s < ha::NSIZES; ++s) std::memcpy(g->size_table + 12 * s, &P.si[s], 12 * sizeof(uint32_t));
+  g->arena_base = (uint64_t)(uintptr_t)P.arena;
+  g->large_units = P.lunits;
+  g->large_classes = P.lnclasses;
   return HA_OK;
 }
 

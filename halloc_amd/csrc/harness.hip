@@ -62,6 +62,44 @@ struct Stream {
 };
 
 // ------------------------------------------------------------------ replay
+// Record of a malloc of `nbytes` that returned p: a slab block is (size class,
+// slab, chunk); a large block is (16 + its class, LARGE_SB, unit).
+__device__ void malloc_rec(const ha::Params& P, void* p, u64 nbytes, u32& r0, u32& r1, u32& r2) {
+  const u64 off = (u64)((char*)p - P.arena);
+  if (off < P.arena_bytes) {
+    const u32 sid = ha::size_id_from_nbytes((u32)nbytes);
+    r0 = sid;
+    r1 = (u32)(off >> P.sb_sz_sh);
+    r2 = ha::chunk_div((u32)(off & (P.sb_sz - 1)), P.si[sid].chunk_sz);
+  } else {
+    const u64 units = (nbytes >> ha::LARGE_UNIT_SH) + ((nbytes & ((1ull << ha::LARGE_UNIT_SH) - 1)) != 0);
+    r0 = ha::NSIZES + (units <= 1 ? 0u : 64u - (u32)__builtin_clzll(units - 1));
+    r1 = ha::LARGE_SB;
+    r2 = (u32)((off - P.arena_bytes) >> ha::LARGE_UNIT_SH);
+  }
+}
+// Record of a free of p, read before the free: a slab block gives (chunks
+// freed, slab, chunk) from its nibble, a large block (units freed, LARGE_SB,
+// unit) from its start word; (0, none, none) when the free changes nothing.
+__device__ void free_rec(const ha::Params& P, void* p, u32& r0, u32& r1, u32& r2) {
+  const u64 off = (u64)((char*)p - P.arena);
+  if (off < P.arena_bytes) {
+    const u32 sb = (u32)(off >> P.sb_sz_sh);
+    const u32 cs = __hip_atomic_load(&P.desc[sb].chunk_sz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cs) {
+      const u32 ich = ha::chunk_div((u32)(off & (P.sb_sz - 1)), cs);
+      const u64 w = __hip_atomic_load(P.nibs + (u64)sb * P.nib_words + ich / 16, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+      const u32 nch = (u32)((w >> (4 * (ich % 16))) & 0xF);
+      if (nch) { r0 = nch; r1 = sb; r2 = ich; }
+    }
+  } else if (off - P.arena_bytes < P.large_bytes && !((off - P.arena_bytes) & ((1ull << ha::LARGE_UNIT_SH) - 1))) {
+    const u32 unit = (u32)((off - P.arena_bytes) >> ha::LARGE_UNIT_SH);
+    const u32 s = __hip_atomic_load(P.lstart + unit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (s) { r0 = 1u << (s - 1); r1 = ha::LARGE_SB; r2 = unit; }
+  }
+}
+
 // One wavefront replays ops step by step: malloc lanes, then free lanes.
 __global__ void __launch_bounds__(64) replay_k(const u32* ops, int nsteps, int lanes, const u32* iheads,
                                                u32* recs, u32* xcc, void** ptrs) {
@@ -76,38 +114,17 @@ __global__ void __launch_bounds__(64) replay_k(const u32* ops, int nsteps, int l
       void* p = iheads ? ha_malloc_on_head(arg, ih) : hamalloc(arg);
       ptrs[i] = p;
       u32 r0 = 0xffffffffu, r1 = 0xffffffffu, r2 = 0xffffffffu;
-      if (p) {
-        u64 off = (u64)((char*)p - P.arena);
-        if (off < P.arena_bytes) {
-          u32 sid = ha::size_id_from_nbytes(arg);
-          r0 = sid;
-          r1 = (u32)(off >> P.sb_sz_sh);
-          r2 = ha::chunk_div((u32)(off & (P.sb_sz - 1)), P.si[sid].chunk_sz);
-        } else {
-          r0 = 16; r1 = 0; r2 = 0;
-        }
-      }
+      if (p) malloc_rec(P, p, arg, r0, r1, r2);
       recs[4 * i] = r0; recs[4 * i + 1] = r1; recs[4 * i + 2] = r2; recs[4 * i + 3] = 0;
     }
     __syncthreads();
-    if (kind == 2) {
-      // record the block's length from its nibble, then free it
+    if (kind == 2 || kind == 3) {
+      // record the block's length from its nibble or start word, then free
+      // it; kind 3 frees the pointer one large unit (4 KiB) into the block
       void* p = ptrs[arg];
+      if (kind == 3 && p) p = (char*)p + (1ull << ha::LARGE_UNIT_SH);
       u32 r0 = 0, r1 = 0xffffffffu, r2 = 0xffffffffu;
-      if (p) {
-        u64 off = (u64)((char*)p - P.arena);
-        if (off < P.arena_bytes) {
-          u32 sb = (u32)(off >> P.sb_sz_sh);
-          u32 cs = __hip_atomic_load(&P.desc[sb].chunk_sz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (cs) {
-            u32 ich = ha::chunk_div((u32)(off & (P.sb_sz - 1)), cs);
-            u64 w = __hip_atomic_load(P.nibs + (u64)sb * P.nib_words + ich / 16, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
-            u32 nch = (u32)((w >> (4 * (ich % 16))) & 0xF);
-            if (nch) { r0 = nch; r1 = sb; r2 = ich; }
-          }
-        }
-      }
+      if (p) free_rec(P, p, r0, r1, r2);
       hafree(p);
       recs[4 * i] = r0; recs[4 * i + 1] = r1; recs[4 * i + 2] = r2; recs[4 * i + 3] = 0;
       ptrs[i] = nullptr;
@@ -137,16 +154,7 @@ __global__ void __launch_bounds__(64) replay_batch_k(const u32* ops, int nsteps,
         void* p = out[j];
         u32* r = recs + 4 * (i * ha::MAXB + j);
         u32 r0 = 0xffffffffu, r1 = 0xffffffffu, r2 = 0xffffffffu;
-        if (p) {
-          u64 off = (u64)((char*)p - P.arena);
-          if (off < P.arena_bytes) {
-            u32 sid = ha::size_id_from_nbytes(arg);
-            r0 = sid; r1 = (u32)(off >> P.sb_sz_sh);
-            r2 = ha::chunk_div((u32)(off & (P.sb_sz - 1)), P.si[sid].chunk_sz);
-          } else {
-            r0 = 16; r1 = 0; r2 = 0;
-          }
-        }
+        if (p) malloc_rec(P, p, arg, r0, r1, r2);
         r[0] = r0; r[1] = r1; r[2] = r2; r[3] = 0;
       }
     }
@@ -157,20 +165,9 @@ __global__ void __launch_bounds__(64) replay_batch_k(const u32* ops, int nsteps,
       for (u32 j = 0; j < c; ++j) {
         void* p = fp[j];
         if (!p) continue;
-        u64 off = (u64)((char*)p - P.arena);
         u32* r = recs + 4 * (i * ha::MAXB + j);
         r[0] = 0; r[1] = 0xffffffffu; r[2] = 0xffffffffu; r[3] = 0;
-        if (off < P.arena_bytes) {
-          u32 sb = (u32)(off >> P.sb_sz_sh);
-          u32 cs = __hip_atomic_load(&P.desc[sb].chunk_sz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (cs) {
-            u32 ich = ha::chunk_div((u32)(off & (P.sb_sz - 1)), cs);
-            u64 w = __hip_atomic_load(P.nibs + (u64)sb * P.nib_words + ich / 16, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
-            u32 nch = (u32)((w >> (4 * (ich % 16))) & 0xF);
-            if (nch) { r[0] = nch; r[1] = sb; r[2] = ich; }
-          }
-        }
+        free_rec(P, p, r[0], r[1], r[2]);
       }
       hafree_batch(fp, (int)c);
       if (kind == 2)
@@ -184,7 +181,7 @@ __global__ void __launch_bounds__(64) replay_batch_k(const u32* ops, int nsteps,
 __global__ void dump_k(u64* out, u64 total) {
   const ha::Params& P = ha::ha_p;
   const u64 n1 = P.nsbs, n2 = 2ull * P.nsbs, n3 = (u64)ha::NSETS * P.nsw, nh = (u64)P.nheads * ha::NSIZES;
-  const u64 n5 = (u64)P.nsbs * P.bit_words;
+  const u64 n5 = (u64)P.nsbs * P.bit_words, n6 = (u64)P.nsbs * P.nib_words;
   for (u64 i = blockIdx.x * (u64)blockDim.x + threadIdx.x; i < total; i += (u64)gridDim.x * blockDim.x) {
     u64 j = i, v = 0;
     if (j < n1) { v = P.ctr[j * ha::CTR_STRIDE]; }
@@ -196,14 +193,17 @@ __global__ void dump_k(u64* out, u64 total) {
     else if ((j -= nh) < nh) { v = P.cached[j * ha::HOT_STRIDE]; }
     else if ((j -= nh) < nh) { v = P.ticket[j * ha::HOT_STRIDE]; }
     else if ((j -= nh) < n5) { v = P.bits[j]; }
-    else { j -= n5; v = P.nibs[j]; }
+    else if ((j -= n5) < n6) { v = P.nibs[j]; }
+    else if ((j -= n6) < ha::LARGE_NCLASSES) { v = P.lticket[j * ha::HOT_STRIDE]; }
+    else if ((j -= ha::LARGE_NCLASSES) < P.lwords) { v = P.lbits[j]; }
+    else { j -= P.lwords; v = P.lstart[j]; }
     out[i] = v;
   }
 }
 
 u64 dump_words_of(const ha::Params& P) {
   return (u64)P.nsbs * 3 + (u64)ha::NSETS * P.nsw + 3ull * P.nheads * ha::NSIZES +
-         (u64)P.nsbs * P.bit_words + (u64)P.nsbs * P.nib_words;
+         (u64)P.nsbs * P.bit_words + (u64)P.nsbs * P.nib_words + ha::LARGE_NCLASSES + P.lwords + P.lunits;
 }
 
 // ---------------------------------------------------------------- plain
@@ -250,6 +250,41 @@ __global__ void read_words_k(void* const* ptrs, int n, u32* out) {
   out[i] = ptrs[i] ? *(const u32*)ptrs[i] : 0u;
 }
 
+// ------------------------------------------ tst/corr/test/test.cu test 0
+// malloc_k (:47-55): thread t of nthreads allocates ntimes 16 B blocks at
+// ptrs[t], ptrs[t + nthreads], ... (a grid-stride loop)
+__global__ void t0_malloc_k(void** ptrs, int ntimes, unsigned long long* nulls) {
+  const u64 tid = blockIdx.x * (u64)blockDim.x + threadIdx.x, nt = (u64)blockDim.x * gridDim.x;
+  u32 nul = 0;
+  for (u64 i = tid; i < (u64)ntimes * nt; i += nt) {
+    void* p = hamalloc(16);
+    ptrs[i] = p;
+    nul += p == nullptr;
+  }
+  if (nul) atomicAdd(nulls, (unsigned long long)nul);
+}
+// check_ptrs_k (:84-95), first half: two u64 holding the block's own address
+__global__ void t0_write_k(void* const* ptrs, u64 n) {
+  const u64 i = blockIdx.x * (u64)blockDim.x + threadIdx.x;
+  if (i >= n || !ptrs[i]) return;
+  u64* p = (u64*)ptrs[i];
+  p[0] = (u64)p;
+  p[1] = (u64)p;
+}
+// read back after the whole grid has written: a block that overlaps another
+// holds the other's address
+__global__ void t0_read_k(void* const* ptrs, u64 n, unsigned long long* bad) {
+  const u64 i = blockIdx.x * (u64)blockDim.x + threadIdx.x;
+  if (i >= n || !ptrs[i]) return;
+  const u64* p = (const u64*)ptrs[i];
+  if (p[0] != (u64)p || p[1] != (u64)p) atomicAdd(bad, 1ull);
+}
+// free_k (:57-63)
+__global__ void t0_free_k(void* const* ptrs, int ntimes) {
+  const u64 tid = blockIdx.x * (u64)blockDim.x + threadIdx.x, nt = (u64)blockDim.x * gridDim.x;
+  for (u64 i = tid; i < (u64)ntimes * nt; i += nt) hafree(ptrs[i]);
+}
+
 // ------------------------------------------------------------------ probe
 __device__ __forceinline__ u32 mix32(u32 x) {
   x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
@@ -268,6 +303,26 @@ __global__ void probe_k(u64* buf, u64 mask, int iters, int shape, u64* sink) {
     acc ^= __hip_atomic_fetch_or(buf + w, 1ull << (lane & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   sink[tid] = acc;
+}
+
+// Dependent-chain probe: every wave makes `rounds` atomic round trips one
+// after another, as an allocator call does (head/ticket, probe, counter on
+// malloc; nibble, bits, counter on free).  Lane 0 issues each atomic on a
+// cache line picked from the previous result, so no round can start before
+// the one before it returns; the result reaches the wave through
+// readfirstlane.  The launch time over the number of rounds is the round-trip
+// latency with the whole chip waiting on memory.
+__global__ void __launch_bounds__(256) chain_k(u64* buf, u64 mask, int rounds, u64* sink) {
+  const u32 tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const u32 lane = threadIdx.x & 63, wave = tid >> 6;
+  u64 prev = 0;
+  for (int r = 0; r < rounds; ++r) {
+    const u64 w = ((u64)mix32(wave * 977u + (u32)r * 31u + (u32)(prev & 1u)) * 16u) & mask;
+    u64 v = 0;
+    if (lane == 0) v = __hip_atomic_fetch_or(buf + w, 1ull << (r & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    prev = (u64)__builtin_amdgcn_readfirstlane((int)(u32)v);
+  }
+  if (lane == 0) sink[wave] = prev;
 }
 
 // Streaming copy of n16 16-byte words, grid-stride: the known-size kernel
@@ -1285,10 +1340,49 @@ __global__ void t_phase_lat_k(TOpts o, void** ptrs, u32* ctrs, u32 try_id, unsig
 }
 
 // the stamped size of every non-NULL pointer (0 for NULL)
-__global__ void t_sizes_k(void* const* ptrs, size_t n, u32* out) {
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
-    out[k] = ptrs[k] ? *(const u32*)ptrs[k] : 0u;
+// check_alloc (tst/common/common.cu:412-529) on the device.  Every pointer an
+// active thread holds must be non-NULL and inside the pool, and its
+// [p, p + size) claims the 8-byte granules it covers in a bitmap over the
+// pool (size: the fixed request, or the size stamped in the block's first
+// word); a granule found claimed already is an overlap.  The reference sorts
+// the pointers and compares neighbours: both flag exactly the sets of held
+// blocks in which two overlap.  Block starts are 8-byte aligned (24 B chunks),
+// so blocks that do not overlap never share a granule.  st[0] counts the
+// blocks checked, st[1] the violations.
+__global__ void t_check_k(TOpts o, void* const* ptrs, const u32* ctrs, int with_ctrs, int stamped, u64* gran,
+                          const char* base, u64 pool_bytes, unsigned long long* st) {
+  const u64 idx = blockIdx.x * (u64)blockDim.x + threadIdx.x;
+  const u32 i = (u32)(idx % o.n), k = (u32)(idx / o.n);
+  const bool held = idx < (u64)o.n * o.nallocs && !t_inactive(o, i) && k < (with_ctrs ? ctrs[i] : o.nallocs);
+  bool bad = false;
+  if (held) {
+    const char* p = (const char*)ptrs[idx];
+    if (!p) {
+      bad = true;
+    } else {
+      const u64 sz = stamped ? *(const u32*)p : o.alloc_sz;
+      const u64 off = (u64)(p - base);
+      if (off >= pool_bytes || off + sz > pool_bytes) {
+        bad = true;
+      } else {
+        for (u64 g = off / 8, g1 = (off + sz + 7) / 8; g < g1;) {
+          const u64 w = g / 64, lo = g % 64, n = g1 - g < 64 - lo ? g1 - g : 64 - lo;
+          const u64 m = (n >= 64 ? ~0ull : ((1ull << n) - 1)) << lo;
+          if (atomicOr((unsigned long long*)(gran + w), (unsigned long long)m) & m) bad = true;
+          g += n;
+        }
+      }
+    }
+  }
+  // one counter update per wave
+  const u64 hm = __ballot(held), bm = __ballot(bad);
+  if ((threadIdx.x & 63) == 0) {
+    if (hm) atomicAdd(&st[0], (unsigned long long)__builtin_popcountll(hm));
+    if (bm) atomicAdd(&st[1], (unsigned long long)__builtin_popcountll(bm));
+  }
 }
+
+
 
 // ---------------------------------------------------------------- samples
 // samples/grid-points/grid-points.cu:61-133: every point gets a 16 B list
@@ -1530,8 +1624,8 @@ int ha_replay(const uint32_t* ops, int nsteps, int lanes, const uint32_t* iheads
   if (!ops || !recs || !xcc || nsteps < 0 || lanes < 1 || lanes > 64) return hfail(HA_E_INVALID, "bad replay arguments");
   const size_t nops = (size_t)nsteps * lanes;
   for (size_t i = 0; i < nops; ++i) {
-    if (ops[2 * i] > 2) return hfail(HA_E_INVALID, "bad op kind");
-    if (ops[2 * i] == 2 && (ops[2 * i + 1] >= nops || ops[2 * i + 1] / lanes >= i / lanes))
+    if (ops[2 * i] > 3) return hfail(HA_E_INVALID, "bad op kind");
+    if (ops[2 * i] >= 2 && (ops[2 * i + 1] >= nops || ops[2 * i + 1] / lanes >= i / lanes))
       return hfail(HA_E_INVALID, "a free must name a malloc of an earlier step");
   }
   DBuf dops, drecs, dxcc, dptrs, dih;
@@ -1638,6 +1732,54 @@ int ha_malloc_sizes(const uint32_t* sizes, int n, int bs, uint64_t* ptrs) {
   return HA_OK;
 }
 
+int ha_run_corr_test0(int nthreads, int nmallocs, int bs, ha_corr0_result* r) {
+  if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
+  if (!r || nthreads < 1 || nmallocs < 1 || bs < 64 || bs > 1024 || nthreads % bs ||
+      (u64)nthreads * (u64)nmallocs > (1ull << 28))
+    return hfail(HA_E_INVALID, "bad arguments");
+  std::memset(r, 0, sizeof *r);
+  const u64 n = (u64)nthreads * (u64)nmallocs;
+  DBuf dp, dc;
+  HH(dp.alloc(n * 8));
+  HH(dc.alloc(16));
+  HH(hipMemset(dp.p, 0, n * 8));
+  HH(hipMemset(dc.p, 0, 16));
+  unsigned long long* cnt = (unsigned long long*)dc.p;
+  hipEvent_t e1, e2;
+  HH(hipEventCreate(&e1));
+  HH(hipEventCreate(&e2));
+  t0_malloc_k<<<nthreads / bs, bs>>>((void**)dp.p, nmallocs, cnt);
+  HH(hipGetLastError());
+  const unsigned g = (unsigned)((n + 255) / 256);
+  t0_write_k<<<g, 256>>>((void* const*)dp.p, n);
+  HH(hipGetLastError());
+  t0_read_k<<<g, 256>>>((void* const*)dp.p, n, cnt + 1);
+  HH(hipGetLastError());
+  HH(hipDeviceSynchronize());
+  // sorted, consecutive blocks at least 16 B apart (test.cu:103-121)
+  std::vector<uint64_t> hp(n);
+  HH(hipMemcpy(hp.data(), dp.p, n * 8, hipMemcpyDeviceToHost));
+  std::sort(hp.begin(), hp.end());
+  for (u64 i = 0; i + 1 < n; ++i)
+    if (hp[i] && hp[i + 1] - hp[i] < 16) ++r->bad_gaps;
+  HH(hipEventRecord(e1, nullptr));
+  t0_free_k<<<nthreads / bs, bs>>>((void* const*)dp.p, nmallocs);
+  HH(hipGetLastError());
+  HH(hipEventRecord(e2, nullptr));
+  HH(hipDeviceSynchronize());
+  float ms = 0;
+  HH(hipEventElapsedTime(&ms, e1, e2));
+  r->free_ms = ms;
+  unsigned long long h[2];
+  HH(hipMemcpy(h, dc.p, 16, hipMemcpyDeviceToHost));
+  r->blocks = n;
+  r->nulls = h[0];
+  r->bad_contents = h[1];
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
+  return HA_OK;
+}
+
 int ha_free_ptrs(const uint64_t* ptrs, int n, int bs) {
   if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
   if (!ptrs || n < 0 || bs < 1 || bs > 1024) return hfail(HA_E_INVALID, "bad arguments");
@@ -1698,6 +1840,38 @@ int ha_probe_atomics(double* out) {
 }
 
 
+int ha_probe_chain(const int* rounds, int n, int nthreads, double* ms) {
+  if (!rounds || !ms || n < 1 || n > 64 || nthreads < 256 || nthreads > (1 << 24) || nthreads % 256)
+    return hfail(HA_E_INVALID, "bad arguments");
+  for (int i = 0; i < n; ++i)
+    if (rounds[i] < 0 || rounds[i] > 4096) return hfail(HA_E_INVALID, "rounds must be in 0..4096");
+  const size_t BYTES = 1ull << 30;
+  DBuf buf, sink;
+  HH(buf.alloc(BYTES));
+  HH(sink.alloc((size_t)nthreads / 64 * 8));
+  HH(hipMemset(buf.p, 0, BYTES));
+  hipEvent_t e0, e1;
+  HH(hipEventCreate(&e0));
+  HH(hipEventCreate(&e1));
+  for (int i = 0; i < n; ++i) {
+    float best = 1e30f;
+    for (int rep = 0; rep < 5; ++rep) {
+      HH(hipEventRecord(e0, 0));
+      chain_k<<<nthreads / 256, 256>>>((u64*)buf.p, BYTES / 8 - 1, rounds[i], (u64*)sink.p);
+      HH(hipEventRecord(e1, 0));
+      HH(hipEventSynchronize(e1));
+      float t = 0;
+      HH(hipEventElapsedTime(&t, e0, e1));
+      best = t < best ? t : best;
+    }
+    ms[i] = best;
+  }
+  HH(hipGetLastError());
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return HA_OK;
+}
+
 int ha_run_test(const ha_test_opts* a, ha_test_result* r) {
   if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
   if (!a || !r || a->test < 0 || a->test > 8 || a->nthreads < 1 || a->nthreads > (1 << 26) || a->bs < 1 ||
@@ -1714,7 +1888,7 @@ int ha_run_test(const ha_test_opts* a, ha_test_result* r) {
   o.distr = a->distr; o.period_mask = a->period_mask; o.group_sh = a->group_sh; o.seed = a->seed;
   for (int x = 0; x < 2; ++x)
     for (int y = 0; y < 2; ++y) o.p[x][y] = a->probabs[x][y];
-  DBuf dp, dc, ds, dsz, dl;
+  DBuf dp, dc, ds, dl;
   HH(dl.alloc(8 * 8));
   {
     const unsigned long long l0[8] = {0, 0, ~0ull, 0, 0, 0, ~0ull, 0};
@@ -1728,7 +1902,6 @@ int ha_run_test(const ha_test_opts* a, ha_test_result* r) {
   HH(hipMemset(dc.p, 0, (size_t)n * 4));
   HH(hipMemset(ds.p, 0, 3 * 8));
   const bool stamped = a->test == 0 || a->test == 1;
-  if (stamped) HH(dsz.alloc(nptrs * 4));
   Stream s;
   HH(hipStreamCreate(&s.s));
   hipEvent_t e0, e1;
@@ -1755,39 +1928,29 @@ int ha_run_test(const ha_test_opts* a, ha_test_result* r) {
   std::memset(r, 0, sizeof *r);
   r->failed_try = -1;
   std::vector<void*> hp(nptrs);
-  std::vector<u32> hc(n), hs(stamped ? nptrs : 0);
-  // check_alloc (tst/common/common.cu:412-529): every pointer an active
-  // thread holds is non-NULL, and sorted, each leaves room for its size
-  // before the next (the requested size when fixed, else the stamped one)
+  std::vector<u32> hc(n);
+  // check_alloc on the device (t_check_k): a granule bitmap over the pool,
+  // cleared before each check; only the two counts come back
+  const ha::Params& HP = ha::host_params();
+  const u64 pool_bytes = HP.arena_bytes + HP.large_bytes;
+  DBuf dgran, dchk;
+  if (stamped) {
+    HH(dgran.alloc((pool_bytes / 8 + 63) / 64 * 8));
+    HH(dchk.alloc(16));
+  }
   auto check = [&](bool with_ctrs) -> int {
-    HH(hipMemcpy(hp.data(), dp.p, nptrs * 8, hipMemcpyDeviceToHost));
-    if (with_ctrs) HH(hipMemcpy(hc.data(), dc.p, (size_t)n * 4, hipMemcpyDeviceToHost));
-    if (stamped) {
-      t_sizes_k<<<1024, 256, 0, s.s>>>((void* const*)dp.p, nptrs, (u32*)dsz.p);
-      HH(hipGetLastError());
-      HH(hipStreamSynchronize(s.s));
-      HH(hipMemcpy(hs.data(), dsz.p, nptrs * 4, hipMemcpyDeviceToHost));
-    }
-    std::vector<std::pair<uint64_t, u32>> live;
-    uint64_t bad = 0;
-    for (u32 i = 0; i < n; ++i) {
-      if (i >= n || ((i >> o.group_sh) & o.period_mask)) continue;
-      const u32 cnt = with_ctrs ? hc[i] : o.nallocs;
-      for (u32 k = 0; k < cnt; ++k) {
-        const size_t idx = i + (size_t)n * k;
-        if (!hp[idx]) {
-          ++bad;
-          continue;
-        }
-        const u32 sz = o.alloc_sz == o.max_alloc_sz ? o.alloc_sz : (stamped ? hs[idx] : 0u);
-        live.push_back({(uint64_t)hp[idx], sz});
-      }
-    }
-    std::sort(live.begin(), live.end());
-    for (size_t k = 0; k + 1 < live.size(); ++k)
-      if (live[k].first + live[k].second > live[k + 1].first) ++bad;
-    r->checked += live.size();
-    r->checks_failed += bad;
+    HH(hipMemsetAsync(dgran.p, 0, (pool_bytes / 8 + 63) / 64 * 8, s.s));
+    HH(hipMemsetAsync(dchk.p, 0, 16, s.s));
+    t_check_k<<<(unsigned)((nptrs + 255) / 256), 256, 0, s.s>>>(o, (void* const*)dp.p, (const u32*)dc.p,
+                                                              with_ctrs ? 1 : 0, o.alloc_sz != o.max_alloc_sz,
+                                                              (u64*)dgran.p, HP.arena, pool_bytes,
+                                                              (unsigned long long*)dchk.p);
+    HH(hipGetLastError());
+    unsigned long long c[2];
+    HH(hipMemcpyAsync(c, dchk.p, 16, hipMemcpyDeviceToHost, s.s));
+    HH(hipStreamSynchronize(s.s));
+    r->checked += c[0];
+    r->checks_failed += c[1];
     return HA_OK;
   };
   const unsigned grid = (n + (u32)a->bs - 1) / (u32)a->bs;

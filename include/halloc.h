@@ -16,8 +16,9 @@
 struct halloc_opts_t {
   /** total memory for allocation, bytes; capped at 3/4 of device memory */
   size_t memory;
-  /** fraction of memory given to the slab heap; the rest is the HIP device
-      heap that serves requests above 3072 bytes */
+  /** fraction of memory given to the slab heap; the rest is the large
+      region that serves requests above 3072 bytes (the reference gives it to
+      the CUDA device heap) */
   double halloc_fraction;
   /** occupancy fraction at which a slab is busy (head slab is replaced) */
   double busy_fraction;

@@ -68,23 +68,31 @@ int ha_free_bytes_c(uint64_t *out);
 const char *ha_last_error(void);
 
 /* Geometry of the initialized heap: nsbs, sb_sz, nheads, nsw, bit_words,
-   nib_words, and the 16 x 12 size table (nchunks chunk_sz chunk_id ncib sparse
-   hash_step roomy busy block_sz nblocks tmod chunk_bytes). */
+   nib_words, the 16 x 12 size table (nchunks chunk_sz chunk_id ncib sparse
+   hash_step roomy busy block_sz nblocks tmod chunk_bytes), the device address
+   of the slab arena (the large region of large_bytes follows its arena_bytes),
+   and the large region's 4 KiB units and classes. */
 typedef struct ha_geometry {
   uint32_t nsbs, sb_sz, sb_sz_sh, nheads, nsw, bit_words, nib_words, pad;
   uint64_t arena_bytes, large_bytes;
   uint32_t size_table[16 * 12];
+  uint64_t arena_base;
+  uint32_t large_units, large_classes;
 } ha_geometry;
 int ha_geometry_c(ha_geometry *g);
 
 /* ---------------------------------------------------------------- harness */
 
 /* Replay nsteps x lanes operations on ONE wavefront (lanes <= 64).
-   ops[2*i] = kind (0 none, 1 malloc, 2 free), ops[2*i+1] = bytes for malloc or
-   the index of the earlier malloc op for free.  Per step the malloc lanes call
-   hamalloc together, then the free lanes call hafree together.
-   recs[4*i..]: malloc -> size class (16 = large, 0xffffffff = NULL), slab,
-   chunk; free -> chunks freed, slab, chunk.  iheads (may be NULL) forces the
+   ops[2*i] = kind (0 none, 1 malloc, 2 free, 3 free of the pointer 4096 bytes
+   into the block: an interior pointer of a large block), ops[2*i+1] = bytes
+   for malloc or the index of the earlier malloc op for free (a second free of
+   the same op is a double free).  Per step the malloc lanes call hamalloc
+   together, then the free lanes call hafree together.
+   recs[4*i..]: malloc -> (size class, slab, chunk), a large block
+   (16 + its class, 0xfffffffe, 4 KiB unit), NULL (0xffffffff x 3); free ->
+   (chunks freed, slab, chunk) or (units freed, 0xfffffffe, unit), read before
+   the free; (0, 0xffffffff, 0xffffffff) when the free changes nothing.  iheads (may be NULL) forces the
    head set of each step's mallocs; otherwise the wave's XCD id picks it.
    xcc[step] receives the head-set index used (before the modulo nheads). */
 int ha_replay(const uint32_t *ops, int nsteps, int lanes, const uint32_t *iheads, uint32_t *recs,
@@ -126,6 +134,18 @@ int ha_malloc_sizes(const uint32_t *sizes, int n, int bs, uint64_t *ptrs);
 int ha_free_ptrs(const uint64_t *ptrs, int n, int bs);
 /* Reads back the first 32-bit word of each block (0 for NULL). */
 int ha_read_words(const uint64_t *ptrs, int n, uint32_t *out);
+
+/* tst/corr/test/test.cu test 0 (:84-132): nthreads threads (blocks of bs)
+   each allocate nmallocs 16 B blocks in the grid-stride order of its malloc_k
+   (:47-55); every block gets two u64 holding its own address, read back after
+   the whole grid wrote (bad_contents counts mismatches); sorted, consecutive
+   blocks must lie at least 16 B apart (bad_gaps); then free_k frees them all
+   (free_ms, HIP events). */
+typedef struct ha_corr0_result {
+  uint64_t blocks, nulls, bad_contents, bad_gaps;
+  double free_ms;
+} ha_corr0_result;
+int ha_run_corr_test0(int nthreads, int nmallocs, int bs, ha_corr0_result *r);
 
 /* Lists (tst/common/common.h:375-399, check_alloc tst/common/common.cu:468-529):
    lane i allocates sizes[j*n + i] for j < per (0 = none) and writes the size
@@ -312,6 +332,15 @@ int ha_probe_copy(uint64_t bytes, int reps, double *gbps);
    out[0] lane-random (one cache line per lane), out[1] wave-run (a wave
    covers 64 consecutive words), out[2] single hot word. */
 int ha_probe_atomics(double *out);
+
+/* Dependent-chain probe: nthreads lanes (a multiple of 256) in waves of 64;
+   each wave makes rounds[i] dependent 64-bit atomic round trips (the next
+   address depends on the previous result), lane 0 issuing, over a 1 GiB
+   buffer.  ms[i] = the best of 5 launch times (HIP events) for rounds[i].
+   The slope over rounds is the atomic round-trip latency with every wave of
+   the grid waiting on memory: the bound of a kernel that is a chain of
+   dependent atomics per wave, as each hamalloc / hafree call is. */
+int ha_probe_chain(const int *rounds, int n, int nthreads, double *ms);
 
 #ifdef __cplusplus
 }

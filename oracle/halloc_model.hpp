@@ -64,10 +64,30 @@
 //   * batched calls (hamalloc_batch): one ticket for all rounds, the first
 //     probes of 4 rounds made before any is checked, one counter update.
 //
-// Parity unpinned: the large-block path (requests above 3072 B) is not in the
-// model; the model records such a request as REC_LARGE.  Its device behaviour
-// is pinned by invariant tests only (alignment, disjointness, reuse, NULL on
-// exhaustion).
+//   * nblocks is rounded down to a multiple of 64 and then by one more run of
+//     64 when THREAD_FREQ (11) would divide the number of runs (first probes
+//     step 11 runs at a time and would reach only every 11th run).
+//   * a head in which a group's probes ran out of tries (31) is replaced even
+//     below its busy limit ("forced"); when no other slab exists it is kept,
+//     and the pair records the head and the count of slabs freed so far: a
+//     later forced replacement of that head keeps it without a search until
+//     a slab becomes free.
+//     The reference replaces a head only when it is missing or at its busy
+//     threshold (src/slab.cuh:385-387).
+//   * requests above 3072 B: the reference hands them to the CUDA device heap
+//     (src/halloc.cu:185-187, 233-235), which is not in its tree.  Here they
+//     come from the large region, the share of the pool outside the slabs
+//     (src/halloc.cu:285-289): class c = ceil(log2(units of 4 KiB)), a block of
+//     2^c units aligned to its size, one bit per unit, per unit the class + 1
+//     of the live block starting there.  A class group of a wave takes one
+//     ticket; blocks of up to 64 units are taken from the free fields of the
+//     words in order from large_start(ticket), the lane of rank r taking the
+//     r-th free field, staying on a word while it shows a free field; larger blocks
+//     take aligned runs of zero words lane by lane.  A free exchanges the start
+//     word with 0 and clears the block's bits only when it read a class, so a
+//     double free, an interior pointer or a unit never handed out is a no-op.
+//     The model states the field search with loops over aligned positions, the
+//     device with word-parallel bit arithmetic (ha_bits.h).
 #pragma once
 
 #include <atomic>
@@ -105,6 +125,11 @@ constexpr u32 MAXB = 16;            // most blocks one lane takes in one batched
 constexpr u32 HEAD_EXT = 0x80000000u;
 constexpr u32 MAX_STEALS = 4;       // idle heads one head search may free before it gives up
 constexpr u32 BATCH_PIPE = MAXB;    // rounds of a batched call whose first probes go out together (all)
+// large region
+constexpr u32 LARGE_UNIT_SH = 12;   // 4 KiB units
+constexpr u32 LARGE_NCLASSES = 40;
+constexpr u32 LARGE_SB = 0xfffffffeu;  // slab field of a large block's record
+constexpr u64 UNIT_NONE = ~0ull;
 
 // slab counter layout (64-bit)
 constexpr u32 CTR_CHUNK_SH = 8;
@@ -225,9 +250,23 @@ inline u32 sb_class(u32 count, const SizeInfo& si) {
 }
 
 // one record per replayed operation
+// malloc: (size class, slab, chunk), a large block (16 + class, LARGE_SB, unit),
+// NULL (REC_NULL x 3); free: (chunks or units freed, slab or LARGE_SB, chunk or unit)
 struct Rec { u32 a, sb, chunk, extra; };
-constexpr u32 REC_LARGE = 16;
 constexpr u32 REC_NULL = 0xffffffffu;
+
+// class of a large request: blocks of 2^c units of 4 KiB
+inline u32 large_class(u64 nbytes) {
+  u64 units = (nbytes >> LARGE_UNIT_SH) + ((nbytes & ((1ull << LARGE_UNIT_SH) - 1)) != 0);
+  u32 c = 0;
+  while ((1ull << c) < units) ++c;
+  return c;
+}
+// first word (or run of words) a group of class c with this ticket looks at
+inline u32 large_start(u32 ticket, u32 c, u32 n) {
+  u32 h = (ticket * 0x9E3779B1u) ^ (c * 0x85EBCA77u);
+  return (u32)(((u64)h * n) >> 32);
+}
 
 // -------------------------------------------------------------------- the heap
 class Heap {
@@ -242,10 +281,20 @@ class Heap {
   std::vector<std::atomic<u64>> bits, nibs, ctr, sets;
   std::vector<std::atomic<u32>> d_size, d_chunk, d_csz, d_head, sb_lock;
   std::vector<std::atomic<u32>> head, cached, head_lock, ticket, search_ep;
+  // per pair: the head whose forced replacement found no other slab, and the
+  // free-slab epoch at that time; free_ep counts slabs that became free
+  std::vector<std::atomic<u32>> forced_head, forced_ep;
+  std::atomic<u32> free_ep{0};
   std::atomic<u32> class_lock[NSIZES] = {};
   std::atomic<u32> active{0};  // pairs that hold a head
   std::vector<uint8_t> arena;  // payload memory (allocated lazily by users that touch data)
-  std::atomic<u64> large_ctr{0};
+  // large region: lbits one bit per unit (the bits past lunits set), lstart
+  // per unit the class + 1 of the live block starting there, a ticket per class
+  u32 lunits = 0, lwords = 0, lnclasses = 0;
+  u64 large_bytes = 0;
+  std::vector<std::atomic<u64>> lbits;
+  std::vector<std::atomic<u32>> lstart;
+  std::atomic<u32> lticket[LARGE_NCLASSES] = {};
 
   // set index helpers: 0 = free, 1+s = roomy[s], 17+s = busy[s], 33+c = sparse[c]
   static constexpr u32 NSETS = 1 + 2 * NSIZES + NCHUNK_IDS;
@@ -280,6 +329,8 @@ class Heap {
     head_lock = std::vector<std::atomic<u32>>((size_t)nheads * NSIZES);
     search_ep = std::vector<std::atomic<u32>>((size_t)nheads * NSIZES);
     ticket = std::vector<std::atomic<u32>>((size_t)nheads * NSIZES);
+    forced_head = std::vector<std::atomic<u32>>((size_t)nheads * NSIZES);
+    forced_ep = std::vector<std::atomic<u32>>((size_t)nheads * NSIZES);
     for (auto& x : bits) x.store(0, std::memory_order_relaxed);
     for (auto& x : nibs) x.store(0, std::memory_order_relaxed);
     for (auto& x : sets) x.store(0, std::memory_order_relaxed);
@@ -292,7 +343,22 @@ class Heap {
     for (size_t i = 0; i < head.size(); ++i) {
       head[i].store(SB_NONE); cached[i].store(SB_NONE); head_lock[i].store(0); ticket[i].store(TICKET_INIT);
       search_ep[i].store(0);
+      forced_head[i].store(SB_NONE); forced_ep[i].store(0);
     }
+    // the rest of the pool is the large region (src/halloc.cu:285-289)
+    const u64 large_memory = memory - halloc_memory;
+    lunits = (u32)std::min<u64>(large_memory >> LARGE_UNIT_SH, 0xfffffff0ull);
+    large_bytes = (u64)lunits << LARGE_UNIT_SH;
+    lwords = (lunits + 63) / 64;
+    while (lnclasses < LARGE_NCLASSES && (1ull << lnclasses) <= lunits) ++lnclasses;
+    lbits = std::vector<std::atomic<u64>>(lwords);
+    lstart = std::vector<std::atomic<u32>>(lunits);
+    for (u32 w = 0; w < lwords; ++w) {
+      u64 v = 0;
+      for (u32 b = 0; b < 64; ++b) if ((u64)w * 64 + b >= lunits) v |= 1ull << b;
+      lbits[w].store(v);
+    }
+    for (auto& x : lstart) x.store(0);
     if (with_arena) arena.assign((size_t)arena_bytes, 0);
     ok = true;
   }
@@ -336,6 +402,7 @@ class Heap {
       if (ctr_count(old) == 0) {
         d_size[sb].store(SZ_NONE); d_chunk[sb].store(SZ_NONE); d_csz[sb].store(0);
         set_add(0, sb);
+        free_ep.fetch_add(1);
         set_remove(set_sparse(chunk_id), sb);
         if (size_id != SZ_NONE) { set_remove(set_roomy(size_id), sb); set_remove(set_busy(size_id), sb); }
         return;
@@ -566,7 +633,10 @@ class Heap {
         u32 nh;
         // below: a forced replacement of a head that still has room
         const bool below = cur != SB_NONE && ctr_count(ctr[cur & ~HEAD_EXT].load()) < head_limit(cur, s);
-        if (cur == SB_NONE || !below || (force && cur == seen)) {
+        const u32 fe = free_ep.load();
+        if (below && forced_head[hi].load() == cur && forced_ep[hi].load() == fe) {
+          nh = cur;  // this head's forced search failed and no slab became free since
+        } else if (cur == SB_NONE || !below || (force && cur == seen)) {
           nh = cached[hi].load();
           cached[hi].store(SB_NONE);
           if (nh == SB_NONE) {
@@ -579,6 +649,8 @@ class Heap {
           }
           if (nh == SB_NONE && below) {
             nh = cur;  // no other slab: keep the head, which still has room
+            forced_ep[hi].store(fe);
+            forced_head[hi].store(cur);
           } else if (nh != SB_NONE) {
             head[hi].store(nh);
             if (cur == SB_NONE) active.fetch_add(1);
@@ -801,19 +873,132 @@ class Heap {
       if (pend[j]) malloc_group(ihead, size_id, pend[j], out + j * WAVE, rec ? rec + j * WAVE : nullptr);
   }
 
+  // ----------------------------------------------------------- large region
+  // One class group of large requests (halloc.hip large_group): lanes g want a
+  // block of class c.  out[l] = arena offset + 1 (0 = NULL).
+  void large_group(u32 c, u64 g, u64* out, Rec* rec) {
+    const u32 nw = lwords;
+    u32 old = lticket[c].fetch_add((u32)__builtin_popcountll(g));
+    u64 unit[WAVE];
+    for (u32 l = 0; l < WAVE; ++l) unit[l] = UNIT_NONE;
+    if (c <= 6) {
+      const u32 k = 1u << c;
+      const u64 fm = k == 64 ? ~0ull : ((1ull << k) - 1);  // one field's bits at position 0
+      u32 w = large_start(old, c, nw), moved = 0;
+      u64 v = lbits[w].load();
+      u64 pend = g;
+      while (pend) {
+        // free aligned fields of the known word value, lowest first
+        u32 freep[64], n = 0;
+        for (u32 p = 0; p < 64; p += k) if (((v >> p) & fm) == 0) freep[n++] = p;
+        if (n == 0) {
+          if (++moved >= nw) break;
+          w = w + 1 == nw ? 0 : w + 1;
+          v = lbits[w].load();
+          continue;
+        }
+        // the lane of rank r among the pending lanes takes field r
+        u32 lanes[64], np = 0;
+        for (u32 l = 0; l < WAVE; ++l) if (pend >> l & 1) lanes[np++] = l;
+        const u32 npick = np < n ? np : n;
+        u64 U = 0;
+        for (u32 r = 0; r < npick; ++r) U |= fm << freep[r];
+        const u64 o = lbits[w].fetch_or(U);
+        u64 back = 0, got = 0;
+        v = o;  // the word as it is now known: what the OR found, plus the fields won
+        for (u32 r = 0; r < npick; ++r) {
+          const u64 f = fm << freep[r];
+          if (o & f) {
+            back |= f & ~o;  // give back the bits set in a field found taken
+          } else {
+            unit[lanes[r]] = (u64)w * 64 + freep[r];
+            got |= 1ull << lanes[r];
+            v |= f;
+          }
+        }
+        if (back) lbits[w].fetch_and(~back);
+        pend &= ~got;
+      }
+    } else {
+      const u32 m = 1u << (c - 6);  // words per block
+      const u32 ngr = nw / m;
+      u32 r = 0;
+      for (u32 l = 0; l < WAVE; ++l) if (g >> l & 1) {
+        const u32 rank = r++;
+        if (ngr == 0) continue;
+        const u32 g0 = large_start(old + rank, c, ngr);
+        for (u32 i = 0; i < ngr && unit[l] == UNIT_NONE; ++i) {
+          const u32 grp = (g0 + i) % ngr;
+          const size_t base = (size_t)grp * m;
+          bool zero = true;
+          for (u32 j = 0; j < m && zero; ++j) zero = lbits[base + j].load() == 0;
+          if (!zero) continue;
+          u32 j = 0;
+          for (; j < m; ++j) {
+            u64 e = 0;
+            if (!lbits[base + j].compare_exchange_strong(e, ~0ull)) break;
+          }
+          if (j == m) unit[l] = (u64)base * 64;
+          else while (j--) lbits[base + j].store(0);
+        }
+      }
+    }
+    for (u32 l = 0; l < WAVE; ++l) if (g >> l & 1) {
+      if (unit[l] != UNIT_NONE) {
+        lstart[unit[l]].store(c + 1);
+        out[l] = arena_bytes + (unit[l] << LARGE_UNIT_SH) + 1;
+        if (rec) rec[l] = Rec{NSIZES + c, LARGE_SB, (u32)unit[l], 0};
+      } else {
+        out[l] = 0;
+        if (rec) rec[l] = Rec{REC_NULL, REC_NULL, REC_NULL, 0};
+      }
+    }
+  }
+
+  // hamalloc above 3072 B for the lanes `active` of one wave: one class group
+  // at a time, groups ordered by their lowest lane; a class past the region's
+  // largest gets NULL
+  void wave_large(u64 active, const u64* nbytes, u64* out, Rec* rec) {
+    u32 cl[WAVE] = {0};
+    u64 rem = 0;
+    for (u32 l = 0; l < WAVE; ++l) if (active >> l & 1) {
+      cl[l] = large_class(nbytes[l]);
+      if (cl[l] < lnclasses) {
+        rem |= 1ull << l;
+      } else {
+        out[l] = 0;
+        if (rec) rec[l] = Rec{REC_NULL, REC_NULL, REC_NULL, 0};
+      }
+    }
+    while (rem) {
+      const u32 l0 = (u32)__builtin_ctzll(rem);
+      u64 g = 0;
+      for (u32 l = 0; l < WAVE; ++l) if ((rem >> l & 1) && cl[l] == cl[l0]) g |= 1ull << l;
+      rem &= ~g;
+      large_group(cl[l0], g, out, rec);
+    }
+  }
+
+  // hafree of a large block at region offset loff (halloc.hip large_free)
+  void large_free(u64 loff) {
+    if (loff & ((1ull << LARGE_UNIT_SH) - 1)) return;
+    const u64 unit = loff >> LARGE_UNIT_SH;
+    const u32 st = lstart[unit].exchange(0);
+    if (st == 0) return;
+    const u32 c = st - 1;
+    for (u64 u = unit; u < unit + (1ull << c); ++u) lbits[u / 64].fetch_and(~(1ull << (u % 64)));
+  }
+
   // hamalloc_batch for the active lanes of one wave: lane l asks for cnt[l]
   // blocks of nbytes[l] (classes above 3072 B go to the device heap one by one)
   void wave_malloc_batch(u32 ihead, u64 active, const u32* nbytes, const u32* cnt, u64* out, Rec* rec) {
     u32 sid[WAVE] = {0};
-    u64 small = 0;
+    u64 small = 0, large = 0;
     for (u32 l = 0; l < WAVE; ++l) if (active >> l & 1) {
       u32 c = cnt[l] > MAXB ? MAXB : cnt[l];
       if (!c) continue;
       if (nbytes[l] > MAX_BLOCK_SZ) {
-        for (u32 j = 0; j < c; ++j) {
-          out[j * WAVE + l] = (1ull << 63) | (large_ctr.fetch_add(1) + 1);
-          if (rec) rec[j * WAVE + l] = Rec{REC_LARGE, 0, 0, 0};
-        }
+        large |= 1ull << l;
       } else {
         sid[l] = size_id_from_nbytes(nbytes[l]);
         small |= 1ull << l;
@@ -821,6 +1006,14 @@ class Heap {
     }
     u32 c2[WAVE];
     for (u32 l = 0; l < WAVE; ++l) c2[l] = cnt[l] > MAXB ? MAXB : cnt[l];
+    // large requests: round j serves the lanes that want more than j blocks
+    u64 nb64[WAVE];
+    for (u32 l = 0; l < WAVE; ++l) nb64[l] = nbytes[l];
+    for (u32 j = 0; j < MAXB; ++j) {
+      u64 a = 0;
+      for (u32 l = 0; l < WAVE; ++l) if ((large >> l & 1) && c2[l] > j) a |= 1ull << l;
+      if (a) wave_large(a, nb64, out + j * WAVE, rec ? rec + j * WAVE : nullptr);
+    }
     while (small) {
       u32 l0 = (u32)__builtin_ctzll(small);
       u64 g = 0;
@@ -833,16 +1026,18 @@ class Heap {
   // hamalloc for the active lanes of one wave; out[] = arena offset + 1 (0 = NULL)
   void wave_malloc(u32 ihead, u64 active, const u32* nbytes, u64* out, Rec* rec) {
     u32 sid[WAVE] = {0};
-    u64 small = 0;
+    u64 small = 0, large = 0;
+    u64 nb64[WAVE] = {0};
     for (u32 l = 0; l < WAVE; ++l) if (active >> l & 1) {
       if (nbytes[l] > MAX_BLOCK_SZ) {
-        out[l] = (1ull << 63) | (large_ctr.fetch_add(1) + 1);
-        if (rec) rec[l] = Rec{REC_LARGE, 0, 0, 0};
+        large |= 1ull << l;
+        nb64[l] = nbytes[l];
       } else {
         sid[l] = size_id_from_nbytes(nbytes[l]);
         small |= 1ull << l;
       }
     }
+    if (large) wave_large(large, nb64, out, rec);
     while (small) {
       u32 l0 = (u32)__builtin_ctzll(small);
       u64 g = 0;
@@ -855,16 +1050,27 @@ class Heap {
   // hafree for the active lanes of one wave (src/halloc.cu:211-256)
   void wave_free(u64 active, const u64* ptr, Rec* rec) {
     u32 sbl[WAVE] = {0};
-    u64 small = 0;
+    u64 small = 0, large = 0;
     for (u32 l = 0; l < WAVE; ++l) if (active >> l & 1) {
       if (rec) rec[l] = Rec{0, REC_NULL, REC_NULL, 0};
       u64 p = ptr[l];
-      if (p == 0 || (p >> 63)) continue;   // NULL, or a large block (device heap)
+      if (p == 0) continue;
       u64 off = p - 1;
-      if (off >= arena_bytes) continue;
+      if (off >= arena_bytes) {
+        if (off - arena_bytes < large_bytes) large |= 1ull << l;
+        continue;
+      }
       sbl[l] = (u32)(off >> sb_sz_sh);
       small |= 1ull << l;
     }
+    // large blocks: every record is read before any of the wave's frees
+    for (u32 l = 0; l < WAVE; ++l) if ((large >> l & 1) && rec) {
+      const u64 loff = ptr[l] - 1 - arena_bytes;
+      if (loff & ((1ull << LARGE_UNIT_SH) - 1)) continue;
+      const u32 st = lstart[loff >> LARGE_UNIT_SH].load();
+      if (st) rec[l] = Rec{1u << (st - 1), LARGE_SB, (u32)(loff >> LARGE_UNIT_SH), 0};
+    }
+    for (u32 l = 0; l < WAVE; ++l) if (large >> l & 1) large_free(ptr[l] - 1 - arena_bytes);
     while (small) {
       u32 l0 = (u32)__builtin_ctzll(small);
       u32 sb = sbl[l0];

@@ -4,9 +4,10 @@ Only tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of bench.py
 import this module.  It is the checker, never the thing measured or shipped:
 the product path (``halloc_amd``) never imports it.
 
-The model restates canonizer/halloc's small-allocation path (src/halloc.cu,
-src/slab.cuh, src/slab.h, src/sbset.*, src/size-info.h, src/utils.cu); see
-oracle/halloc_model.hpp for the function-by-function citations.
+The model restates canonizer/halloc's allocation path (src/halloc.cu,
+src/slab.cuh, src/slab.h, src/sbset.*, src/size-info.h, src/utils.cu) and the
+large region that replaces the reference's delegation to the CUDA device heap;
+see oracle/halloc_model.hpp for the function-by-function citations.
 """
 from __future__ import annotations
 
@@ -21,7 +22,7 @@ LIB_PATH = os.path.join(HERE, "build", "libhalloc_model.so")
 
 NSIZES = 16
 SZ_NONE = 0xFFFFFFFF
-REC_LARGE = 16
+LARGE_SB = 0xFFFFFFFE   # slab field of a large block's record (class field: 16 + its class)
 REC_NULL = 0xFFFFFFFF
 NSETS = 1 + 2 * NSIZES + 4
 MAXB = 16
@@ -79,6 +80,7 @@ def lib() -> ctypes.CDLL:
             ("hm_nsbs", u32, [vp]),
             ("hm_replay", ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp]),
             ("hm_replay_batch", ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp]),
+            ("hm_wave", ctypes.c_int, [vp, vp, vp, ctypes.c_int, vp, vp]),
             ("hm_dump_words", u64, [vp]),
             ("hm_dump", ctypes.c_int, [vp, vp]),
             ("hm_check_dump", i64, [P(HmOpts), vp, vp]),
@@ -148,6 +150,19 @@ class ModelHeap:
             raise ValueError(f"hm_replay_batch failed: {rc}")
         return recs
 
+    def wave(self, kinds, args):
+        """One wave step with explicit pointers: lanes of kind 1 malloc args[l]
+        bytes, then lanes of kind 2 free the pointer args[l] (a value returned
+        here earlier; 0 is NULL).  Returns (recs (lanes, 4), ptrs (lanes,))."""
+        k = np.ascontiguousarray(kinds, dtype=np.uint32)
+        a = np.ascontiguousarray(args, dtype=np.uint64)
+        lanes = len(k)
+        recs = np.zeros((lanes, 4), dtype=np.uint32)
+        ptrs = np.zeros(lanes, dtype=np.uint64)
+        if lib().hm_wave(self.h, k.ctypes.data, a.ctypes.data, lanes, recs.ctypes.data, ptrs.ctypes.data) != 0:
+            raise ValueError("hm_wave failed")
+        return recs, ptrs
+
     def malloc_threads(self, sizes, nthreads: int = 8) -> np.ndarray:
         """Mallocs of `sizes` in 64-lane waves raced over nthreads threads;
         returns arena offset + 1 per request (0 = NULL)."""
@@ -172,7 +187,7 @@ class ModelHeap:
 def check_dump(opts: HmOpts, dump: np.ndarray):
     """Invariants over a canonical dump; returns (total, per-invariant counts)."""
     d = np.ascontiguousarray(dump, dtype=np.uint64)
-    detail = np.zeros(5, dtype=np.int64)
+    detail = np.zeros(7, dtype=np.int64)
     tot = lib().hm_check_dump(ctypes.byref(opts), d.ctypes.data, detail.ctypes.data)
     return int(tot), detail
 

@@ -11,6 +11,7 @@
 #include "halloc_model.hpp"
 
 using namespace hamodel;
+using u8 = uint8_t;
 
 extern "C" {
 
@@ -21,7 +22,9 @@ struct hm_opts_t {
   int32_t sb_sz_sh, nheads, hash_mode, pad;
   double hash_step_frac;
 };
-struct hm_op_t { uint32_t kind, arg; };  // kind 0 none, 1 malloc(arg = bytes), 2 free(arg = op index)
+// kind 0 none, 1 malloc(arg = bytes), 2 free(arg = op index), 3 free of the
+// pointer 4096 bytes into op arg's block (an interior pointer of a large block)
+struct hm_op_t { uint32_t kind, arg; };
 
 static Opts to_opts(const hm_opts_t* o) {
   Opts r;
@@ -84,9 +87,11 @@ int hm_replay(void* hv, const hm_op_t* ops, const uint32_t* iheads, int nsteps, 
     std::memset(mrec, 0, sizeof mrec); std::memset(frec, 0, sizeof frec);
     for (int l = 0; l < lanes; ++l) {
       if (row[l].kind == 1) { am |= 1ull << l; nbytes[l] = row[l].arg; }
-      else if (row[l].kind == 2) {
+      else if (row[l].kind == 2 || row[l].kind == 3) {
         if ((size_t)row[l].arg >= (size_t)st * lanes) return -2;  // must name an earlier op
         fm |= 1ull << l; fptr[l] = ptrs[row[l].arg];
+        // kind 3: a pointer one large unit (4 KiB) into the block
+        if (row[l].kind == 3 && fptr[l]) fptr[l] += 1ull << LARGE_UNIT_SH;
       }
     }
     if (am) h->wave_malloc(iheads ? iheads[st] : 0, am, nbytes, outp, mrec);
@@ -101,6 +106,31 @@ int hm_replay(void* hv, const hm_op_t* ops, const uint32_t* iheads, int nsteps, 
       ptrs[i] = 0;
       std::memcpy(recs + 4 * i, &frec[l], 16);
     }
+  }
+  return 0;
+}
+
+// One wave step with explicit pointers: the lanes with kind 1 call malloc of
+// arg bytes together, then the lanes with kind 2 free the pointer arg (arena
+// offset + 1, as returned in ptrs; 0 is NULL).  recs: 4 u32 per lane;
+// ptrs[l] receives lane l's malloc result.
+int hm_wave(void* hv, const uint32_t* kinds, const uint64_t* args, int lanes, uint32_t* recs, uint64_t* ptrs) {
+  Heap* h = (Heap*)hv;
+  if (lanes < 1 || lanes > 64) return -1;
+  u64 am = 0, fm = 0;
+  u32 nbytes[WAVE] = {0};
+  u64 outp[WAVE] = {0}, fptr[WAVE] = {0};
+  Rec mrec[WAVE], frec[WAVE];
+  std::memset(mrec, 0, sizeof mrec); std::memset(frec, 0, sizeof frec);
+  for (int l = 0; l < lanes; ++l) {
+    if (kinds[l] == 1) { am |= 1ull << l; nbytes[l] = (u32)args[l]; }
+    else if (kinds[l] == 2) { fm |= 1ull << l; fptr[l] = args[l]; }
+  }
+  if (am) h->wave_malloc(0, am, nbytes, outp, mrec);
+  if (fm) h->wave_free(fm, fptr, frec);
+  for (int l = 0; l < lanes; ++l) {
+    ptrs[l] = (am >> l & 1) ? outp[l] : 0;
+    std::memcpy(recs + 4 * l, (am >> l & 1) ? &mrec[l] : &frec[l], 16);
   }
   return 0;
 }
@@ -221,11 +251,14 @@ int hm_replay_batch(void* hv, const uint32_t* ops, const uint32_t* iheads, int n
 //   [3*nheads*16]     heads, cached heads, tickets
 //   [nsbs*bit_words]  bit arrays
 //   [nsbs*nib_words]  allocation-size nibbles
-uint64_t hm_dump_words(void* hv) {
-  Heap* h = (Heap*)hv;
-  return (u64)h->nsbs * 3 + (u64)Heap::NSETS * h->nsw + 3ull * h->nheads * NSIZES +
-         (u64)h->nsbs * h->bit_words + (u64)h->nsbs * h->nib_words;
+//   [LARGE_NCLASSES]  large-region tickets
+//   [lwords]          large-region unit bits
+//   [lunits]          large-region start words (class + 1 of the block starting there)
+static u64 dump_words(const Heap& h) {
+  return (u64)h.nsbs * 3 + (u64)Heap::NSETS * h.nsw + 3ull * h.nheads * NSIZES + (u64)h.nsbs * h.bit_words +
+         (u64)h.nsbs * h.nib_words + LARGE_NCLASSES + h.lwords + h.lunits;
 }
+uint64_t hm_dump_words(void* hv) { return dump_words(*(Heap*)hv); }
 int hm_dump(void* hv, uint64_t* out) {
   Heap* h = (Heap*)hv;
   u64* p = out;
@@ -240,6 +273,9 @@ int hm_dump(void* hv, uint64_t* out) {
   for (auto& x : h->ticket) *p++ = x.load();
   for (auto& x : h->bits) *p++ = x.load();
   for (auto& x : h->nibs) *p++ = x.load();
+  for (auto& x : h->lticket) *p++ = x.load();
+  for (auto& x : h->lbits) *p++ = x.load();
+  for (auto& x : h->lstart) *p++ = x.load();
   return 0;
 }
 
@@ -250,11 +286,17 @@ int hm_dump(void* hv, uint64_t* out) {
 //   4 every non-head slab with count > 0 is in a set (busy/roomy/sparse)
 //   5 a slab's head references (counter and descriptor) equal the number of
 //     head and cached-head words naming it
-// returns the number of violations; detail[0..4] counts per invariant
+//   6 non-head slabs sit in exactly the set their count names (free, or
+//     sparse / roomy / busy of their class), and in no other
+//   7 large region: every live block (a nonzero start word) is aligned to
+//     its size, inside the region, its units' bits set and not overlapping
+//     another live block; every set bit inside the region belongs to a live
+//     block; the bits past the region are set
+// returns the number of violations; detail[0..6] counts per invariant
 int64_t hm_check_dump(const hm_opts_t* o, const uint64_t* d, int64_t* detail) {
   Heap h(to_opts(o));
   if (!h.ok) return -1;
-  for (int i = 0; i < 5; ++i) detail[i] = 0;
+  for (int i = 0; i < 7; ++i) detail[i] = 0;
   const u64* ctr = d;
   const u64* desc = ctr + h.nsbs;
   const u64* sets = desc + 2ull * h.nsbs;
@@ -306,8 +348,39 @@ int64_t hm_check_dump(const hm_opts_t* o, const uint64_t* d, int64_t* detail) {
     u32 dref = (u32)(desc[2 * b + 1] >> 32);
     if (ctr_head_refs(ctr[b]) != refs[b] || dref != refs[b]) detail[4]++;
   }
+  // 6: set membership of non-head slabs by their count
+  for (u32 b = 0; b < h.nsbs; ++b) {
+    if (refs[b] || (desc[2 * b + 1] >> 32)) continue;
+    const u32 size_id = (u32)desc[2 * b], chunk_id = (u32)(desc[2 * b] >> 32);
+    const u32 count = ctr_count(ctr[b]);
+    u32 want = 0xffffffffu;  // the one set the slab belongs in
+    if (ctr_chunk(ctr[b]) == CTR_CHUNK_NONE) want = 0;
+    else if (size_id < NSIZES && chunk_id < NCHUNK_IDS) {
+      const u32 cls = sb_class(count, h.si[size_id]);
+      want = cls == CLS_FREE ? 0xffffffffu : h.class_set(cls, size_id, chunk_id);
+    }
+    for (u32 set = 0; set < Heap::NSETS; ++set)
+      if (in_set(set, b) != (set == want)) { detail[5]++; break; }
+  }
+  // 7: the large region
+  const u64* lbits = nibs + (u64)h.nsbs * h.nib_words + LARGE_NCLASSES;
+  const u64* lstart = lbits + h.lwords;
+  std::vector<u8> owned((size_t)h.lwords * 64, 0);
+  for (u32 u = 0; u < h.lunits; ++u) {
+    if (!lstart[u]) continue;
+    const u64 c = lstart[u] - 1, k = 1ull << c;
+    if (c >= h.lnclasses || u % k || u + k > h.lunits) { detail[6]++; continue; }
+    for (u64 x = u; x < u + k; ++x) {
+      if (owned[x] || !((lbits[x / 64] >> (x % 64)) & 1)) detail[6]++;
+      owned[x] = 1;
+    }
+  }
+  for (u64 x = 0; x < (u64)h.lwords * 64; ++x) {
+    const bool bit = (lbits[x / 64] >> (x % 64)) & 1;
+    if (x >= h.lunits ? !bit : (bit && !owned[x])) detail[6]++;
+  }
   int64_t tot = 0;
-  for (int i = 0; i < 5; ++i) tot += detail[i];
+  for (int i = 0; i < 7; ++i) tot += detail[i];
   return tot;
 }
 

@@ -140,3 +140,53 @@ def serve_checksums(tag, produced):
     h = _hash3(np.repeat(tag[nz], cnt), t.astype(np.uint32), np.uint32(0x5E7A11C3)).astype(np.uint64)
     out[nz] = (np.add.reduceat(h, starts) & 0xFFFFFFFF).astype(np.uint32)
     return out
+
+
+# --------------------------------------------------------- large region
+LARGE_UNIT = 4096
+LARGE_NCLASSES = 40
+LARGE_SB = 0xFFFFFFFE
+
+
+def large_units(memory, halloc_fraction=0.75):
+    """4 KiB units of the large region: the pool outside the slab share."""
+    return min((memory - int(halloc_fraction * memory)) // LARGE_UNIT, 0xFFFFFFF0)
+
+
+def large_class_of(nbytes):
+    """Class of a large request: the least c with 2^c units of 4 KiB >= nbytes."""
+    units = -(-int(nbytes) // LARGE_UNIT)
+    return (units - 1).bit_length()
+
+
+def large_state(dump, lunits):
+    """(tickets, unit bits, start words) of the large region: the last words of a dump."""
+    lwords = (lunits + 63) // 64
+    d = np.asarray(dump)
+    lstart = d[len(d) - lunits:]
+    lbits = d[len(d) - lunits - lwords:len(d) - lunits]
+    tick = d[len(d) - lunits - lwords - LARGE_NCLASSES:len(d) - lunits - lwords]
+    return tick, lbits, lstart
+
+
+def unit_used(lbits, u):
+    return bool((int(lbits[u // 64]) >> (u % 64)) & 1)
+
+
+def live_large_blocks(ops, recs):
+    """(first unit, units) of the large blocks live after a replay (kind 2 or 3 frees
+    by record: only a free whose record names units takes a block away)."""
+    nsteps, lanes = ops.shape[:2]
+    alive = {}
+    for st in range(nsteps):
+        for l in range(lanes):
+            i = st * lanes + l
+            k, a = ops[st, l]
+            r = recs[st, l]
+            if k == 1 and r[0] != REC_NULL and r[0] >= 16:
+                alive[i] = (int(r[2]), 1 << (int(r[0]) - 16))
+            elif k in (2, 3) and r[1] == LARGE_SB:
+                for j, (u, n) in list(alive.items()):
+                    if u == int(r[2]):
+                        del alive[j]
+    return list(alive.values())

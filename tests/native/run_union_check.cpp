@@ -129,6 +129,59 @@ int main() {
     const u32 n = 1u + (u32)(rng() % 0xFFFFFFFFull), x = (u32)rng();
     CHECK(ha::mod_by(x, n, ha::mod_magic(n)) == x % n);
   }
+  // unit fields of the large region: free / used aligned fields, widening,
+  // select and lowest-n against loops over the aligned positions
+  for (int t = 0; t < 400000; ++t) {
+    const u32 k = 1u << (rng() % 7);
+    u64 v = rng();
+    if (t % 3 == 0) v &= rng();  // sparser words
+    if (t % 5 == 0) v |= rng();  // denser words
+    if (t % 97 == 0) v = 0;
+    const u64 fm = k == 64 ? ~0ull : ((1ull << k) - 1);
+    u64 want_free = 0, want_used = 0;
+    for (u32 p = 0; p < 64; p += k) {
+      if (((v >> p) & fm) == 0) want_free |= 1ull << p;
+      else want_used |= 1ull << p;
+    }
+    CHECK(ha::free_fields(v, k) == want_free);
+    CHECK(ha::used_fields(v, k) == want_used);
+    u64 wide = 0;
+    for (u32 p = 0; p < 64; p += k)
+      if (want_free >> p & 1) wide |= fm << p;
+    CHECK(ha::widen_fields(want_free, k) == wide);
+    const u64 x = rng() & rng();
+    u32 r = 0;
+    for (u32 b = 0; b < 64; ++b)
+      if (x >> b & 1) CHECK(ha::select_bit(x, r++) == b);
+    const u32 n = (u32)(rng() % 66);
+    u64 low = 0;
+    u32 cnt = 0;
+    for (u32 b = 0; b < 64 && cnt < n; ++b)
+      if (x >> b & 1) { low |= 1ull << b; ++cnt; }
+    CHECK(ha::lowest_bits(x, n) == low);
+  }
+  // the largest block a region's words can hand out, against a scan of every
+  // aligned block of every class
+  for (int t = 0; t < 3000; ++t) {
+    const u32 nw = 1 + (u32)(rng() % 40), ncl = 1 + (u32)(rng() % 12);
+    u64 lb[40];
+    for (u32 w = 0; w < nw; ++w) {
+      lb[w] = rng() % 4 == 0 ? 0ull : (rng() & rng() & rng());
+      if (rng() % 6 == 0) lb[w] = ~0ull;
+    }
+    u64 want = 0;
+    for (u32 c = 0; c < ncl; ++c) {
+      const u64 k = 1ull << c;
+      bool found = false;
+      for (u64 u = 0; u + k <= (u64)nw * 64 && !found; u += k) {
+        bool z = true;
+        for (u64 x2 = u; x2 < u + k && z; ++x2) z = !((lb[x2 / 64] >> (x2 % 64)) & 1);
+        found = z;
+      }
+      if (found) want = k;
+    }
+    CHECK(ha::large_max_units(lb, nw, ncl) == want);
+  }
   if (fails) {
     std::printf("%d failures\n", fails);
     return 1;

@@ -42,12 +42,30 @@ def test_drop_in_host_api_exported(built):
         assert mangled in out, mangled
 
 
-def test_rdc_archive_holds_device_api(built):
-    # libhalloc.a carries hamalloc / hafree as relocatable device code for gfx950
+LLVM = "/opt/rocm/lib/llvm/bin"
+
+
+def test_rdc_archive_holds_device_api(built, tmp_path):
+    # libhalloc.a carries the host API and, as relocatable gfx950 device code
+    # (LLVM bitcode in the object's offload bundle), the device definitions of
+    # hamalloc(size_t), hafree(void*) and the batched calls that user code
+    # built with -fgpu-rdc links against
     archive = os.path.join(ROOT, "halloc_amd", "lib", "libhalloc.a")
     assert os.path.exists(archive)
     out = subprocess.run([NM, archive], capture_output=True, text=True).stdout
     assert "_Z7ha_init13halloc_opts_t" in out
+    subprocess.run([f"{LLVM}/llvm-ar", "x", archive, "halloc.o"], cwd=tmp_path, check=True)
+    targets = subprocess.run([f"{LLVM}/clang-offload-bundler", "--type=o", "--input=halloc.o", "--list"],
+                             cwd=tmp_path, capture_output=True, text=True, check=True).stdout.split()
+    dev = [t for t in targets if t.endswith("gfx950")]
+    assert dev, targets
+    subprocess.run([f"{LLVM}/clang-offload-bundler", "--type=o", "--input=halloc.o", f"--targets={dev[0]}",
+                    "--output=dev.bc", "--unbundle"], cwd=tmp_path, check=True)
+    ir = subprocess.run([f"{LLVM}/llvm-dis", "dev.bc", "-o", "-"], cwd=tmp_path, capture_output=True, text=True,
+                        check=True).stdout
+    defined = set(re.findall(r"^define [^@]*@(\S+?)\(", ir, flags=re.M))
+    for sym in ("_Z8hamallocm", "_Z6hafreePv", "_Z14hamalloc_batchmiPPv", "_Z12hafree_batchPKPvi"):
+        assert sym in defined, sym
 
 
 def test_default_options_match_reference(built):

@@ -13,8 +13,9 @@ import pytest
 
 import halloc_amd as H
 from oracle import model as M
-from tests.helpers import (BLOCK_SIZES, REC_NULL, assert_disjoint, live_intervals, pressure_program,
-                           graph_sizes, random_batch_ops, random_ops, sweep_sizes, wave_refree_ops)
+from tests.helpers import (BLOCK_SIZES, LARGE_SB, REC_NULL, assert_disjoint, graph_sizes, large_class_of,
+                           large_state, large_units, live_intervals, pressure_program, random_batch_ops, random_ops,
+                           sweep_sizes, wave_refree_ops)
 
 pytestmark = pytest.mark.gpu
 
@@ -162,11 +163,67 @@ def test_replay_shared_and_stolen_heads(halloc):
     assert ((d[:nsbs] >> np.uint64(16)) & np.uint64(0xFF)).max() > 1
 
 
-def test_replay_large_requests(halloc):
-    say("replay large")
+def large_program(rng, nsteps, lanes):
+    """Mixed program over both paths: mallocs of slab and large sizes (up to
+    4 MiB blocks), frees, and frees of interior pointers of live large blocks
+    (op kind 3, 4 KiB into the block).  Then, with no malloc in between, every
+    live block is freed, freed again and its interior pointer freed: the
+    second and third free of each block change nothing."""
+    sizes = BLOCK_SIZES[::3] + [3073, 4096, 5000, 9000, 17000, 40000, 65536, 70000, 300000, 1 << 20, 4 << 20]
+    ops = np.zeros((nsteps, lanes, 2), dtype=np.uint32)
+    live = []
+    for st in range(nsteps):
+        new = []
+        for l in range(lanes):
+            u = rng.random()
+            if live and u < 0.3:
+                ops[st, l] = (2, live.pop(int(rng.integers(len(live)))))
+            elif live and u < 0.38:
+                big = [j for j in live if ops.reshape(-1, 2)[j][1] > 4096]
+                if big:
+                    ops[st, l] = (3, big[int(rng.integers(len(big)))])
+            else:
+                ops[st, l] = (1, int(rng.choice(sizes)))
+                new.append(st * lanes + l)
+        live.extend(new)
+    rounds = -(-len(live) // lanes)
+    tail = np.zeros((3 * rounds, lanes, 2), dtype=np.uint32)
+    for kind_i, kind in enumerate((2, 2, 3)):
+        for j, op in enumerate(live):
+            tail[kind_i * rounds + j // lanes, j % lanes] = (kind, op)
+    return np.concatenate([ops, tail]), nsteps + rounds
+
+
+@pytest.mark.parametrize("lanes", [8, 64])
+def test_replay_large_requests(halloc, lanes):
+    # requests above 3072 B from the large region, replayed bit-exactly with
+    # their (class, unit) records and the whole state, including interior
+    # pointers and double frees (both no-ops)
+    say(f"replay large lanes={lanes}")
     opts = halloc(H.HallocOpts(memory=256 << 20, sb_sz_sh=21, nheads=1))
-    rng = np.random.default_rng(11)
-    ops = random_ops(rng, 200, 8, BLOCK_SIZES, large_every=10)
+    rng = np.random.default_rng(11 + lanes)
+    ops, refree = large_program(rng, 300 if lanes == 8 else 60, lanes)
+    recs_g, recs_m, heap = replay_both(opts, ops)
+    big = (ops[..., 0] == 1) & (recs_g[..., 1] == LARGE_SB)
+    say(f"large blocks={int(big.sum())} interior frees={int((ops[..., 0] == 3).sum())}")
+    assert big.sum() > 50
+    assert (recs_g[ops[..., 0] == 3][:, 0] == 0).all()   # an interior pointer frees nothing
+    assert (recs_g[refree:][ops[refree:, :, 0] != 0][:, 0] == 0).all()  # nor a second free
+    assert (recs_g == recs_m).all()
+    assert_same_state(heap, opts)
+    # everything came back
+    g = H.Halloc.geometry()
+    d = H.Halloc.dump()
+    assert ((d[:g["nsbs"]] >> np.uint64(32)) == 0).all()
+    _, lbits, lstart = large_state(d, g["large_units"])
+    assert not lstart.any() and sum(bin(int(w)).count("1") for w in lbits) == len(lbits) * 64 - g["large_units"]
+
+
+def test_replay_large_random_sizes_one_lane(halloc):
+    say("replay large one lane")
+    opts = halloc(H.HallocOpts(memory=64 << 20, sb_sz_sh=21, nheads=1))
+    rng = np.random.default_rng(99)
+    ops = random_ops(rng, 3000, 1, BLOCK_SIZES, large_every=3)
     recs_g, recs_m, heap = replay_both(opts, ops)
     assert (recs_g == recs_m).all()
     assert_same_state(heap, opts)
@@ -381,36 +438,81 @@ def test_autotune_small_space():
 
 
 def _large_block(n):
-    """Bytes of the large-path block that serves a request of n bytes."""
-    units = (int(n) + 4095) // 4096
-    return 4096 << max(0, (units - 1).bit_length())
+    """Bytes of the large-region block that serves a request of n bytes."""
+    return 4096 << large_class_of(n)
 
 
 def test_large_blocks_reuse_and_exhaustion(halloc):
-    # requests above 3 KiB: 4 KiB-aligned power-of-two blocks from the region
-    # outside the slabs; freed blocks come back on reuse; running out gives
-    # NULL, never an overlap
+    # requests above 3 KiB: blocks of 4 KiB << c aligned to their size, from
+    # the region outside the slabs; a kernel that only allocates gets NULL
+    # only when no aligned run of free units is left; freed units serve any
+    # class again
     say("large blocks")
-    halloc(H.HallocOpts(memory=64 << 20))
+    opts = halloc(H.HallocOpts(memory=64 << 20))
     g = H.Halloc.geometry()
-    assert g["large_bytes"] == (64 << 20) - int(0.75 * (64 << 20))  # the share outside the slabs
+    lunits = large_units(64 << 20)
+    assert g["large_bytes"] == lunits * 4096 == (64 << 20) - int(0.75 * (64 << 20))
+    base = g["arena_base"] + g["arena_bytes"]
+    assert g["large_units"] == lunits
     rng = np.random.default_rng(12)
     sizes = rng.integers(3073, 65537, 160).astype(np.uint32)
     p1 = H.Halloc.malloc_sizes(sizes)
-    assert (p1 != 0).all() and (p1 % 4096 == 0).all()
-    assert (H.Halloc.read_words(p1) == sizes).all()
+    assert (p1 != 0).all()
     ext = np.array([_large_block(s) for s in sizes], dtype=np.uint64)
+    assert ((p1 - np.uint64(base)) % ext == 0).all()            # aligned to the block size
+    assert (H.Halloc.read_words(p1) == sizes).all()
     o = np.argsort(p1)
     assert (p1[o][:-1] + ext[o][:-1] <= p1[o][1:]).all()
     H.Halloc.free_ptrs(p1)
-    p2 = H.Halloc.malloc_sizes(sizes)
-    assert set(p2.tolist()) == set(p1.tolist())
-    H.Halloc.free_ptrs(p2)
-    big = np.full(400, 65536, dtype=np.uint32)
-    p3 = H.Halloc.malloc_sizes(big)
-    got = p3[p3 != 0]
-    assert 0 < len(got) < len(big)
-    assert len(got) * 65536 <= g["large_bytes"]
-    s3 = np.sort(got)
-    assert (s3[:-1] + 65536 <= s3[1:]).all()
-    H.Halloc.free_ptrs(p3)
+    # every unit as a 4 KiB block: exactly lunits succeed
+    p4 = H.Halloc.malloc_sizes(np.full(lunits + 300, 4096, dtype=np.uint32))
+    assert int((p4 != 0).sum()) == lunits
+    H.Halloc.free_ptrs(p4)
+    # the same units as 64 KiB blocks (class 4), then 1 MiB blocks (class 8)
+    for n, k in ((65536, 16), (1 << 20, 256)):
+        p = H.Halloc.malloc_sizes(np.full(lunits // k + 40, n, dtype=np.uint32))
+        got = np.sort(p[p != 0])
+        assert len(got) == lunits // k, (n, len(got))
+        assert (got[:-1] + np.uint64(n) <= got[1:]).all()
+        H.Halloc.free_ptrs(p)
+    d = H.Halloc.dump()
+    _, lbits, lstart = large_state(d, lunits)
+    assert not lbits.any() and not lstart.any()
+    assert M.check_dump(model_opts(opts), d)[0] == 0
+
+
+def test_large_double_and_interior_frees_are_noops_under_load(halloc):
+    # a grid frees half of its large blocks twice and hands interior and
+    # unaligned pointers of the other half to hafree: the other half keeps its
+    # contents, later blocks never overlap it, and the region's invariants hold
+    say("large double and interior frees, 1 GiB")
+    opts = halloc(H.HallocOpts(memory=1 << 30))
+    lunits = large_units(1 << 30)
+    rng = np.random.default_rng(13)
+    sizes = rng.integers(3073, 300000, 500).astype(np.uint32)   # about 110 MiB of the 256 MiB region
+    p = H.Halloc.malloc_sizes(sizes)
+    ok = p != 0
+    assert ok.all(), int((~ok).sum())
+    half = np.zeros(len(p), dtype=bool)
+    half[::2] = True
+    gone, keep = p[half & ok], p[~half & ok]
+    H.Halloc.free_ptrs(gone)
+    H.Halloc.free_ptrs(gone)                                    # double free
+    big = keep[sizes[~half & ok] > 4096]
+    H.Halloc.free_ptrs(np.concatenate([big + np.uint64(4096), keep + np.uint64(512)]))
+    assert (H.Halloc.read_words(keep) == sizes[~half & ok]).all()
+    d = H.Halloc.dump()
+    tot, det = M.check_dump(model_opts(opts), d)
+    assert tot == 0, det
+    _, lbits, lstart = large_state(d, lunits)
+    assert int((lstart != 0).sum()) == len(keep)
+    more = rng.integers(3073, 300000, 300).astype(np.uint32)
+    q = H.Halloc.malloc_sizes(more)
+    allp = np.concatenate([keep, q[q != 0]])
+    alls = np.concatenate([[_large_block(s) for s in sizes[~half & ok]], [_large_block(s) for s in more[q != 0]]])
+    o = np.argsort(allp)
+    assert (allp[o][:-1] + alls[o][:-1].astype(np.uint64) <= allp[o][1:]).all()
+    assert (H.Halloc.read_words(keep) == sizes[~half & ok]).all()
+    H.Halloc.free_ptrs(np.concatenate([keep, q]))
+    _, lbits, lstart = large_state(H.Halloc.dump(), lunits)
+    assert not lstart.any() and sum(bin(int(w)).count("1") for w in lbits) == len(lbits) * 64 - lunits

@@ -65,6 +65,15 @@ def test_max_prime_below_matches_sieve():
         assert L.hm_max_prime_below(n, nb) == want, (n, nb)
 
 
+def test_golden_size_table_is_reproducible():
+    # the committed table is what its generator (an independent restatement
+    # of src/halloc.cu:363-394 and src/utils.cu:8-24) writes
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(GOLDEN, "gen_size_table.py"), "--check"])
+    assert r.returncode == 0
+
+
 def test_reference_size_table_matches_golden():
     # the reference's own table at 4 MiB slabs (src/halloc.cu:363-394)
     gold = json.load(open(os.path.join(GOLDEN, "size_table_ref_sb22.json")))["rows"]
