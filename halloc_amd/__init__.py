@@ -20,7 +20,9 @@ import numpy as np
 __all__ = ["HallocOpts", "Halloc", "HallocError", "lib", "LIB_PATH"]
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libhalloc_amd.so")
+# HALLOC_AMD_LIB selects another build of the library (libhalloc_amd_ool.so:
+# the out-of-line entry points)
+LIB_PATH = os.environ.get("HALLOC_AMD_LIB") or os.path.join(PKG, "libhalloc_amd.so")
 NSIZES = 16
 SIZE_TABLE_FIELDS = ("nchunks", "chunk_sz", "chunk_id", "ncib", "sparse", "hash_step",
                      "roomy", "busy", "block_sz", "nblocks", "tmod", "chunk_bytes")
@@ -46,7 +48,7 @@ class _Opts(ctypes.Structure):
                 ("nheads", ctypes.c_int32),
                 ("hash_step_frac", ctypes.c_double),
                 ("cached_arena", ctypes.c_int32),
-                ("pad", ctypes.c_int32)]
+                ("fixed_sets", ctypes.c_int32)]
 
 
 class _Geometry(ctypes.Structure):
@@ -148,9 +150,10 @@ class _ServeResult(ctypes.Structure):
 class HallocOpts:
     """Fields and defaults of halloc_opts_t (src/halloc.h:15-34), plus nheads
     (0: sized to the heap for live blocks of many size classes),
-    hash_step_frac (0 keeps the reference's probe stride) and cached_arena
+    hash_step_frac (0 keeps the reference's probe stride), cached_arena
     (0: uncached arena, coherent across XCDs; 1: L2-cached arena with a
-    release in every hafree)."""
+    release in every hafree) and fixed_sets (0: the sets in use shrink as
+    size classes appear; 1: every wave keeps its own set)."""
     memory: int = 512 << 20
     halloc_fraction: float = 0.75
     busy_fraction: float = 0.835
@@ -160,11 +163,12 @@ class HallocOpts:
     nheads: int = 16
     hash_step_frac: float = 0.0
     cached_arena: int = 0
+    fixed_sets: int = 0
 
     def _c(self) -> _Opts:
         return _Opts(self.memory, self.halloc_fraction, self.busy_fraction, self.roomy_fraction,
                      self.sparse_fraction, self.sb_sz_sh, self.nheads, self.hash_step_frac,
-                     self.cached_arena, 0)
+                     self.cached_arena, self.fixed_sets)
 
 
 _lib = None

@@ -91,6 +91,8 @@ struct Params {
   u32* forced;      // per pair: the head whose forced replacement found no slab, and the free epoch then
   u32* active;      // pairs that hold a head
   u32* free_ep;     // slabs that became free so far
+  u32* spread;      // head sets in use now (adaptive sets): waves fold their set below it
+  u32* class_seen;  // size classes that have held a head, one bit each
   // large-block region: one bit per 4 KiB unit (lbits, the bits past lunits
   // set), per unit the class + 1 of the live block that starts there (lstart,
   // 0 elsewhere), per class a ticket that spreads the first word probed
@@ -103,7 +105,8 @@ struct Params {
   u32 nsbs, sb_sz, sb_sz_sh, nheads;
   u32 nsw, bit_words, nib_words, initialized;
   u32 cached_arena;       // hafree releases before publishing
-  u32 nh_m, nh8_m, pad;   // mod_magic of nheads and of nheads / 8
+  u32 nh_m, nh8_m;        // mod_magic of nheads and of nheads / 8
+  u32 fixed_sets;         // 1: every wave keeps its own head set (no spread)
   u64 cuda_memory;
   SizeInfo si[NSIZES];
 };

@@ -58,12 +58,26 @@ HA_DEV u64 shfl64(u64 v, u32 src) {
 HA_DEV u32 xcc_id() { return __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)); }
 // HW_ID bits 8..15: the wave's CU, shader array and shader engine
 HA_DEV u32 cu_bits() { return __builtin_amdgcn_s_getreg((4) | (8 << 6) | ((8 - 1) << 11)); }
-// The head set a wave uses: its XCD's, and with more than 8 sets (a multiple
-// of 8), one of the XCD's nheads / 8 sets picked by the wave's CU.
+// The head set a wave would use with every set in use: its XCD's, and with
+// more than 8 sets (a multiple of 8), one of the XCD's nheads / 8 sets picked
+// by the wave's CU.
 HA_DEV u32 head_set() {
   const u32 x = xcc_id(), n = ha_p.nheads;
   if (n <= 8) return mod_by(x, n, ha_p.nh_m);
   return x + 8u * mod_by(cu_bits(), n / 8u, ha_p.nh8_m);  // x < 8: already below n
+}
+// Set `full` (< nheads) folded onto the first s sets (s = nheads, or a power
+// of two below it): the XCD part keeps its low bits, the CU part its low bits.
+HA_DEV u32 fold_set(u32 full, u32 s) {
+  if (s >= ha_p.nheads) return full;
+  if (s <= 8) return full & (s - 1);
+  return (full & 7u) + 8u * ((full >> 3) & (s / 8u - 1));
+}
+// The sets in use now, read through the scalar cache: a stale value (the
+// spread only shrinks while a kernel runs) still names valid sets.
+HA_DEV u32 spread_cached() {
+  if (ha_p.fixed_sets) return ha_p.nheads;
+  return *(const __attribute__((address_space(4))) u32*)(uintptr_t)ha_p.spread;
 }
 
 // ------------------------------------------------ agent-scope atomic helpers
@@ -266,9 +280,12 @@ HA_DEV bool free_scarce() {
   return n < lim;
 }
 
+HA_DEV u32 adopt_head(u32 ihead, u32 size_id);
+
 // find_sb_for_size (src/slab.cuh:259-372): roomy -> sparse -> free -> busy;
-// then, for a head needed now, another XCD's head of the class, then an idle
-// head of another class.  A cache refill (now == false) skips the free set
+// for a head needed now, first (after sparse) a head of the class that a pair
+// outside the spread holds, and at the end another XCD's head of the class,
+// then an idle head of another class.  A cache refill (now == false) skips the free set
 // when free slabs are scarce; a head needed now then tries sharing first.
 HA_DEV u32 find_sb_for_size(u32 size_id, u32 chunk_id, u32 ihead, bool now) {
   const SizeInfo& s = ha_p.si[size_id];
@@ -310,6 +327,10 @@ again:
     }
     unlock(l);
     if (found) return sb;
+  }
+  if (now) {
+    const u32 a = adopt_head(ihead, size_id);
+    if (a != SB_NONE) return a;
   }
   const bool scarce = free_scarce();
   if (scarce && !now) return SB_NONE;
@@ -372,6 +393,71 @@ again:
   return SB_NONE;
 }
 
+// ------------------------------------------------------------- set spread
+// With adaptive sets (fixed_sets 0), waves use the first `spread` head sets
+// only.  The spread starts at nheads and shrinks as size classes take their
+// first head: to the largest power of two s with 3 * s * classes <= slabs,
+// so that the heads of every class seen stay within a third of the heap.
+
+// spread for ncl classes (a power of two, at most nheads, at least 1)
+HA_DEV u32 spread_for(u32 ncl) {
+  const u32 per = ha_p.nsbs / (3u * (ncl ? ncl : 1u));
+  if (per >= ha_p.nheads) return ha_p.nheads;
+  u32 s = 1;
+  while (2 * s <= per && 2 * s <= ha_p.nheads) s *= 2;
+  return s >= ha_p.nheads ? ha_p.nheads : s;
+}
+
+// Size class c holds a head for the first time: count it and shrink the
+// spread if the classes seen need it.  The pairs at and above the new spread
+// keep their heads; their waves fold onto the pairs below from then on, and
+// a pair below that needs a head of the class adopts one of them
+// (adopt_head), while their empty heads and cached heads go to whichever
+// search needs a slab (steal_idle_head).  No head is detached under waves
+// that may still be reserving in it.
+HA_DEV void note_class(u32 c) {
+  if (ha_p.fixed_sets) return;
+  const u32 bit = 1u << c;
+  if (ld32(ha_p.class_seen) & bit) return;
+  const u32 old = __hip_atomic_fetch_or(ha_p.class_seen, bit, __ATOMIC_RELAXED, HA_SCOPE);
+  if (old & bit) return;
+  const u32 want = spread_for((u32)__builtin_popcount(old | bit));
+  u32 cur = ld32(ha_p.spread);
+  while (want < cur) {
+    u32 e = cur;
+    if (__hip_atomic_compare_exchange_strong(ha_p.spread, &e, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             HA_SCOPE))
+      break;
+    cur = e;
+  }
+}
+
+// Takes over the head of class size_id held by a pair outside the spread:
+// the word moves to the caller's pair, the slab stays a head with the same
+// reference, so waves still reserving in it stay consistent.  Caller holds
+// the head lock of (ihead, size_id); the other pair's lock is only tried.
+HA_DEV u32 adopt_head(u32 ihead, u32 size_id) {
+  if (ha_p.fixed_sets) return SB_NONE;
+  const SizeInfo& s = ha_p.si[size_id];
+  for (u32 ih = ld32(ha_p.spread); ih < ha_p.nheads; ++ih) {
+    if (ih == ihead) continue;
+    u32* hp = hot(ha_p.head, ih, size_id);
+    u32 w = ld32(hp);
+    if (w == SB_NONE || ctr_count(ld64(ctr_of(w & ~HEAD_EXT))) >= head_limit(w, s)) continue;
+    u32* lp = hot(ha_p.head_lock, ih, size_id);
+    if (!try_lock(lp)) continue;
+    w = ld32(hp);
+    const bool ok = w != SB_NONE;
+    if (ok) {
+      st32(hp, SB_NONE);
+      a_add32(ha_p.active, ~0u);  // that pair no longer holds a head
+    }
+    unlock(lp);
+    if (ok) return w;
+  }
+  return SB_NONE;
+}
+
 // new_sb_for_size (src/slab.cuh:378-445).  `seen` is the head word the caller
 // last used.  A head replaced since then is returned as it is; otherwise the
 // wave that takes the head lock replaces a missing or full head, and a wave
@@ -387,6 +473,14 @@ again:
 // searching again.
 HA_DEV u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen, bool force = false) {
   const SizeInfo& s = ha_p.si[size_id];
+  if (!ha_p.fixed_sets) {
+    // a wave with a stale spread names a pair outside the spread: serve it
+    // from the pair its set folds onto.  `seen` stays the head the caller
+    // last used: when that is the folded pair's head, it is replaced as for
+    // any caller of that pair
+    const u32 sp = ld32(ha_p.spread);
+    if (ihead >= sp) ihead = fold_set(ihead, sp);
+  }
   u32* hp = hot(ha_p.head, ihead, size_id);
   u32* lp = hot(ha_p.head_lock, ihead, size_id);
   u32* cp = hot(ha_p.cached, ihead, size_id);
@@ -424,7 +518,10 @@ HA_DEV u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen, bool force = false)
         } else if (nh != SB_NONE) {
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
           st32(hp, nh);
-          if (cur == SB_NONE) a_add32(ha_p.active, 1u);
+          if (cur == SB_NONE) {
+            a_add32(ha_p.active, 1u);
+            note_class(size_id);
+          }
           if (cur != SB_NONE) {
             u32 csb = cur & ~HEAD_EXT;
             u32* l = ha_p.sb_lock + csb;
@@ -1163,19 +1260,28 @@ HA_DEV void large_free(u64 off) {
 // The entry points are inlined into the calling kernel (link-time inlining
 // under -fgpu-rdc): a call would make every kernel keep its live values in
 // callee-saved registers and pay the callee's register saves per call; inlined,
-// the 16 B malloc/free kernel is about 5% faster.
+// the 16 B malloc/free kernel is about 5% faster.  Built with
+// -DHALLOC_OUT_OF_LINE (libhalloc_ool.a) they are out-of-line calls, as the
+// reference's __noinline__ hamalloc / hafree are (src/halloc.h:38,41): one
+// copy of the allocator however many call sites a program has.
+#ifdef HALLOC_OUT_OF_LINE
+#define HA_API __device__ __attribute__((noinline))
+#else
 #define HA_API __device__ __attribute__((always_inline))
+#endif
 
 HA_API void* hamalloc(size_t nbytes) {
   // requests above 3072 B go to the large-block region (src/halloc.cu:185-195
   // sends them to the CUDA device heap)
   if (nbytes > ha::MAX_BLOCK_SZ) return ha::large_alloc(nbytes);
-  return ha::malloc_small((ha::u32)nbytes, ha::rfl(ha::head_set()));
+  return ha::malloc_small((ha::u32)nbytes, ha::rfl(ha::fold_set(ha::head_set(), ha::spread_cached())));
 }
 
 // Batched allocation: lane l gets cnt (<= HA_MAX_BATCH) blocks of nbytes into
 // out[0..cnt) (NULL where memory ran out); returns the number obtained.
-HA_API int hamalloc_batch(size_t nbytes, int cnt, void** out) {
+namespace ha {
+// hamalloc_batch on head set ihead (already folded to the spread)
+HA_DEV int malloc_batch_on(size_t nbytes, int cnt, void** out, u32 ihead) {
   if (cnt <= 0) return 0;
   ha::u32 c = cnt > (int)ha::MAXB ? ha::MAXB : (ha::u32)cnt;
   int got = 0;
@@ -1185,7 +1291,6 @@ HA_API int hamalloc_batch(size_t nbytes, int cnt, void** out) {
   }
   const ha::u32 me = ha::lane_id();
   const ha::u32 sid = ha::size_id_from_nbytes((ha::u32)nbytes);
-  const ha::u32 ihead = ha::rfl(ha::head_set());
   ha::u64 rem = ha::ballot(true);
   while (rem) {
     ha::u32 l0 = (ha::u32)__builtin_ctzll(rem);
@@ -1196,6 +1301,11 @@ HA_API int hamalloc_batch(size_t nbytes, int cnt, void** out) {
   }
   for (ha::u32 j = 0; j < c; ++j) got += out[j] != nullptr;
   return got;
+}
+}  // namespace ha
+
+HA_API int hamalloc_batch(size_t nbytes, int cnt, void** out) {
+  return ha::malloc_batch_on(nbytes, cnt, out, ha::rfl(ha::fold_set(ha::head_set(), ha::spread_cached())));
 }
 
 // Batched free: lane l frees ptrs[0..cnt) (NULL entries are skipped); round
@@ -1293,12 +1403,24 @@ HA_API void hafree_batch(void* const* ptrs, int cnt) {
   }
 }
 
-// hamalloc with the head set chosen by the caller instead of by the XCD id;
-// the replay harness uses it to pin behaviour across head sets
-__device__ void* ha_malloc_on_head(size_t nbytes, unsigned ihead) {
-  if (nbytes > ha::MAX_BLOCK_SZ) return ha::large_alloc(nbytes);
-  return ha::malloc_small((ha::u32)nbytes, ha::rfl(ihead % ha::ha_p.nheads));
+// hamalloc and hamalloc_batch with the full-spread head set chosen by the
+// caller instead of by the XCD id, folded onto the spread read coherently;
+// the replay harness uses them to pin behaviour across head sets
+namespace ha {
+HA_DEV u32 fold_coherent(unsigned full) {
+  const u32 f = full % ha_p.nheads;
+  return ha_p.fixed_sets ? f : fold_set(f, ld32(ha_p.spread));
 }
+}  // namespace ha
+__device__ void* ha_malloc_on_head(size_t nbytes, unsigned full) {
+  if (nbytes > ha::MAX_BLOCK_SZ) return ha::large_alloc(nbytes);
+  return ha::malloc_small((ha::u32)nbytes, ha::rfl(ha::fold_coherent(full)));
+}
+__device__ int ha_malloc_batch_on_head(size_t nbytes, int cnt, void** out, unsigned full) {
+  return ha::malloc_batch_on(nbytes, cnt, out, ha::rfl(ha::fold_coherent(full)));
+}
+// the full-spread head set of the calling wave (what hamalloc folds)
+__device__ unsigned ha_head_set_full() { return ha::rfl(ha::head_set()); }
 
 HA_API void hafree(void* p) {
   if (!p) return;  // src/halloc.cu:240
@@ -1437,6 +1559,7 @@ __global__ void init_state_k(Params P) {
   }
   // the units past the end of the large region are never free
   if (i == 0 && P.lwords && P.lunits % 64) P.lbits[P.lwords - 1] = ~low_mask(P.lunits % 64);
+  if (i == 0) *P.spread = P.nheads;  // every set in use until size classes appear
   if (i < P.nheads * NSIZES) {
     P.head[i * HOT_STRIDE] = SB_NONE;
     P.cached[i * HOT_STRIDE] = SB_NONE;
@@ -1476,6 +1599,7 @@ static int init_locked(const ha_opts_c& o) {
     return fail(HA_E_INVALID, "fractions must satisfy 0 <= sparse <= roomy <= busy <= 1, 0 < halloc <= 1");
   if (!(o.hash_step_frac >= 0 && o.hash_step_frac < 1)) return fail(HA_E_INVALID, "hash_step_frac must be in [0, 1)");
   if (o.cached_arena != 0 && o.cached_arena != 1) return fail(HA_E_INVALID, "cached_arena must be 0 or 1");
+  if (o.fixed_sets != 0 && o.fixed_sets != 1) return fail(HA_E_INVALID, "fixed_sets must be 0 or 1");
   int dev = 0;
   HA_HIP(hipGetDevice(&dev));
   hipDeviceProp_t prop;
@@ -1496,6 +1620,7 @@ static int init_locked(const ha_opts_c& o) {
     P.nheads = 1;
     while (2 * P.nheads <= 16 && 2 * P.nheads * NSIZES * 3 <= P.nsbs) P.nheads *= 2;
   }
+  P.fixed_sets = (u32)o.fixed_sets;
   P.nh_m = mod_magic(P.nheads);
   P.nh8_m = mod_magic(P.nheads / 8 ? P.nheads / 8 : 1);
   P.cached_arena = (u32)o.cached_arena;
@@ -1529,6 +1654,8 @@ static int init_locked(const ha_opts_c& o) {
   u64 o_forced = off; off = align_up(off + hot_bytes, 256);
   u64 o_active = off; off = align_up(off + 4, 256);
   u64 o_free_ep = off; off = align_up(off + 4, 256);
+  u64 o_spread = off; off = align_up(off + 4, 256);
+  u64 o_seen = off; off = align_up(off + 4, 256);
   u64 o_lbits = off; off = align_up(off + (u64)P.lwords * 8, 256);
   u64 o_lstart = off; off = align_up(off + (u64)P.lunits * 4, 256);
   u64 o_ltick = off; off = align_up(off + (u64)LARGE_NCLASSES * HOT_STRIDE * 4, 256);
@@ -1557,6 +1684,8 @@ static int init_locked(const ha_opts_c& o) {
   P.forced = (u32*)(m + o_forced);
   P.active = (u32*)(m + o_active);
   P.free_ep = (u32*)(m + o_free_ep);
+  P.spread = (u32*)(m + o_spread);
+  P.class_seen = (u32*)(m + o_seen);
   P.large = P.arena + P.arena_bytes;
   P.lbits = (u64*)(m + o_lbits);
   P.lstart = (u32*)(m + o_lstart);
@@ -1616,7 +1745,7 @@ void ha_default_opts(ha_opts_c* o) {
   o->nheads = 16;
   o->hash_step_frac = 0.0;
   o->cached_arena = 0;
-  o->pad = 0;
+  o->fixed_sets = 0;
 }
 
 int ha_init_c(const ha_opts_c* o) {

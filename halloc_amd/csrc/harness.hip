@@ -19,7 +19,9 @@
 #include "ha_internal.h"
 #include "ha_workload.h"
 
-__device__ void* ha_malloc_on_head(size_t nbytes, unsigned ihead);
+__device__ void* ha_malloc_on_head(size_t nbytes, unsigned full);
+__device__ int ha_malloc_batch_on_head(size_t nbytes, int cnt, void** out, unsigned full);
+__device__ unsigned ha_head_set_full();
 __device__ int hamalloc_batch(size_t nbytes, int cnt, void** out);
 __device__ void hafree_batch(void* const* ptrs, int cnt);
 
@@ -43,7 +45,6 @@ int hfail(int code, const std::string& m) { return ha::fail(code, m); }
     if (e_ != hipSuccess) return hfail(HA_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-__device__ __forceinline__ u32 xcc_id_h() { return __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)); }
 
 // Device buffer that frees itself.
 struct DBuf {
@@ -108,10 +109,13 @@ __global__ void __launch_bounds__(64) replay_k(const u32* ops, int nsteps, int l
   for (int st = 0; st < nsteps; ++st) {
     const size_t i = (size_t)st * lanes + l;
     const u32 kind = ops[2 * i], arg = ops[2 * i + 1];
-    const u32 ih = iheads ? iheads[st] : xcc_id_h();
+    // the full-spread head set of the step (given, or the wave's own); the
+    // allocator folds it onto the sets in use, read coherently, so the
+    // model can follow the fold step by step
+    const u32 ih = iheads ? iheads[st] : ha_head_set_full();
     if (l == 0) xcc[st] = ih;
     if (kind == 1) {
-      void* p = iheads ? ha_malloc_on_head(arg, ih) : hamalloc(arg);
+      void* p = ha_malloc_on_head(arg, ih);
       ptrs[i] = p;
       u32 r0 = 0xffffffffu, r1 = 0xffffffffu, r2 = 0xffffffffu;
       if (p) malloc_rec(P, p, arg, r0, r1, r2);
@@ -145,10 +149,11 @@ __global__ void __launch_bounds__(64) replay_batch_k(const u32* ops, int nsteps,
   for (int st = 0; st < nsteps; ++st) {
     const size_t i = (size_t)st * lanes + l;
     const u32 kind = ops[4 * i], arg = ops[4 * i + 1], cnt = ops[4 * i + 2];
-    if (l == 0) xcc[st] = xcc_id_h();
+    const u32 ih = ha_head_set_full();
+    if (l == 0) xcc[st] = ih;
     if (kind == 1) {
       void** out = ptrs + i * ha::MAXB;
-      hamalloc_batch(arg, (int)cnt, out);
+      ha_malloc_batch_on_head(arg, (int)cnt, out, ih);
       const u32 c = cnt > ha::MAXB ? ha::MAXB : cnt;
       for (u32 j = 0; j < c; ++j) {
         void* p = out[j];
@@ -194,7 +199,8 @@ __global__ void dump_k(u64* out, u64 total) {
     else if ((j -= nh) < nh) { v = P.ticket[j * ha::HOT_STRIDE]; }
     else if ((j -= nh) < n5) { v = P.bits[j]; }
     else if ((j -= n5) < n6) { v = P.nibs[j]; }
-    else if ((j -= n6) < ha::LARGE_NCLASSES) { v = P.lticket[j * ha::HOT_STRIDE]; }
+    else if ((j -= n6) < 2) { v = j == 0 ? *P.spread : *P.class_seen; }
+    else if ((j -= 2) < ha::LARGE_NCLASSES) { v = P.lticket[j * ha::HOT_STRIDE]; }
     else if ((j -= ha::LARGE_NCLASSES) < P.lwords) { v = P.lbits[j]; }
     else { j -= P.lwords; v = P.lstart[j]; }
     out[i] = v;
@@ -203,7 +209,7 @@ __global__ void dump_k(u64* out, u64 total) {
 
 u64 dump_words_of(const ha::Params& P) {
   return (u64)P.nsbs * 3 + (u64)ha::NSETS * P.nsw + 3ull * P.nheads * ha::NSIZES +
-         (u64)P.nsbs * P.bit_words + (u64)P.nsbs * P.nib_words + ha::LARGE_NCLASSES + P.lwords + P.lunits;
+         (u64)P.nsbs * P.bit_words + (u64)P.nsbs * P.nib_words + 2 + ha::LARGE_NCLASSES + P.lwords + P.lunits;
 }
 
 // ---------------------------------------------------------------- plain

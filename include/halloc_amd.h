@@ -34,14 +34,12 @@ typedef struct ha_opts_c {
   double roomy_fraction;     /* default 0.6 */
   double sparse_fraction;    /* default 0.012 */
   int32_t sb_sz_sh;          /* default 22 (4 MiB slabs), 21..26 */
-  int32_t nheads;            /* head slabs per size class (1..64), default 16: a wave uses its XCD's
-                                set (xcc_id % nheads); above 8 (a multiple of 8), one of its XCD's
-                                nheads / 8 sets, picked by the wave's CU.  0: the largest power of
-                                two up to 16 with nheads * 16 classes * 3 <= slabs (2 at 512 MiB,
-                                4 at 1 GiB, 16 from 4 GiB), for workloads whose live blocks span
-                                many size classes: each (set, class) pair pins a partly filled head
-                                slab, and with 16 sets and 16 live classes at 512 MiB the heads
-                                take the slabs the largest classes need, so requests fail early */
+  int32_t nheads;            /* head-slab sets (1..64), default 16: each (set, size class) pair
+                                has its own head slab, ticket and lock.  A wave's full-spread set
+                                is its XCD's (xcc_id % nheads); above 8 (a multiple of 8), one of
+                                its XCD's nheads / 8 sets, picked by the wave's CU.  0: the
+                                largest power of two up to 16 with nheads * 16 classes * 3 <=
+                                slabs (2 at 512 MiB, 4 at 1 GiB, 16 from 4 GiB) */
   double hash_step_frac;     /* 0 (default): the probe stride's prime is the largest below
                                 nblocks/256 + nblocks/64 (src/halloc.cu:378-381); else below
                                 hash_step_frac * nblocks, 0 < hash_step_frac < 1 */
@@ -49,10 +47,18 @@ typedef struct ha_opts_c {
                                 across the XCDs' L2s, and kept for reuse after ha_shutdown;
                                 1: L2-cached (hipMalloc) arena, and every hafree performs an
                                 agent-scope release before the blocks can move to another XCD */
-  int32_t pad;
+  int32_t fixed_sets;        /* 0 (default): adaptive sets.  Waves use the first `spread` sets
+                                only; the spread starts at nheads and shrinks, as size classes take
+                                their first head, to the largest power of two s with
+                                3 * s * classes seen <= slabs, and the pairs above it give their
+                                head slabs back to the slab sets.  So one hot class keeps every
+                                set, and many live classes on a small heap do not pin the slabs
+                                the others need (each pair holds a partly filled head slab).
+                                1: every wave keeps its full-spread set. */
 } ha_opts_c;
 
-/* Fills *o with the defaults of src/halloc.h:30-32 and nheads = 16. */
+/* Fills *o with the defaults of src/halloc.h:30-32, nheads = 16 and adaptive
+   sets. */
 void ha_default_opts(ha_opts_c *o);
 
 /* Replaces ha_init (src/halloc.cu:267-424).  Initializes the current device. */
@@ -94,7 +100,8 @@ int ha_geometry_c(ha_geometry *g);
    (chunks freed, slab, chunk) or (units freed, 0xfffffffe, unit), read before
    the free; (0, 0xffffffff, 0xffffffff) when the free changes nothing.  iheads (may be NULL) forces the
    head set of each step's mallocs; otherwise the wave's XCD id picks it.
-   xcc[step] receives the head-set index used (before the modulo nheads). */
+   xcc[step] receives the step's full-spread head set (given, or the wave's
+   own), which the allocator folds onto the sets in use. */
 int ha_replay(const uint32_t *ops, int nsteps, int lanes, const uint32_t *iheads, uint32_t *recs,
               uint32_t *xcc);
 
@@ -103,7 +110,7 @@ int ha_replay(const uint32_t *ops, int nsteps, int lanes, const uint32_t *iheads
    hafree_batch that keeps the op's pointers, so that a later op frees the same
    blocks again), arg (bytes, or the index of an earlier kind-1 op whose blocks
    are freed), count (kind 1), 0.  recs hold 4 u32 per (op, j < 16), as for ha_replay.  xcc[step]
-   receives the wave's XCD id. */
+   receives the wave's full-spread head set. */
 int ha_replay_batch(const uint32_t *ops, int nsteps, int lanes, uint32_t *recs, uint32_t *xcc);
 
 /* Canonical state dump (layout documented in oracle/model_capi.cpp). */

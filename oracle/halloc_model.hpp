@@ -88,6 +88,12 @@
 //     double free, an interior pointer or a unit never handed out is a no-op.
 //     The model states the field search with loops over aligned positions, the
 //     device with word-parallel bit arithmetic (ha_bits.h).
+//   * adaptive head sets (fixed_sets 0): waves fold their set onto the first
+//     `spread` sets; the spread starts at nheads and shrinks when a class takes
+//     its first head, to the largest power of two s with 3 * s * classes seen
+//     <= slabs.  A pair that needs a head adopts a head of its class from a
+//     pair outside the spread (after the roomy and sparse sets, before a free
+//     slab); a pair outside the spread is served from the pair it folds onto.
 #pragma once
 
 #include <atomic>
@@ -186,6 +192,7 @@ struct Opts {
   int nheads = 1;
   int hash_mode = 1;
   double hash_step_frac = 0.0;   // 0: the reference's prime bound
+  int fixed_sets = 0;            // 0: adaptive head sets (the spread shrinks as classes appear)
   u64 device_memory = 0;         // 0: no 3/4-of-device cap
 };
 
@@ -285,6 +292,9 @@ class Heap {
   // free-slab epoch at that time; free_ep counts slabs that became free
   std::vector<std::atomic<u32>> forced_head, forced_ep;
   std::atomic<u32> free_ep{0};
+  // adaptive head sets: waves fold their set onto the first `spread` sets;
+  // class_seen has a bit per size class that has held a head
+  std::atomic<u32> spread{0}, class_seen{0};
   std::atomic<u32> class_lock[NSIZES] = {};
   std::atomic<u32> active{0};  // pairs that hold a head
   std::vector<uint8_t> arena;  // payload memory (allocated lazily by users that touch data)
@@ -359,6 +369,7 @@ class Heap {
       lbits[w].store(v);
     }
     for (auto& x : lstart) x.store(0);
+    spread.store(nheads);
     if (with_arena) arena.assign((size_t)arena_bytes, 0);
     ok = true;
   }
@@ -565,6 +576,10 @@ class Heap {
       unlock(sb_lock[sb]);
       if (found) return sb;
     }
+    if (now) {
+      const u32 a = adopt_head(ihead, size_id);
+      if (a != SB_NONE) return a;
+    }
     const bool scarce = free_scarce();
     if (scarce && !now) return SB_NONE;
     if (scarce && now && !tried_share) {
@@ -615,6 +630,58 @@ class Heap {
     return SB_NONE;
   }
 
+  // ------------------------------------------------------------ set spread
+  // Adaptive sets (halloc.hip note_class / adopt_head): the spread starts at
+  // nheads; when a size class takes its first head, it becomes the largest
+  // power of two s (at most nheads) with 3 * s * (classes seen) <= nsbs, if
+  // that is smaller.  Pairs at and above it keep their heads until a pair
+  // below adopts them (or steals them when empty).
+  u32 fold_set(u32 full, u32 s) const {
+    if (s >= nheads) return full;
+    if (s <= 8) return full % s;
+    return full % 8 + 8 * ((full / 8) % (s / 8));
+  }
+  u32 fold_now(u32 full) const {
+    const u32 f = full % nheads;
+    return opts.fixed_sets ? f : fold_set(f, spread.load());
+  }
+  u32 spread_for(u32 ncl) const {
+    u32 per = nsbs / (3 * (ncl ? ncl : 1)), best = 1;
+    if (per >= nheads) return nheads;
+    for (u32 s = 1; s <= nheads && s <= per; s *= 2) best = s;
+    return best >= nheads ? nheads : best;
+  }
+  void note_class(u32 c) {
+    if (opts.fixed_sets) return;
+    const u32 bit = 1u << c;
+    if (class_seen.load() & bit) return;
+    const u32 old = class_seen.fetch_or(bit);
+    if (old & bit) return;
+    const u32 want = spread_for((u32)__builtin_popcount(old | bit));
+    u32 cur = spread.load();
+    while (want < cur && !spread.compare_exchange_strong(cur, want)) {}
+  }
+  // take over the head of class size_id held by a pair outside the spread
+  // (the word moves, the slab keeps its reference); caller holds the head
+  // lock of (ihead, size_id), the other pair's is only tried
+  u32 adopt_head(u32 ihead, u32 size_id) {
+    if (opts.fixed_sets) return SB_NONE;
+    const SizeInfo& s = si[size_id];
+    for (u32 ih = spread.load(); ih < nheads; ++ih) {
+      if (ih == ihead) continue;
+      const size_t hi = (size_t)ih * NSIZES + size_id;
+      u32 w = head[hi].load();
+      if (w == SB_NONE || ctr_count(ctr[w & ~HEAD_EXT].load()) >= head_limit(w, s)) continue;
+      if (!try_lock(head_lock[hi])) continue;
+      w = head[hi].load();
+      const bool ok = w != SB_NONE;
+      if (ok) { head[hi].store(SB_NONE); active.fetch_sub(1); }
+      unlock(head_lock[hi]);
+      if (ok) return w;
+    }
+    return SB_NONE;
+  }
+
   // new_sb_for_size (src/slab.cuh:378-445).  `seen` is the head word the
   // caller last used: a head replaced since then is returned as it is; a
   // thread that finds the head lock taken waits for a change, and gets
@@ -623,6 +690,13 @@ class Heap {
   // replaced even below its limit (halloc.hip new_sb_for_size)
   u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen, bool force = false) {
     const SizeInfo& s = si[size_id];
+    if (!opts.fixed_sets) {
+      // a pair outside the spread (a stale wave's) is served from the pair
+      // its set folds onto, `seen` kept: when it is the folded pair's head it
+      // is replaced as for any caller of that pair
+      const u32 sp = spread.load();
+      if (ihead >= sp) ihead = fold_set(ihead, sp);
+    }
     size_t hi = (size_t)ihead * NSIZES + size_id;
     for (;;) {
       u32 cur = head[hi].load();
@@ -653,7 +727,7 @@ class Heap {
             forced_head[hi].store(cur);
           } else if (nh != SB_NONE) {
             head[hi].store(nh);
-            if (cur == SB_NONE) active.fetch_add(1);
+            if (cur == SB_NONE) { active.fetch_add(1); note_class(size_id); }
             if (cur != SB_NONE) {
               u32 csb = cur & ~HEAD_EXT;
               lock(sb_lock[csb]); detach_head(csb); unlock(sb_lock[csb]);
@@ -992,6 +1066,7 @@ class Heap {
   // hamalloc_batch for the active lanes of one wave: lane l asks for cnt[l]
   // blocks of nbytes[l] (classes above 3072 B go to the device heap one by one)
   void wave_malloc_batch(u32 ihead, u64 active, const u32* nbytes, const u32* cnt, u64* out, Rec* rec) {
+    const u32 ih = fold_now(ihead);  // once per call, as the device folds it at entry
     u32 sid[WAVE] = {0};
     u64 small = 0, large = 0;
     for (u32 l = 0; l < WAVE; ++l) if (active >> l & 1) {
@@ -1019,12 +1094,15 @@ class Heap {
       u64 g = 0;
       for (u32 l = 0; l < WAVE; ++l) if ((small >> l & 1) && sid[l] == sid[l0]) g |= 1ull << l;
       small &= ~g;
-      malloc_batch_group(ihead % nheads, sid[l0], g, c2, out, rec);
+      malloc_batch_group(ih, sid[l0], g, c2, out, rec);
     }
   }
 
-  // hamalloc for the active lanes of one wave; out[] = arena offset + 1 (0 = NULL)
-  void wave_malloc(u32 ihead, u64 active, const u32* nbytes, u64* out, Rec* rec) {
+  // hamalloc for the active lanes of one wave; out[] = arena offset + 1 (0 = NULL).
+  // raw: use pair ihead as it is, without folding it onto the spread (a wave
+  // that read the spread before it shrank)
+  void wave_malloc(u32 ihead, u64 active, const u32* nbytes, u64* out, Rec* rec, bool raw = false) {
+    const u32 ih = raw ? ihead % nheads : fold_now(ihead);  // once per call, as the device folds it at entry
     u32 sid[WAVE] = {0};
     u64 small = 0, large = 0;
     u64 nb64[WAVE] = {0};
@@ -1043,7 +1121,7 @@ class Heap {
       u64 g = 0;
       for (u32 l = 0; l < WAVE; ++l) if ((small >> l & 1) && sid[l] == sid[l0]) g |= 1ull << l;
       small &= ~g;
-      malloc_group(ihead % nheads, sid[l0], g, out, rec);
+      malloc_group(ih, sid[l0], g, out, rec);
     }
   }
 

@@ -39,15 +39,16 @@ class HmOpts(ctypes.Structure):
                 ("sb_sz_sh", ctypes.c_int32),
                 ("nheads", ctypes.c_int32),
                 ("hash_mode", ctypes.c_int32),
-                ("pad", ctypes.c_int32),
+                ("fixed_sets", ctypes.c_int32),
                 ("hash_step_frac", ctypes.c_double)]
 
 
 def make_opts(memory=512 << 20, halloc_fraction=0.75, busy_fraction=0.835, roomy_fraction=0.6,
-              sparse_fraction=0.012, sb_sz_sh=22, nheads=1, hash_mode=1, hash_step_frac=0.0) -> HmOpts:
-    """Defaults follow src/halloc.h:30-32."""
+              sparse_fraction=0.012, sb_sz_sh=22, nheads=1, hash_mode=1, hash_step_frac=0.0,
+              fixed_sets=0) -> HmOpts:
+    """Defaults follow src/halloc.h:30-32; fixed_sets 0 = adaptive head sets."""
     return HmOpts(memory, halloc_fraction, busy_fraction, roomy_fraction, sparse_fraction,
-                  sb_sz_sh, nheads, hash_mode, 0, hash_step_frac)
+                  sb_sz_sh, nheads, hash_mode, fixed_sets, hash_step_frac)
 
 
 _lib = None
@@ -81,6 +82,7 @@ def lib() -> ctypes.CDLL:
             ("hm_replay", ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp]),
             ("hm_replay_batch", ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp]),
             ("hm_wave", ctypes.c_int, [vp, vp, vp, ctypes.c_int, vp, vp]),
+            ("hm_wave_raw", ctypes.c_int, [vp, u32, vp, ctypes.c_int, vp]),
             ("hm_dump_words", u64, [vp]),
             ("hm_dump", ctypes.c_int, [vp, vp]),
             ("hm_check_dump", i64, [P(HmOpts), vp, vp]),
@@ -162,6 +164,15 @@ class ModelHeap:
         if lib().hm_wave(self.h, k.ctypes.data, a.ctypes.data, lanes, recs.ctypes.data, ptrs.ctypes.data) != 0:
             raise ValueError("hm_wave failed")
         return recs, ptrs
+
+    def wave_raw(self, pair: int, sizes):
+        """One wave of mallocs on head-set pair `pair` unfolded (a wave with a
+        stale spread); returns arena offset + 1 per lane (0 = NULL)."""
+        sz = np.ascontiguousarray(sizes, dtype=np.uint32)
+        ptrs = np.zeros(len(sz), dtype=np.uint64)
+        if lib().hm_wave_raw(self.h, pair, sz.ctypes.data, len(sz), ptrs.ctypes.data) != 0:
+            raise ValueError("hm_wave_raw failed")
+        return ptrs
 
     def malloc_threads(self, sizes, nthreads: int = 8) -> np.ndarray:
         """Mallocs of `sizes` in 64-lane waves raced over nthreads threads;

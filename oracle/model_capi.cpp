@@ -19,7 +19,7 @@ extern "C" {
 struct hm_opts_t {
   uint64_t memory;
   double halloc_fraction, busy_fraction, roomy_fraction, sparse_fraction;
-  int32_t sb_sz_sh, nheads, hash_mode, pad;
+  int32_t sb_sz_sh, nheads, hash_mode, fixed_sets;
   double hash_step_frac;
 };
 // kind 0 none, 1 malloc(arg = bytes), 2 free(arg = op index), 3 free of the
@@ -32,6 +32,7 @@ static Opts to_opts(const hm_opts_t* o) {
   r.roomy_fraction = o->roomy_fraction; r.sparse_fraction = o->sparse_fraction;
   r.sb_sz_sh = o->sb_sz_sh; r.nheads = o->nheads; r.hash_mode = o->hash_mode;
   r.hash_step_frac = o->hash_step_frac;
+  r.fixed_sets = o->fixed_sets;
   return r;
 }
 
@@ -132,6 +133,19 @@ int hm_wave(void* hv, const uint32_t* kinds, const uint64_t* args, int lanes, ui
     ptrs[l] = (am >> l & 1) ? outp[l] : 0;
     std::memcpy(recs + 4 * l, (am >> l & 1) ? &mrec[l] : &frec[l], 16);
   }
+  return 0;
+}
+
+// One wave of mallocs on head-set pair `pair` as given (not folded onto the
+// spread): what a wave whose spread value is stale does.  ptrs: per lane.
+int hm_wave_raw(void* hv, uint32_t pair, const uint32_t* nbytes, int lanes, uint64_t* ptrs) {
+  Heap* h = (Heap*)hv;
+  if (lanes < 1 || lanes > 64) return -1;
+  u32 nb[WAVE] = {0};
+  u64 out[WAVE] = {0};
+  for (int l = 0; l < lanes; ++l) nb[l] = nbytes[l];
+  h->wave_malloc(pair, lanes == 64 ? ~0ull : ((1ull << lanes) - 1), nb, out, nullptr, true);
+  for (int l = 0; l < lanes; ++l) ptrs[l] = out[l];
   return 0;
 }
 
@@ -251,12 +265,13 @@ int hm_replay_batch(void* hv, const uint32_t* ops, const uint32_t* iheads, int n
 //   [3*nheads*16]     heads, cached heads, tickets
 //   [nsbs*bit_words]  bit arrays
 //   [nsbs*nib_words]  allocation-size nibbles
+//   [2]               head sets in use (the spread), classes that held a head (bits)
 //   [LARGE_NCLASSES]  large-region tickets
 //   [lwords]          large-region unit bits
 //   [lunits]          large-region start words (class + 1 of the block starting there)
 static u64 dump_words(const Heap& h) {
   return (u64)h.nsbs * 3 + (u64)Heap::NSETS * h.nsw + 3ull * h.nheads * NSIZES + (u64)h.nsbs * h.bit_words +
-         (u64)h.nsbs * h.nib_words + LARGE_NCLASSES + h.lwords + h.lunits;
+         (u64)h.nsbs * h.nib_words + 2 + LARGE_NCLASSES + h.lwords + h.lunits;
 }
 uint64_t hm_dump_words(void* hv) { return dump_words(*(Heap*)hv); }
 int hm_dump(void* hv, uint64_t* out) {
@@ -273,6 +288,8 @@ int hm_dump(void* hv, uint64_t* out) {
   for (auto& x : h->ticket) *p++ = x.load();
   for (auto& x : h->bits) *p++ = x.load();
   for (auto& x : h->nibs) *p++ = x.load();
+  *p++ = h->spread.load();
+  *p++ = h->class_seen.load();
   for (auto& x : h->lticket) *p++ = x.load();
   for (auto& x : h->lbits) *p++ = x.load();
   for (auto& x : h->lstart) *p++ = x.load();
@@ -286,8 +303,13 @@ int hm_dump(void* hv, uint64_t* out) {
 //   4 every non-head slab with count > 0 is in a set (busy/roomy/sparse)
 //   5 a slab's head references (counter and descriptor) equal the number of
 //     head and cached-head words naming it
-//   6 non-head slabs sit in exactly the set their count names (free, or
-//     sparse / roomy / busy of their class), and in no other
+//   6 a free non-head slab is in the free set; any other non-head slab has a
+//     nonzero count, is not in the free set, and is in a sparse / roomy /
+//     busy set of its class whose level is at most the one its count names
+//     (a wave that loaded a head word before the head was detached can still
+//     add blocks to it; only frees move a slab between sets, so its count may
+//     have risen past its set's range, never fallen).  Further entries in
+//     class sets are stale hints, which pops check and drop
 //   7 large region: every live block (a nonzero start word) is aligned to
 //     its size, inside the region, its units' bits set and not overlapping
 //     another live block; every set bit inside the region belongs to a live
@@ -348,22 +370,32 @@ int64_t hm_check_dump(const hm_opts_t* o, const uint64_t* d, int64_t* detail) {
     u32 dref = (u32)(desc[2 * b + 1] >> 32);
     if (ctr_head_refs(ctr[b]) != refs[b] || dref != refs[b]) detail[4]++;
   }
-  // 6: set membership of non-head slabs by their count
+  // 6: set membership of non-head slabs by their count.  A free slab is in
+  // the free set.  Any other slab holds chunks, is not in the free set (whose
+  // pop checks only is_head), and is in a set of its own class at a level no
+  // higher than its count's, where a search of that level or a lower one
+  // finds it.  Extra entries in class sets are stale hints that every pop
+  // drops after checking the slab (a transition that files a slab by a count
+  // read before a concurrent last free marked it free leaves one:
+  // src/slab.cuh:176-221 files without the slab lock), so they are allowed.
   for (u32 b = 0; b < h.nsbs; ++b) {
     if (refs[b] || (desc[2 * b + 1] >> 32)) continue;
     const u32 size_id = (u32)desc[2 * b], chunk_id = (u32)(desc[2 * b] >> 32);
     const u32 count = ctr_count(ctr[b]);
-    u32 want = 0xffffffffu;  // the one set the slab belongs in
-    if (ctr_chunk(ctr[b]) == CTR_CHUNK_NONE) want = 0;
-    else if (size_id < NSIZES && chunk_id < NCHUNK_IDS) {
-      const u32 cls = sb_class(count, h.si[size_id]);
-      want = cls == CLS_FREE ? 0xffffffffu : h.class_set(cls, size_id, chunk_id);
+    bool ok;
+    if (ctr_chunk(ctr[b]) == CTR_CHUNK_NONE) {
+      ok = in_set(0, b);
+    } else {
+      ok = false;
+      if (!in_set(0, b) && size_id < NSIZES && chunk_id < NCHUNK_IDS && count > 0) {
+        const u32 cls = sb_class(count, h.si[size_id]);
+        for (u32 lvl = CLS_SPARSE; lvl <= cls && !ok; ++lvl) ok = in_set(h.class_set(lvl, size_id, chunk_id), b);
+      }
     }
-    for (u32 set = 0; set < Heap::NSETS; ++set)
-      if (in_set(set, b) != (set == want)) { detail[5]++; break; }
+    if (!ok) detail[5]++;
   }
   // 7: the large region
-  const u64* lbits = nibs + (u64)h.nsbs * h.nib_words + LARGE_NCLASSES;
+  const u64* lbits = nibs + (u64)h.nsbs * h.nib_words + 2 + LARGE_NCLASSES;
   const u64* lstart = lbits + h.lwords;
   std::vector<u8> owned((size_t)h.lwords * 64, 0);
   for (u32 u = 0; u < h.lunits; ++u) {

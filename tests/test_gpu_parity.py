@@ -27,7 +27,8 @@ def say(msg):
 def model_opts(o: H.HallocOpts):
     return M.make_opts(memory=o.memory, halloc_fraction=o.halloc_fraction, busy_fraction=o.busy_fraction,
                        roomy_fraction=o.roomy_fraction, sparse_fraction=o.sparse_fraction,
-                       sb_sz_sh=o.sb_sz_sh, nheads=o.nheads, hash_mode=1, hash_step_frac=o.hash_step_frac)
+                       sb_sz_sh=o.sb_sz_sh, nheads=o.nheads, hash_mode=1, hash_step_frac=o.hash_step_frac,
+                       fixed_sets=o.fixed_sets)
 
 
 @pytest.fixture
@@ -145,7 +146,7 @@ def test_replay_shared_and_stolen_heads(halloc):
     # share another set's head of their class; later size sweeps take over
     # idle heads of old classes
     say("replay shared heads")
-    opts = halloc(H.HallocOpts(nheads=8))
+    opts = halloc(H.HallocOpts(nheads=8, fixed_sets=1))
     rng = np.random.default_rng(5)
     nw = 600
     ops = np.zeros((2 * nw, 64, 2), dtype=np.uint32)
@@ -192,6 +193,67 @@ def large_program(rng, nsteps, lanes):
         for j, op in enumerate(live):
             tail[kind_i * rounds + j // lanes, j % lanes] = (kind, op)
     return np.concatenate([ops, tail]), nsteps + rounds
+
+
+def spread_word(g, d):
+    """(sets in use, classes seen) from a dump."""
+    lw = (g["large_units"] + 63) // 64
+    b = len(d) - g["large_units"] - lw - 40 - 2
+    return int(d[b]), int(d[b + 1])
+
+
+@pytest.mark.parametrize("nheads", [16, 8])
+def test_replay_adaptive_sets(halloc, nheads):
+    # the wave's own head set (XCD and CU), folded onto the sets in use as
+    # size classes appear: one class first, then all of them; records, the
+    # spread and the whole state equal the model's
+    say(f"replay adaptive sets nheads={nheads}")
+    opts = halloc(H.HallocOpts(memory=512 << 20, nheads=nheads))
+    rng = np.random.default_rng(70 + nheads)
+    one = np.zeros((40, 64, 2), dtype=np.uint32)
+    one[:, :, 0] = 1
+    one[:, :, 1] = 16
+    mixed = random_ops(rng, 200, 64, BLOCK_SIZES + [20, 40, 200, 700, 2500])
+    mixed[mixed[..., 0] == 2, 1] += 40 * 64   # frees name ops of the mixed part
+    ops = np.concatenate([one, mixed])
+    recs_g, recs_m, heap = replay_both(opts, ops)
+    g = H.Halloc.geometry()
+    sp, seen = spread_word(g, H.Halloc.dump())
+    say(f"spread {sp}, classes seen {bin(seen).count('1')}")
+    assert sp < nheads and bin(seen).count("1") == 16
+    assert (recs_g == recs_m).all()
+    assert_same_state(heap, opts)
+
+
+def test_adaptive_sets_many_classes_default_options(halloc):
+    # the drop-in defaults (16 sets, adaptive) at 512 MiB with requests over
+    # all 16 classes, in kernels of 20 000 until one returns NULL: the sets
+    # in use have folded to 2, no request failed before the requested bytes
+    # passed half of the slab heap, and when one did no slab was free
+    say("adaptive sets: mixed classes until exhaustion, default options")
+    opts = halloc(H.HallocOpts())
+    g = H.Halloc.geometry()
+    rng = np.random.default_rng(31)
+    held, total = [], 0
+    for rnd in range(200):
+        sizes = rng.integers(8, 3073, 20000).astype(np.uint32)
+        p = H.Halloc.malloc_sizes(sizes, bs=128)
+        held.append(p)
+        if (p == 0).any():
+            break
+        total += int(sizes.sum())
+    say(f"first NULL in round {rnd}: {total / g['arena_bytes']:.3f} of the slab bytes requested before")
+    assert (p == 0).any() and total > 0.5 * g["arena_bytes"]
+    d = H.Halloc.dump()
+    nsw = g["nsw"]
+    assert not d[3 * g["nsbs"]:3 * g["nsbs"] + nsw].any()          # no free slab left
+    assert spread_word(g, d)[0] == 2
+    assert M.check_dump(model_opts(opts), d)[0] == 0
+    allp = np.concatenate(held)
+    H.Halloc.free_ptrs(allp[allp != 0])
+    assert H.Halloc.free_bytes() == g["arena_bytes"]
+    d = H.Halloc.dump()
+    assert ((d[:g["nsbs"]] >> np.uint64(32)) == 0).all()
 
 
 @pytest.mark.parametrize("lanes", [8, 64])

@@ -283,6 +283,37 @@ def test_checker_catches_planted_mistakes():
     free_b = int(np.nonzero(((d[:nsbs] >> np.uint64(8)) & np.uint64(0xF)) == 0xF)[0][0])
     bad[3 * nsbs + free_b // 64] &= ~np.uint64(1 << (free_b % 64))
     assert M.check_dump(opts, bad)[1][2] > 0
+    # 6: a filed slab (not a head, holding chunks) and the sets it is in;
+    # 1 KiB blocks fill and detach a few slabs
+    ops = np.zeros((100, 64, 2), dtype=np.uint32)
+    ops[:, :, 0] = 1
+    ops[:, :, 1] = 1024
+    h.replay(ops)
+    d = h.dump()
+    assert M.check_dump(opts, d)[0] == 0
+    desc_hi = d[nsbs:3 * nsbs].reshape(nsbs, 2)[:, 1] >> np.uint64(32)
+    filed = [b for b in range(nsbs) if desc_hi[b] == 0 and (d[b] >> np.uint64(32)) > 0]
+    assert filed
+    b = filed[0]
+    size_id = int(d[nsbs + 2 * b]) & 0xFFFFFFFF
+    word = lambda s: 3 * nsbs + s * nsw + b // 64
+    bit = np.uint64(1 << (b % 64))
+    ins = [s for s in range(M.NSETS) if d[word(s)] & bit]
+    assert ins and 0 not in ins
+    # ... also in the free set, whose pop would re-initialize it
+    bad = d.copy()
+    bad[word(0)] |= bit
+    assert M.check_dump(opts, bad)[1][5] > 0
+    # ... moved to the busy set of another size class
+    bad = d.copy()
+    for s in ins:
+        bad[word(s)] &= ~bit
+    bad[word(1 + M.NSIZES + (size_id + 1) % M.NSIZES)] |= bit
+    assert M.check_dump(opts, bad)[1][5] > 0
+    # ... and a stale entry of a class set on a free slab is a hint, not an error
+    bad = d.copy()
+    bad[3 * nsbs + (1 + M.NSIZES + size_id) * nsw + free_b // 64] |= np.uint64(1 << (free_b % 64))
+    assert M.check_dump(opts, bad)[0] == 0
 
 
 def test_concurrent_model_and_glibc_baselines_run_clean():
@@ -317,9 +348,9 @@ def freeslabs_sizes():
 
 
 def test_model_freeslabs_with_idle_head_stealing():
-    # 24 slabs, 8 heads per class: the sweep parks idle heads of 13 classes
+    # 24 slabs, 8 fixed head sets: the sweep parks idle heads of 13 classes
     # (up to 16 slabs per class); each class must still get 40% of memory
-    opts = M.make_opts(memory=64 << 20, sb_sz_sh=21, nheads=8)
+    opts = M.make_opts(memory=64 << 20, sb_sz_sh=21, nheads=8, fixed_sets=1)
     h = M.ModelHeap(opts)
     for size in freeslabs_sizes():
         n = int(0.4 * (64 << 20) / size) // 64 * 64
@@ -336,9 +367,10 @@ def test_model_freeslabs_with_idle_head_stealing():
 
 
 def test_model_mixed_sizes_over_eight_heads_share_instead_of_failing():
-    # 16 classes x 8 XCD heads want 128 head slabs from a 96-slab pool: pairs
-    # without a slab share another XCD's head of their class, so nothing fails
-    opts = M.make_opts(nheads=8)
+    # 16 classes x 8 fixed XCD sets want 128 head slabs from a 96-slab pool:
+    # pairs without a slab share another XCD's head of their class, so
+    # nothing fails
+    opts = M.make_opts(nheads=8, fixed_sets=1)
     h = M.ModelHeap(opts)
     rng = np.random.default_rng(21)
     nw = 1024
@@ -471,3 +503,138 @@ def test_hash_step_frac_prime_bound(frac):
         P = set(primes_below(bound))
         assert hs % k == 0 and p in P and bound % p and nb % p
         assert not any(q in P and bound % q and nb % q for q in range(p + 1, bound))
+
+
+# ------------------------------------------------------- adaptive head sets
+def _spread_and_seen(h):
+    n = len(h.dump())
+    d = h.dump()
+    lunits = (h.opts.memory - int(h.opts.halloc_fraction * h.opts.memory)) // 4096
+    lwords = (lunits + 63) // 64
+    base = n - lunits - lwords - 40 - 2
+    return int(d[base]), int(d[base + 1])
+
+
+def test_adaptive_spread_follows_the_classes_seen():
+    # 96 slabs, 16 sets: one class keeps every set; each new class lowers the
+    # spread to the largest power of two s with 3 * s * classes <= 96, and the
+    # pairs above it hold no head afterwards
+    opts = M.make_opts(nheads=16)
+    h = M.ModelHeap(opts)
+    nw = 64
+    ops = np.zeros((nw, 64, 2), dtype=np.uint32)
+    ops[:, :, 0] = 1
+    ops[:, :, 1] = 16
+    h.replay(ops, np.arange(nw, dtype=np.uint32) % 16)
+    assert _spread_and_seen(h) == (16, 1)
+    # classes seen -> spread: 1-2 -> 16, 3-4 -> 8, 5-8 -> 4, 9-16 -> 2
+    expect = [None, 16, 16, 8, 8, 4, 4, 4, 4, 2, 2, 2, 2, 2, 2, 2, 2]
+    for k, size in enumerate(BLOCK_SIZES[1:], start=2):
+        ops[:, :, 1] = size
+        h.replay(ops, np.arange(nw, dtype=np.uint32) % 16)
+        sp, seen = _spread_and_seen(h)
+        assert seen == (1 << k) - 1
+        assert sp == expect[k], (k, sp)
+    nsw = (h.nsbs + 63) // 64
+    hb = 3 * h.nsbs + M.NSETS * nsw
+
+    def heads():
+        return h.dump()[hb:hb + 16 * 16].reshape(16, 16)
+
+    before = heads()
+    outside = int((before[2:] != 0xFFFFFFFF).sum())
+    assert outside > 0                               # taken before the spread shrank
+    # more 16 B blocks than the two inner heads and their cached heads hold:
+    # each inner pair then adopts a 16 B head of an outer pair instead of
+    # taking a free slab, and no outer pair takes a new head
+    ops[:, :, 1] = 16
+    for _ in range(250):
+        h.replay(ops, np.arange(nw, dtype=np.uint32) % 16)
+    after = heads()
+    adopted = set(before[2:, 0].tolist()) & set(after[:2, 0].tolist())
+    assert len(adopted) == 2 and (after[2:, 0] == 0xFFFFFFFF).sum() == 2
+    assert ((after[2:] != 0xFFFFFFFF) <= (before[2:] != 0xFFFFFFFF)).all()
+    assert M.check_dump(opts, h.dump())[0] == 0
+
+
+def test_adaptive_sets_serve_many_classes_on_a_small_heap():
+    # 16 classes over 16 sets on 96 slabs, one wave at a time: every request
+    # is served until the requested bytes pass 55% of the slab heap
+    opts = M.make_opts(nheads=16)
+    h = M.ModelHeap(opts)
+    rng = np.random.default_rng(0)
+    nw = 2600
+    ops = np.zeros((nw, 64, 2), dtype=np.uint32)
+    ops[:, :, 0] = 1
+    ops[:, :, 1] = rng.integers(8, 3073, (nw, 64))
+    recs, _ = h.replay(ops, np.arange(nw, dtype=np.uint32) % 16)
+    nul = (recs[:, :, 0] == M.REC_NULL).ravel()
+    first = int(np.nonzero(nul)[0][0]) if nul.any() else nul.size
+    assert ops[..., 1].ravel()[:first].astype(np.int64).sum() > 0.55 * 96 * (4 << 20)
+    assert M.check_dump(opts, h.dump())[0] == 0
+
+
+def test_adaptive_sets_null_only_when_the_class_has_no_room():
+    # one lane, 16 sets, 96 slabs, sizes over all 16 classes until the first
+    # NULL: then no slab is free, and every slab of the failing class is at
+    # the limit the scheme fills it to
+    opts = M.make_opts(nheads=16)
+    t = M.size_table(opts)
+    rng = np.random.default_rng(5)
+    n = 400000
+    sizes = rng.integers(8, 3073, n).astype(np.uint32)
+    ops = np.zeros((n, 1, 2), dtype=np.uint32)
+    ops[:, 0, 0] = 1
+    ops[:, 0, 1] = sizes
+    iheads = np.arange(n, dtype=np.uint32) % 16
+    recs, _ = M.ModelHeap(opts).replay(ops, iheads)
+    nul = np.nonzero(recs[:, 0, 0] == M.REC_NULL)[0]
+    assert len(nul) > 0
+    first = int(nul[0])
+    h = M.ModelHeap(opts)
+    h.replay(ops[:first + 1], iheads[:first + 1])
+    d = h.dump()
+    nsbs, nsw = h.nsbs, (h.nsbs + 63) // 64
+    sets = d[3 * nsbs:3 * nsbs + M.NSETS * nsw]
+    assert not sets[:nsw].any()                       # no free slab
+    sid = int(M.lib().hm_size_id(int(sizes[first])))
+    busy, nch = int(t[sid][7]), int(t[sid][0])
+    desc = d[nsbs:3 * nsbs].reshape(nsbs, 2)
+    cnt = d[:nsbs] >> np.uint64(32)
+    mine = (desc[:, 0] & np.uint64(0xFFFFFFFF)) == sid
+    assert mine.any()
+    assert (cnt[mine] >= busy).all()                   # every slab of the class is full
+    filled = float((cnt.astype(np.float64) * np.array([int(t[int(x) & 0xFFFF][11]) if (int(x) & 0xFFFFFFFF) < 16 else 0
+                                                    for x in desc[:, 0]])).sum()) / (nsbs * (4 << 20))
+    assert filled > 0.6, filled                        # most of the heap is in use by then
+
+
+@pytest.mark.timeout(120)
+def test_stale_waves_outside_the_spread_are_served_and_exhaust_cleanly():
+    # waves that still use a set above the spread (read before it shrank)
+    # are served from the pair their set folds onto, which is replaced as it
+    # fills; allocating on such a set until the heap is exhausted ends with
+    # NULLs, not a loop, and the outer pair takes no head
+    opts = M.make_opts(memory=64 << 20, sb_sz_sh=21, nheads=16)
+    h = M.ModelHeap(opts)
+    ops = np.zeros((16, 64, 2), dtype=np.uint32)
+    ops[:, :, 0] = 1
+    ops[:, :, 1] = np.array(BLOCK_SIZES)[:, None]
+    h.replay(ops, np.zeros(16, dtype=np.uint32))
+    sp, _ = _spread_and_seen(h)
+    assert sp == 1                                   # 24 slabs, 16 classes
+    got = nul = 0
+    for _ in range(20000):
+        p = h.wave_raw(13, np.full(64, 48, dtype=np.uint32))
+        got += int((p != 0).sum())
+        nul += int((p == 0).sum())
+        if nul:
+            break
+    # the other 15 classes keep a head (and a cached head) each on this
+    # 24-slab heap; the 48 B class fills the rest
+    assert nul > 0 and got * 48 > 0.3 * 24 * (2 << 20)
+    d = h.dump()
+    nsw = (h.nsbs + 63) // 64
+    hb = 3 * h.nsbs + M.NSETS * nsw
+    assert (d[hb + 13 * 16:hb + 14 * 16] == 0xFFFFFFFF).all()
+    assert M.check_dump(opts, d)[0] == 0
