@@ -31,7 +31,8 @@ EXPORTS = ("ha_default_opts", "ha_init_c", "ha_shutdown_c", "ha_extfrag_c", "ha_
            "ha_last_error", "ha_geometry_c", "ha_replay", "ha_dump_words", "ha_dump",
            "ha_run_plain", "ha_malloc_sizes", "ha_free_ptrs", "ha_read_words", "ha_probe_atomics",
            "ha_run_lists", "ha_run_phase", "ha_replay_batch", "ha_run_graphs", "ha_run_serve",
-           "ha_probe_copy", "ha_run_test", "ha_run_sample", "ha_run_corr_test0", "ha_probe_chain")
+           "ha_probe_copy", "ha_run_test", "ha_run_sample", "ha_run_corr_test0", "ha_probe_chain",
+           "ha_free_bytes_noalloc_c")
 
 
 class HallocError(RuntimeError):
@@ -199,6 +200,7 @@ def lib() -> ctypes.CDLL:
         "ha_malloc_sizes": (i32, [vp, i32, i32, vp]),
         "ha_run_corr_test0": (i32, [i32, i32, i32, P(_Corr0Result)]),
         "ha_probe_chain": (i32, [vp, i32, i32, vp]),
+        "ha_free_bytes_noalloc_c": (i32, [P(ctypes.c_uint64)]),
         "ha_free_ptrs": (i32, [vp, i32, i32]),
         "ha_read_words": (i32, [vp, i32, vp]),
         "ha_probe_atomics": (i32, [vp]),
@@ -247,6 +249,13 @@ class Halloc:
     def free_bytes() -> int:
         v = ctypes.c_uint64(0)
         _check(lib().ha_free_bytes_c(ctypes.byref(v)))
+        return int(v.value)
+
+    @staticmethod
+    def free_bytes_noalloc() -> int:
+        """free_bytes() reduced in the library's device scratch array: no allocation."""
+        v = ctypes.c_uint64(0)
+        _check(lib().ha_free_bytes_noalloc_c(ctypes.byref(v)))
         return int(v.value)
 
     @staticmethod

@@ -1570,6 +1570,8 @@ __global__ void init_state_k(Params P) {
 }
 
 static u64 align_up(u64 x, u64 a) { return (x + a - 1) / a * a; }
+static int scratch_ptr(u64** out);
+static int g_heap_dev = -1;  // the device ha_init initialized
 
 static int shutdown_locked() {
   if (!g_host.initialized) return HA_OK;
@@ -1602,6 +1604,8 @@ static int init_locked(const ha_opts_c& o) {
   if (o.fixed_sets != 0 && o.fixed_sets != 1) return fail(HA_E_INVALID, "fixed_sets must be 0 or 1");
   int dev = 0;
   HA_HIP(hipGetDevice(&dev));
+  u64* sp = nullptr;
+  if (int rc = scratch_ptr(&sp)) return rc;  // the scratch address of this device, once
   hipDeviceProp_t prop;
   HA_HIP(hipGetDeviceProperties(&prop, dev));
   u64 memory = std::min<u64>(o.memory, 3ull * (u64)prop.totalGlobalMem / 4ull);  // src/halloc.cu:282
@@ -1699,34 +1703,123 @@ static int init_locked(const ha_opts_c& o) {
   HA_HIP(hipDeviceSynchronize());
   g_host = P;
   g_meta = meta;
+  g_heap_dev = dev;
   return HA_OK;
 }
 
 const Params& host_params() { return g_host; }
 
-// free bytes and free-slab bytes over the heap (src/statistics.cuh:34-48)
-__global__ void free_mem_k(u64* out) {
-  u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ha_p.nsbs) return;
-  u32 cs = ha_p.desc[i].chunk_sz;
-  u32 used = cs ? chunk_mul(ctr_count(ha_p.ctr[(u64)i * CTR_STRIDE]), cs) : 0u;
-  atomicAdd((unsigned long long*)&out[0], (unsigned long long)(ha_p.sb_sz - used));
-  if (cs == 0) atomicAdd((unsigned long long*)&out[1], (unsigned long long)ha_p.sb_sz);
+// Free bytes and free-slab bytes over the heap (find_free_mem_k,
+// src/statistics.cuh:34-48) as a two-stage reduction over a buffer of
+// STAT_WORDS words: STAT_PARTS workgroups stride over the slabs and each
+// writes its two partial sums (words 2b, 2b + 1); then one workgroup adds the
+// partials and writes the totals (words 2 * STAT_PARTS and 2 * STAT_PARTS + 1).
+// Every word a stage reads was written by the stage before it in the same
+// call, so the buffer needs no clearing, and it holds STAT_WORDS words for any
+// number of slabs.
+constexpr u32 STAT_PARTS = 32, STAT_BS = 256;
+constexpr u32 STAT_WORDS = 2 * STAT_PARTS + 2;
+
+__global__ void __launch_bounds__(STAT_BS) free_mem_part_k(u64* buf) {
+  __shared__ u64 red[2][STAT_BS];
+  u64 f = 0, fs = 0;
+  for (u32 i = blockIdx.x * STAT_BS + threadIdx.x; i < ha_p.nsbs; i += STAT_PARTS * STAT_BS) {
+    const u32 cs = ld32(&ha_p.desc[i].chunk_sz);
+    const u32 used = cs ? chunk_mul(ctr_count(ld64(ctr_of(i))), cs) : 0u;
+    f += ha_p.sb_sz - used;
+    if (cs == 0) fs += ha_p.sb_sz;
+  }
+  red[0][threadIdx.x] = f;
+  red[1][threadIdx.x] = fs;
+  __syncthreads();
+  for (u32 h = STAT_BS / 2; h; h >>= 1) {
+    if (threadIdx.x < h) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + h];
+      red[1][threadIdx.x] += red[1][threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    buf[2 * blockIdx.x] = red[0][0];
+    buf[2 * blockIdx.x + 1] = red[1][0];
+  }
+}
+
+__global__ void __launch_bounds__(64) free_mem_sum_k(u64* buf) {
+  const u32 t = threadIdx.x;
+  u64 f = t < STAT_PARTS ? buf[2 * t] : 0ull, fs = t < STAT_PARTS ? buf[2 * t + 1] : 0ull;
+  for (u32 off = 32; off; off >>= 1) {
+    f += __shfl_xor(f, off);
+    fs += __shfl_xor(fs, off);
+  }
+  if (t == 0) {
+    buf[2 * STAT_PARTS] = f;
+    buf[2 * STAT_PARTS + 1] = fs;
+  }
+}
+
+// both stages on the null stream over buf (STAT_WORDS device words), then the
+// totals copied to the host
+static int stats_on(u64* buf, u64* free_mem, u64* free_slab_mem) {
+  free_mem_part_k<<<STAT_PARTS, STAT_BS>>>(buf);
+  HA_HIP(hipGetLastError());
+  free_mem_sum_k<<<1, 64>>>(buf);
+  HA_HIP(hipGetLastError());
+  u64 h[2];
+  HA_HIP(hipMemcpy(h, buf + 2 * STAT_PARTS, 16, hipMemcpyDeviceToHost));
+  *free_mem = h[0];
+  if (free_slab_mem) *free_slab_mem = h[1];
+  return HA_OK;
 }
 
 static int stats_locked(u64* free_mem, u64* free_slab_mem) {
   if (!g_host.initialized) return fail(HA_E_STATE, "allocator not initialized");
   u64* d = nullptr;
-  HA_HIP(hipMalloc(&d, 16));
-  HA_HIP(hipMemset(d, 0, 16));
-  free_mem_k<<<(g_host.nsbs + 127) / 128, 128>>>(d);
-  HA_HIP(hipGetLastError());
-  u64 h[2];
-  HA_HIP(hipMemcpy(h, d, 16, hipMemcpyDeviceToHost));
+  HA_HIP(hipMalloc(&d, STAT_WORDS * 8));
+  const int rc = stats_on(d, free_mem, free_slab_mem);
   HA_HIP(hipFree(d));
-  *free_mem = h[0];
-  if (free_slab_mem) *free_slab_mem = h[1];
+  return rc;
+}
+
+// ------------------------------------------- statistics without allocation
+// ha_free_bytes_noalloc_c runs the statistics kernels over `scratch`, a
+// file-scope device array, instead of a buffer from hipMalloc.  Each device
+// holds its own copy of the array for the life of the process.  Host code
+// never passes the array's name to a kernel (on the host it is a stand-in,
+// not a device address): the address comes from hipGetSymbolAddress, taken
+// once per device with that device current, at ha_init or at the first call.
+constexpr u32 SCRATCH_WORDS = 128;  // 1 KiB per device
+static_assert(STAT_WORDS <= SCRATCH_WORDS, "the statistics buffer must fit the scratch array");
+static __device__ __align__(256) u64 scratch[SCRATCH_WORDS];
+
+constexpr int MAX_DEVICES = 64;
+static u64* g_scratch_ptr[MAX_DEVICES] = {};
+
+// the scratch address on the current device (set up on the first use there)
+static int scratch_ptr(u64** out) {
+  int dev = 0;
+  HA_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(HA_E_INVALID, "device ordinal out of range");
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(HA_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = (u64*)p;
+  }
+  *out = g_scratch_ptr[dev];
   return HA_OK;
+}
+
+static int stats_noalloc_locked(u64* free_mem) {
+  if (!g_host.initialized) return fail(HA_E_STATE, "allocator not initialized");
+  int dev = 0;
+  HA_HIP(hipGetDevice(&dev));
+  if (dev != g_heap_dev) return fail(HA_E_STATE, "the current device is not the one ha_init initialized");
+  if (STAT_WORDS > SCRATCH_WORDS) return fail(HA_E_INVALID, "statistics buffer larger than the scratch array");
+  u64* buf = nullptr;
+  const int rc = scratch_ptr(&buf);
+  if (rc) return rc;
+  return stats_on(buf, free_mem, nullptr);
 }
 
 }  // namespace ha
@@ -1763,6 +1856,12 @@ int ha_free_bytes_c(uint64_t* out) {
   if (!out) return ha::fail(HA_E_INVALID, "null output");
   std::lock_guard<std::mutex> g(ha::g_mu);
   return ha::stats_locked((ha::u64*)out, nullptr);
+}
+
+int ha_free_bytes_noalloc_c(uint64_t* out) {
+  if (!out) return ha::fail(HA_E_INVALID, "null output");
+  std::lock_guard<std::mutex> g(ha::g_mu);
+  return ha::stats_noalloc_locked((ha::u64*)out);
 }
 
 int ha_extfrag_c(int ideal, double* out) {

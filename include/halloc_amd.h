@@ -70,6 +70,16 @@ int ha_extfrag_c(int ideal, double *out);
 /* Free bytes of the slab heap, the counterpart of find_free_mem_k
    (src/statistics.cuh:34-48), summed on the device. */
 int ha_free_bytes_c(uint64_t *out);
+/* No counterpart in the reference.  ha_free_bytes_c without allocation, for
+   callers that cannot allocate device memory (a serving loop polling the
+   heap between launches, harnesses that forbid hipMalloc): the same value,
+   reduced in a 1 KiB device array the library owns (per device, for the life
+   of the process) instead of a hipMalloc buffer; it calls no hipMalloc or
+   hipFree.  The scratch need is fixed (66 words) for any heap size, and the
+   call fails with HA_E_STATE when the current device is not the one
+   ha_init_c initialized.  Calls on one GPU share the array; the library lock
+   serializes them within a process. */
+int ha_free_bytes_noalloc_c(uint64_t *out);
 /* Description of the last failure on this thread. */
 const char *ha_last_error(void);
 
