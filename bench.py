@@ -19,8 +19,9 @@ Also reported (each on its own freshly initialized heap):
                    32768 tokens of 16 B) growing in 256 B pages, <= 65536
                    live sequences, 16 GiB pool, watermark admission;
                    p50/p99 page-malloc latency and request latency
-  phase            2^20 lanes x 4 slots, sizes 8*U{2..32} B, 5 phases
-                   between fill 0.91 and 0.09 of a 512 MiB pool
+  phase            the reference's phase-throughput with its experiments'
+                   flags: 2^20 threads, U{16..256} B, 5 allocation and 5
+                   free phases of 8 chain steps, 512 MiB pool
   sweep            2^19 lanes, U{1..8} requests log-uniform on [16 B, 3 KiB]
                    plus 1% on (3 KiB, 64 KiB], random-order frees, 1 GiB
                    pool; per size class; the large path timed alone
@@ -28,10 +29,15 @@ Also reported (each on its own freshly initialized heap):
                    algorithmic bytes per pair = 4.05 reference-scheme atomics
                    x 8 B + 32 B payload = 64.4 B; traffic from the committed
                    rocprofv3 counter summary when present
-  atomic_roofline  the same kernel against the measured atomic rate of the
-                   chip (ha_probe_atomics, distinct cache lines)
-  cpu_baseline     the host model of the scheme (oracle/), multithreaded, on a
-                   bounded sample of the same workload, and glibc malloc/free
+  atomics          atomics per pair: the reference scheme's 4.05, this
+                   build's count from the code, and the measured TCC_ATOMIC
+                   count per dispatch from the committed counter summary
+  chain            the binding resource: the same grid making only the
+                   kernel's dependent atomic round trips per wave
+                   (ha_probe_chain), against the kernel's launch time
+  cpu_baseline     the host model of the scheme (oracle/), multithreaded over
+                   every usable host core, on a bounded sample of the same
+                   workload, and glibc malloc/free
 """
 import argparse
 import json
@@ -49,11 +55,28 @@ PAIR_ALGO_ATOMICS = 4.05
 HBM_PEAK_GBS = 8000.0
 
 
+def host_cpu():
+    """(usable cores, CPU model name) of this host."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        cores = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return cores, model
+
+
 def cpu_baseline(budget_s: float):
-    """Host model (oracle) and glibc on the host cores; bounded sample."""
+    """Host model (oracle) and glibc on every usable host core; bounded sample."""
     from oracle import model as M
     M.build()
-    cores = min(os.cpu_count() or 1, 16)
+    cores, cpu_model = host_cpu()
     opts = M.make_opts(nheads=16)
     # calibrate on a small sample, then size the sample to the budget
     r0, _ = M.bench_plain(opts, cores, NTHREADS, 1)
@@ -66,6 +89,7 @@ def cpu_baseline(budget_s: float):
     grate, gbad = M.bench_glibc(cores, NTHREADS, giters)
     return {
         "value": rate, "unit": "pairs/s", "cores": cores, "kind": "port",
+        "nproc": os.cpu_count(), "cpu_model": cpu_model,
         "sample": f"host model, 2^20 lanes x {iters} passes of malloc(16)/write/verify/free "
                   f"in 64-lane waves over {cores} threads ({t_model:.1f} s); violations {bad}",
         "glibc": {"value": grate, "unit": "pairs/s", "cores": cores,
@@ -146,16 +170,31 @@ def bench_serving(H, nreq=100000, memory=16 << 30, seed=0):
 
 
 def bench_phase(H, memory=512 << 20):
+    """The phase workload as the reference's phase-throughput runs it (tst/perf/
+    phase-throughput with the flags of its experiments, tst/exp/common.pl:
+    -f0.95 -F0.05 -e0.91 -g5): 2^20 threads, one block of U{16..256} B each,
+    alternating allocation and free phases (10 launches = 5 of each, 8 chain
+    steps per launch); f / F / e are the reference's steady-state fractions
+    (the share of threads holding a block at the end of an allocation and of
+    a free phase, and the share acting per step), which fix the chain's
+    probabilities (tst/common/common.cu:314-355).  The heap uses the library
+    defaults: 16 head sets, adaptive."""
+    from halloc_amd import harness
+    o = harness.defaults_for("phase-throughput")
+    o.parse(["-n1048576", "-s16", "-S256", "-duniform", "-l1", "-i8", "-t10", "-f0.95", "-F0.05",
+             "-e0.91", "-g5", f"-m{memory}"])
     H.Halloc.init(H.HallocOpts(memory=memory))
-    r = H.Halloc.run_phase(nthreads=1 << 20, slots=4, nphases=5, max_steps=400)
+    r = H.Halloc.run_test(harness.TESTS["phase-throughput"], o.nthreads, o.bs, o.ntries, o.nallocs, o.niters,
+                          o.alloc_sz, o.max_alloc_sz, o.max_alloc_sh, harness.DISTRS[o.distr], o.period_mask,
+                          o.group_sh, o.probabs, 0)
     H.Halloc.shutdown()
+    ms = r["malloc_ms"] + r["free_ms"]
     pairs = (r["mallocs"] + r["frees"]) / 2
-    return {"workload": "2^20 lanes x 4 slots, sizes 8*U{2..32} B, p 0.95/0.05, fill 0.91/0.09, 5 phases, "
-                        f"{memory >> 20} MiB pool",
-            "pairs_per_s": pairs / (r["kernel_ms"] / 1e3), "mallocs": r["mallocs"], "frees": r["frees"],
-            "nulls": r["nulls"], "phase_steps": r["phase_steps"], "kernel_ms": r["kernel_ms"],
-            "peak_fill": float(r["trace"][:, 1].max()) if len(r["trace"]) else None,
-            "violations": r["errors"] + (r["mallocs"] != r["frees"])}
+    return {"workload": "phase-throughput -n1048576 -s16 -S256 -l1 -i8 -t10 -f0.95 -F0.05 -e0.91 -g5, "
+                        f"{memory >> 20} MiB pool, library defaults (16 head sets, adaptive)",
+            "chain_probabilities": [[float(x) for x in row] for row in o.probabs],
+            "pairs_per_s": pairs / (ms / 1e3), "mallocs": r["mallocs"], "frees": r["frees"],
+            "nulls": r["nulls"], "kernel_ms": ms, "violations": r["nulls"] + r["checks_failed"]}
 
 
 def bench_sweep(H, memory=1 << 30, seed=0, rounds=8):
@@ -202,14 +241,26 @@ def bench_sweep(H, memory=1 << 30, seed=0, rounds=8):
     return out
 
 
-def traffic_from_profiles():
+def plain_counters():
+    """The committed counter summary of the plain kernel (profiles/), or {}."""
     p = os.path.join(ROOT, "profiles", "plain_16B_counters.json")
-    if not os.path.exists(p):
-        return None
     try:
-        return json.load(open(p)).get("hbm_bytes_per_launch")
+        return json.load(open(p))
     except (OSError, ValueError):
-        return None
+        return {}
+
+
+# Atomics per 64-lane wave of the plain kernel (one 16 B class group, its
+# blocks in at most two 64-block runs), counted from the code: malloc takes a
+# ticket (1), reserves its run with one OR per bit word touched (1-2, about
+# 1.5), adds to the slab counter (1) and records block lengths with one OR
+# per nibble word touched (4-5, about 4.5); free clears the nibbles (4.5) and
+# the bits (1.5) the same way and subtracts from the counter (1).
+CODE_ATOMICS_PER_WAVE = 1 + 1.5 + 1 + 4.5 + 4.5 + 1.5 + 1
+# dependent round trips of one iteration of a wave: head load with the
+# ticket, probe, counter (malloc); descriptor, nibbles, counter (free); the
+# payload write and read-back overlap the rest
+CHAIN_ROUNDS = 6
 
 
 def main():
@@ -235,10 +286,11 @@ def main():
     import halloc_amd as H
     import numpy as np
 
-    probe = copy_gbps = None
+    probe = copy_gbps = chain = None
     if not args.no_probe:
         probe = H.Halloc.probe_atomics()
         copy_gbps = H.Halloc.probe_copy(1 << 30, 5)
+        chain = H.Halloc.probe_chain((0, 1, 2, 4, CHAIN_ROUNDS, 8, 16), nthreads=NTHREADS)
     H.Halloc.init(H.HallocOpts(memory=512 << 20, nheads=args.nheads))
     r = H.Halloc.run_plain(nthreads=NTHREADS, bs=BS, size=16, iters=1, warmup=args.warmup,
                            launches=args.steps)
@@ -254,7 +306,8 @@ def main():
     value = pairs / wall_s
     avg_launch_s = kern_s / args.steps
     achieved = NTHREADS * PAIR_ALGO_BYTES / avg_launch_s / 1e9
-    traffic = traffic_from_profiles()
+    counters = plain_counters()
+    traffic = counters.get("hbm_bytes_per_launch")
     out = {
         "metric": "16B malloc+free pairs/s on 1 MI355X, batched vs plain; p99 serve latency",
         "value": value,
@@ -280,14 +333,25 @@ def main():
     }
     if copy_gbps is not None:
         out["roofline"]["copy_probe_gbps"] = copy_gbps
+    measured = counters.get("tcc_atomics_per_pair")
+    out["atomics"] = {
+        "reference_scheme_per_pair": PAIR_ALGO_ATOMICS,
+        "this_build_per_pair_by_code": CODE_ATOMICS_PER_WAVE / 64,
+        "measured_per_pair": measured,
+        "measured_rate": measured * NTHREADS / avg_launch_s if measured else None,
+        "source": "TCC_ATOMIC per dispatch / 2^20 pairs (profiles/plain_16B_counters.json)",
+    }
     if probe is not None:
-        ach = NTHREADS * PAIR_ALGO_ATOMICS / avg_launch_s
-        out["atomic_roofline"] = {"achieved": ach, "peak": float(probe[0]), "unit": "atomics/s",
-                                  "frac": ach / float(probe[0]),
-                                  "note": "reference-scheme atomics (4.05/pair) per second vs the measured "
-                                          "rate of 64-bit atomics on distinct cache lines",
-                                  "probe": {"lane_random": float(probe[0]), "wave_run": float(probe[1]),
-                                            "one_word": float(probe[2])}}
+        out["atomics"]["probe_rates"] = {"lane_random": float(probe[0]), "wave_run": float(probe[1]),
+                                         "one_word": float(probe[2])}
+    if chain is not None:
+        # the bound of a chain of dependent atomics per wave: the same grid,
+        # every wave making CHAIN_ROUNDS dependent round trips and nothing else
+        t = chain[CHAIN_ROUNDS] * 1e-3
+        out["chain"] = {"rounds_per_iteration": CHAIN_ROUNDS, "probe_launch_s": t,
+                        "round_trip_s": (chain[16] - chain[0]) * 1e-3 / 16,
+                        "kernel_launch_s": avg_launch_s, "frac": t / avg_launch_s,
+                        "probe_ms_by_rounds": {str(k): v for k, v in chain.items()}}
     extras = [] if args.no_extras else [e for e in args.extras.split(",") if e]
     for e in extras:
         if e not in ("batching", "serving", "phase", "sweep"):
