@@ -35,6 +35,7 @@ constexpr u32 MAX_BLOCK_SZ = 3072;
 constexpr u32 BLOCK_STEP = 16;
 constexpr u32 MAX_NTRIES = 32;       // src/slab.h:36
 constexpr u32 CHECK_NTRIES = 2;      // src/slab.h:39
+constexpr u32 CROWDED_NTRIES = 8;   // failed probe rounds of one sequence that mark a head crowded
 constexpr u32 STEP_FREQ = 64;        // src/slab.h:48
 constexpr u32 THREAD_FREQ = 11;      // src/halloc.cu:40
 constexpr u32 RUN = 64;              // blocks per class-group run (one per wave lane)

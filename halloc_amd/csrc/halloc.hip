@@ -464,8 +464,9 @@ HA_DEV u32 adopt_head(u32 ihead, u32 size_id) {
 // that finds the lock taken waits for the head to change or the lock to be
 // released.  A waiter whose wait saw a slab search fail gets SB_NONE too
 // rather than repeating the search; otherwise it looks again.  force: the
-// caller's probes ran out of tries in `seen`, which is then replaced even
-// below its limit -- a deliberate difference: the reference replaces a head
+// caller's probes ran out of tries in `seen`, or one probe sequence of its
+// group failed CROWDED_NTRIES rounds there (a crowded head), and `seen` is
+// then replaced even below its limit -- a deliberate difference: the reference replaces a head
 // only when it is missing or at its busy threshold (src/slab.cuh:385-387).
 // When such a forced search finds no other slab the head is kept, and the
 // pair records the head and the free-slab epoch: a later forced replacement
@@ -651,14 +652,19 @@ HA_DEV void* malloc_group(u32 sid, u64 g, u32 ihead, u32 me) {
   void* p = nullptr;
   u32 my_ich = 0;
   while (want) {
-    bool need = false, spent = false;
+    bool need = false, spent = false, crowded = false;
     const u32 hsb = hword == SB_NONE ? SB_NONE : (hword & ~HEAD_EXT);
     const u32 limit = head_limit(hword, s);
     if (hsb == SB_NONE) {
       need = true;
     } else {
+      const u32 itry0 = itry;
       const u64 reserved = probe_run(s, hsb, limit, old, want, me, itry, my_ich, 0, &spent);
       if (reserved != want) need = true;
+      // a probe sequence this long says the head is crowded: the group's last
+      // lanes wait for the luckiest of log(64) / log(1 / fill) rounds, so the
+      // head is replaced even below its limit (CROWDED_NTRIES)
+      if (itry - itry0 >= CROWDED_NTRIES) need = crowded = true;
       if (reserved) {
         // sb_ctr_inc (src/slab.cuh:122-155): one atomic for the group
         const u32 change = k * (u32)__builtin_popcountll(reserved);
@@ -684,7 +690,7 @@ HA_DEV void* malloc_group(u32 sid, u64 g, u32 ihead, u32 me) {
     }
     if (need) {
       u32 nh = 0;
-      if (me == lead) nh = new_sb_for_size(sid, ihead, hword, spent && want != 0);
+      if (me == lead) nh = new_sb_for_size(sid, ihead, hword, (spent && want != 0) || crowded);
       hword = rdl(nh, lead);
       if (hword == SB_NONE) want = 0;
     }

@@ -74,6 +74,11 @@
 //     a slab becomes free.
 //     The reference replaces a head only when it is missing or at its busy
 //     threshold (src/slab.cuh:385-387).
+//   * a head in which one probe sequence of a group failed CROWDED_NTRIES (8)
+//     rounds is "crowded" and replaced the same way, even when the group was
+//     served: a group waits for its last lane, about log(lanes) / log(1/fill)
+//     rounds, so a wave pays far more than one thread for a head near its
+//     limit or full of scattered holes.
 //   * requests above 3072 B: the reference hands them to the CUDA device heap
 //     (src/halloc.cu:185-187, 233-235), which is not in its tree.  Here they
 //     come from the large region, the share of the pool outside the slabs
@@ -119,6 +124,7 @@ constexpr u32 MAX_BLOCK_SZ = 3072;  // src/size-info.h:62
 constexpr u32 BLOCK_STEP = 16;      // bit-array granularity, src/size-info.h:50
 constexpr u32 MAX_NTRIES = 32;      // src/slab.h:36
 constexpr u32 CHECK_NTRIES = 2;     // src/slab.h:39
+constexpr u32 CROWDED_NTRIES = 8;  // failed probe rounds of one sequence that mark a head crowded
 constexpr u32 STEP_FREQ = 64;       // src/slab.h:48
 constexpr u32 THREAD_FREQ = 11;     // src/halloc.cu:40
 constexpr u32 THREAD_MOD_SH = 2;    // src/halloc.cu:44
@@ -292,6 +298,7 @@ class Heap {
   // free-slab epoch at that time; free_ep counts slabs that became free
   std::vector<std::atomic<u32>> forced_head, forced_ep;
   std::atomic<u32> free_ep{0};
+  std::atomic<u64> n_crowded{0};  // probe sequences that marked their head crowded
   // adaptive head sets: waves fold their set onto the first `spread` sets;
   // class_seen has a bit per size class that has held a head
   std::atomic<u32> spread{0}, class_seen{0};
@@ -825,14 +832,19 @@ class Heap {
     u32 itry = 0;
     u64 want = g;
     while (want) {
-      bool need = false, spent = false;
+      bool need = false, spent = false, crowded = false;
       const u32 hsb = hword == SB_NONE ? SB_NONE : (hword & ~HEAD_EXT);
       const u32 limit = head_limit(hword, s);
       if (hsb == SB_NONE) {
         need = true;
       } else {
+        const u32 itry0 = itry;
         u64 reserved = probe_in(s, hsb, limit, want, ich0, ich, itry, false, &spent);
         if (reserved != want) need = true;
+        if (itry - itry0 >= CROWDED_NTRIES) {
+          need = crowded = true;
+          n_crowded.fetch_add(1);
+        }
         if (reserved) {
           // sb_ctr_inc (src/slab.cuh:122-155), one group per (slab, ncib)
           u32 change = s.ncib * (u32)__builtin_popcountll(reserved);
@@ -850,7 +862,7 @@ class Heap {
         }
       }
       if (need) {
-        u32 nh = new_sb_for_size(size_id, ihead, hword, spent && want != 0);
+        u32 nh = new_sb_for_size(size_id, ihead, hword, (spent && want != 0) || crowded);
         hword = nh;
         if (nh == SB_NONE) {
           for (u32 l = 0; l < WAVE; ++l) if (want >> l & 1) {

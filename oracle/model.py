@@ -79,6 +79,7 @@ def lib() -> ctypes.CDLL:
             ("hm_create", vp, [P(HmOpts)]),
             ("hm_destroy", None, [vp]),
             ("hm_nsbs", u32, [vp]),
+            ("hm_crowded", u64, [vp]),
             ("hm_replay", ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp]),
             ("hm_replay_batch", ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp]),
             ("hm_wave", ctypes.c_int, [vp, vp, vp, ctypes.c_int, vp, vp]),
@@ -164,6 +165,10 @@ class ModelHeap:
         if lib().hm_wave(self.h, k.ctypes.data, a.ctypes.data, lanes, recs.ctypes.data, ptrs.ctypes.data) != 0:
             raise ValueError("hm_wave failed")
         return recs, ptrs
+
+    def crowded(self) -> int:
+        """Probe sequences so far that marked their head crowded."""
+        return int(lib().hm_crowded(self.h))
 
     def wave_raw(self, pair: int, sizes):
         """One wave of mallocs on head-set pair `pair` unfolded (a wave with a

@@ -71,6 +71,8 @@ void* hm_create(const hm_opts_t* o) {
 }
 void hm_destroy(void* h) { delete (Heap*)h; }
 uint32_t hm_nsbs(void* h) { return ((Heap*)h)->nsbs; }
+// probe sequences so far that marked their head crowded (the crowded-head rule)
+uint64_t hm_crowded(void* h) { return ((Heap*)h)->n_crowded.load(); }
 
 // Replay nsteps x lanes operations: per step, the malloc lanes run as one wave
 // call, then the free lanes run as one wave call.  recs: 4 u32 per op.

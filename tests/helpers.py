@@ -74,6 +74,31 @@ def pressure_program(seed=3, nsteps=6500):
     return np.stack(steps)
 
 
+def crowded_program(seed=5, size=256, fill_steps=100, churn_steps=200):
+    """One wave: fill a head with `size` B blocks to about 78% of a 2 MiB slab
+    (64 lanes x fill_steps), then churn: every step frees 32 random live blocks
+    and allocates 32, so the head keeps its fill while its free blocks scatter
+    and a group's probes need many rounds (the crowded-head rule)."""
+    rng = np.random.default_rng(seed)
+    lanes = 64
+    ops = np.zeros((fill_steps + churn_steps, lanes, 2), dtype=np.uint32)
+    live = []
+    for st in range(fill_steps):
+        ops[st, :, 0] = 1
+        ops[st, :, 1] = size
+        live.extend(range(st * lanes, (st + 1) * lanes))
+    for st in range(fill_steps, fill_steps + churn_steps):
+        new = []
+        for l in range(lanes):
+            if l % 2 == 0:
+                ops[st, l] = (2, live.pop(int(rng.integers(len(live)))))
+            else:
+                ops[st, l] = (1, size)
+                new.append(st * lanes + l)
+        live.extend(new)
+    return ops
+
+
 def random_batch_ops(rng, nsteps, lanes, sizes, max_count=16, p_free=0.4):
     """Random batched replay program: [kind, arg, count, 0] per (step, lane)."""
     ops = np.zeros((nsteps, lanes, 4), dtype=np.uint32)

@@ -13,7 +13,7 @@ import pytest
 
 import halloc_amd as H
 from oracle import model as M
-from tests.helpers import (BLOCK_SIZES, LARGE_SB, REC_NULL, assert_disjoint, graph_sizes, large_class_of,
+from tests.helpers import (crowded_program, BLOCK_SIZES, LARGE_SB, REC_NULL, assert_disjoint, graph_sizes, large_class_of,
                            large_state, large_units, live_intervals, pressure_program, random_batch_ops, random_ops,
                            sweep_sizes, wave_refree_ops)
 
@@ -139,6 +139,22 @@ def test_replay_pressure_exhaustion_and_reuse(halloc):
     phase1 = set(recs_g[:2500, :, 1][ops[:2500, :, 0] == 1].tolist()) - {REC_NULL}
     phase3 = set(recs_g[5500:, :, 1][ops[5500:, :, 0] == 1].tolist()) - {REC_NULL}
     assert phase3 and phase3 <= phase1
+
+
+def test_replay_crowded_head_replaced_below_its_limit(halloc):
+    # a head kept at about 78% fill by churn: a group whose probes need 12
+    # failed rounds has its head replaced though the count is below the limit
+    say("replay crowded head")
+    opts = halloc(H.HallocOpts(memory=32 << 20, sb_sz_sh=21, nheads=1))
+    ops = crowded_program()
+    mh = M.ModelHeap(model_opts(opts))
+    mh.replay(ops)
+    assert mh.crowded() >= 1
+    recs_g, recs_m, heap = replay_both(opts, ops)
+    assert (recs_g == recs_m).all()
+    assert_same_state(heap, opts)
+    table = M.size_table(model_opts(opts))
+    assert_disjoint(live_intervals(ops, recs_g, table), 1 << 21)
 
 
 def test_replay_shared_and_stolen_heads(halloc):

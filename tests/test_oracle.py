@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import model as M
-from tests.helpers import (BLOCK_SIZES, assert_disjoint, live_intervals, pressure_program, random_batch_ops,
+from tests.helpers import (REC_NULL, crowded_program, BLOCK_SIZES, assert_disjoint, live_intervals, pressure_program, random_batch_ops,
                            random_ops, wave_refree_ops)
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
@@ -314,6 +314,34 @@ def test_checker_catches_planted_mistakes():
     bad = d.copy()
     bad[3 * nsbs + (1 + M.NSIZES + size_id) * nsw + free_b // 64] |= np.uint64(1 << (free_b % 64))
     assert M.check_dump(opts, bad)[0] == 0
+
+
+def test_crowded_head_is_replaced_below_its_limit():
+    # the crowded-head rule: a probe sequence of 12 failed rounds replaces the
+    # head even though its count is below the busy limit; the old head is filed
+    # by its count and the live blocks stay disjoint
+    opts = M.make_opts(memory=32 << 20, sb_sz_sh=21, nheads=1)
+    h = M.ModelHeap(opts)
+    ops = crowded_program()
+    recs, _ = h.replay(ops)
+    assert h.crowded() >= 1
+    assert not ((ops[..., 0] == 1) & (recs[..., 0] == REC_NULL)).any()
+    d = h.dump()
+    assert M.check_dump(opts, d)[0] == 0
+    nsbs = h.nsbs
+    count = (d[:nsbs] >> np.uint64(32)).astype(np.int64)
+    refs = d[nsbs:3 * nsbs].reshape(nsbs, 2)[:, 1] >> np.uint64(32)
+    table = M.size_table(opts)
+    busy = int(table[8][M.SIZE_TABLE_FIELDS.index("busy")])
+    used = [b for b in range(nsbs) if count[b] > 0]
+    assert len(used) >= 2
+    detached = [b for b in used if refs[b] == 0]
+    assert detached and all(count[b] < busy for b in detached)
+    assert_disjoint(live_intervals(ops, recs, table), 1 << 21)
+    # the same program without the churn never crowds its head
+    h2 = M.ModelHeap(opts)
+    h2.replay(ops[:100])
+    assert h2.crowded() == 0
 
 
 def test_concurrent_model_and_glibc_baselines_run_clean():
