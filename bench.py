@@ -250,13 +250,14 @@ def plain_counters():
         return {}
 
 
-# Atomics per 64-lane wave of the plain kernel (one 16 B class group, its
-# blocks in at most two 64-block runs), counted from the code: malloc takes a
-# ticket (1), reserves its run with one OR per bit word touched (1-2, about
-# 1.5), adds to the slab counter (1) and records block lengths with one OR
-# per nibble word touched (4-5, about 4.5); free clears the nibbles (4.5) and
-# the bits (1.5) the same way and subtracts from the counter (1).
-CODE_ATOMICS_PER_WAVE = 1 + 1.5 + 1 + 4.5 + 4.5 + 1.5 + 1
+# Atomics per 64-lane wave of the plain kernel, counted from the code.  Every
+# wave is one 16 B class group of 64 lanes, so the group's tickets stay
+# multiples of 64 and its blocks fill one aligned 64-block run: malloc takes a
+# ticket (1), reserves the run with one OR on its bit word (1), adds to the
+# slab counter (1) and records the block lengths with one OR per nibble word
+# (4); free clears the nibbles (4) and the bits (1) the same way and subtracts
+# from the counter (1).
+CODE_ATOMICS_PER_WAVE = 1 + 1 + 1 + 4 + 4 + 1 + 1
 # dependent round trips of one iteration of a wave: head load with the
 # ticket, probe, counter (malloc); descriptor, nibbles, counter (free); the
 # payload write and read-back overlap the rest
