@@ -16,7 +16,7 @@ one-for-one; the kernels are the library's (``ha_run_test``):
     python -m halloc_amd.harness phase-alloc-write -f0.95 -F0.05 -e0.91 -s16 -S256 -l1 -i1 -t10
     python -m halloc_amd.harness phase-extfrag -f0.95 -F0.05 -e0.91 -s16 -S256 -l1 -i1 -t10
     python -m halloc_amd.harness freeslabs
-    python -m halloc_amd.harness run-all [--quick]
+    python -m halloc_amd.harness run-all [--quick] [--part I/N]
     python -m halloc_amd.harness throughput-all [--quick]   (tst/perf/throughput/throughput-all.sh)
     python -m halloc_amd.harness exp scaling|speed|all [--quick] [--out DIR]   (tst/exp/scaling, tst/exp/speed)
     python -m halloc_amd.harness grid-points      (samples/grid-points)
@@ -537,9 +537,14 @@ def write_exp_csvs(tables: dict, outdir: str) -> list[str]:
     return paths
 
 
-def run_all(quick: bool = False, memory: int = 512 * 1024 * 1024, out=print) -> tuple[int, int]:
+def run_all(quick: bool = False, memory: int = 512 * 1024 * 1024, out=print, part=(0, 1)) -> tuple[int, int]:
+    """part (i, n): only the entries whose index is i modulo n, so that the
+    full sweep (about half an hour) can run in pieces."""
     ntests = nok = 0
-    for test, argv in run_all_sweep(memory=memory, quick=quick):
+    i, n = part
+    for k, (test, argv) in enumerate(run_all_sweep(memory=memory, quick=quick)):
+        if k % n != i:
+            continue
         o = defaults_for(test)
         o.parse(argv)
         ok = run_test(test, o, out=lambda m: None)
@@ -599,7 +604,12 @@ def main(argv=None) -> int:
         return 0
     test, args = argv[0], argv[1:]
     if test == "run-all":
-        n, ok = run_all(quick="--quick" in args)
+        # run-all [--quick] [--part I/N]
+        part = (0, 1)
+        if "--part" in args:
+            i, n = args[args.index("--part") + 1].split("/")
+            part = (int(i), int(n))
+        n, ok = run_all(quick="--quick" in args, part=part)
         return 0 if ok == n else 1
     if test == "exp":
         # exp scaling|speed|all [--quick] [--out DIR]

@@ -92,6 +92,19 @@ def test_run_all_sweep_list():
     assert {a[1] for a in prob} == {"-t16384", "-t3277"}
 
 
+def test_run_all_parts_cover_the_sweep_once(monkeypatch):
+    # run-all --part I/N: the N parts together run every entry exactly once
+    seen = []
+    monkeypatch.setattr(harness, "run_test", lambda test, o, out=None: seen.append((test, o.nthreads)) or True)
+    full = [(t, a) for t, a in run_all_sweep(quick=True)]
+    for i in range(3):
+        n, ok = harness.run_all(quick=True, out=lambda m: None, part=(i, 3))
+        assert n == ok == len(range(i, len(full), 3))
+    assert len(seen) == len(full)
+    assert [t for t, _ in seen] != [t for t, _ in full] or len(full) < 3
+    assert sorted(t for t, _ in seen) == sorted(t for t, _ in full)
+
+
 def test_throughput_all_list():
     # throughput-all.sh: the 16 size classes in order, 4 / 2 / 1 allocations
     # per thread, threads halved every two classes above 256 B
