@@ -128,7 +128,7 @@ int64_t ha_dump_words(void);
 int ha_dump(uint64_t *out, int64_t nwords);
 
 /* Plain 16 B throughput (tst/perf/priv-throughput/priv-throughput.cu:14-27):
-   nthreads lanes in blocks of bs; each lane does iters x { p = hamalloc(size);
+   nthreads lanes in blocks of bs (at most 256); each lane does iters x { p = hamalloc(size);
    write a 16 B pattern {tid, ~tid, tid ^ 0x9E3779B9, iter}; read it back;
    hafree(p) }; `launches` timed launches after `warmup` untimed ones.
    ms[i] receives launch i's duration (HIP events); total_ms is their sum and
