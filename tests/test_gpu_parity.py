@@ -362,16 +362,51 @@ def test_freeslabs_reuse_across_classes(halloc):
     assert M.check_dump(model_opts(opts), d)[0] == 0
 
 
+def expected_extfrag(g, d, ideal):
+    # ha_extfrag (src/statistics.cuh:50-69): 1 - max_alloc / free_mem, from
+    # the dump alone.  free_mem: per slab, its bytes less its count x chunk
+    # size, plus the whole large region (the reference adds its device heap);
+    # max_alloc: the largest 2^c units whose aligned run of unit bits is all
+    # clear (+ the bytes of the free slabs when ideal)
+    nsbs, sb = g["nsbs"], g["sb_sz"]
+    count = (d[:nsbs] >> np.uint64(32)).astype(np.int64)
+    cs = (d[nsbs:3 * nsbs].reshape(nsbs, 2)[:, 1] & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    chunk = np.where(cs != 0, (1 << (cs & 0xFFFF)) * (cs >> 16), 0)  # chunk size: (multiplier << 16) | shift
+    free_mem = int((sb - count * chunk).sum()) + g["large_bytes"]
+    free_slabs = int(sb * (cs == 0).sum())
+    _, lbits, _ = large_state(d, g["large_units"])
+    used = np.unpackbits(np.asarray(lbits, dtype="<u8").view(np.uint8), bitorder="little")[:g["large_units"]]
+    best, k = 0, 1
+    while k <= g["large_units"]:
+        runs = used[:g["large_units"] // k * k].reshape(-1, k)
+        if not (~runs.any(axis=1)).any():
+            break
+        best, k = k, 2 * k
+    max_alloc = best * 4096 + (free_slabs if ideal else 0)
+    return 1.0 - max_alloc / free_mem
+
+
 def test_extfrag_and_free_bytes(halloc):
+    # ha_extfrag against the same formula evaluated on the dumped state, on an
+    # empty heap (0.75 and 0: the large region is a quarter of the pool and the
+    # slabs all free), after one 4 KiB block (which halves the largest aligned
+    # run: 0.875 and 0.125 exactly), and after 2^16 blocks of 64 B
     say("extfrag")
-    opts = halloc(H.HallocOpts())
+    halloc(H.HallocOpts())
     g = H.Halloc.geometry()
     assert H.Halloc.free_bytes() == g["arena_bytes"]
+    assert abs(H.Halloc.extfrag(False) - 0.75) < 1e-12 and abs(H.Halloc.extfrag(True)) < 1e-12
+    big = H.Halloc.malloc_sizes(np.array([4096], dtype=np.uint32))
+    assert big[0] != 0
+    assert abs(H.Halloc.extfrag(False) - 0.875) < 1e-12 and abs(H.Halloc.extfrag(True) - 0.125) < 1e-12
     ptrs = H.Halloc.malloc_sizes(np.full(1 << 16, 64, dtype=np.uint32))
     assert H.Halloc.free_bytes() == g["arena_bytes"] - 64 * (1 << 16)
-    f = H.Halloc.extfrag(True)
-    assert 0.0 <= f < 1.0
+    d = H.Halloc.dump()
+    for ideal in (False, True):
+        assert abs(H.Halloc.extfrag(ideal) - expected_extfrag(g, d, ideal)) < 1e-12
     H.Halloc.free_ptrs(ptrs)
+    H.Halloc.free_ptrs(big)
+    assert abs(H.Halloc.extfrag(False) - 0.75) < 1e-12
 
 
 def test_size_sweep_lists_random_free_order(halloc):
