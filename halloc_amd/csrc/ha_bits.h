@@ -49,7 +49,8 @@ HA_HD inline u64 spread_bits(u64 x, u32 f) {
 
 // A run of 64 k-chunk slots starts at chunk s (slot of lane l = s + l*k,
 // unwrapped); k is a power of two (1, 2, 4 or 8), so the divisions by k are
-// shifts.  For lane `me` and the lanes in bitmap B, return the unwrapped
+// shifts.  s must be a multiple of k (mod 2^32): for other starts the unions
+// below are wrong, so a caller that derives s from block positions checks it.  For lane `me` and the lanes in bitmap B, return the unwrapped
 // index of the word holding me's slot, the union of B-lanes' fields inside that
 // word, and the lowest B-lane in that word (64 if none).
 //   bit array:    64 chunks per word, field = k one-bits

@@ -916,7 +916,10 @@ HA_DEV void free_rounds(const u64* off, const bool* act) {
             sbb = rdl(ich[j], l1) - l1 * k[j];
             Bm = ballot(valid && !ina && ich[j] == sbb + me * k[j]);
           }
-          fast[j] = (Am | Bm) == A[j];
+          // the run helpers take slot starts on multiples of k, as every block
+          // of this size has; a block of another size of the chunk (a slab
+          // taken over from the sparse set) may start anywhere
+          fast[j] = (Am | Bm) == A[j] && ((sa | sbb) & (k[j] - 1)) == 0;
           st[j] = ina ? sa : sbb;
           myseg[j] = ina ? Am : Bm;
         }
@@ -1384,7 +1387,8 @@ HA_API void hafree_batch(void* const* ptrs, int cnt) {
       sbb = rdl(ich, l1) - l1 * k;
       Bm = ballot(act && !ina && ich == sbb + me * k);
     }
-    if ((Am | Bm) == A) {
+    // run starts on multiples of k only (see free_rounds)
+    if ((Am | Bm) == A && ((sa | sbb) & (k - 1)) == 0) {
       fast |= 1u << j;
       inam |= (u32)ina << j;
       rk = wrl(k, j, rk);

@@ -99,6 +99,39 @@ def crowded_program(seed=5, size=256, fill_steps=100, churn_steps=200):
     return ops
 
 
+def fill_program(size, nsteps, lanes=64):
+    """Every lane of one wave allocates `size` B in every step."""
+    ops = np.zeros((nsteps, lanes, 2), dtype=np.uint32)
+    ops[:, :, 0] = 1
+    ops[:, :, 1] = size
+    return ops
+
+
+def mixed_sizes_tail(recs, big=128):
+    """Continues a fill of 16 B blocks (1 chunk each) whose records are recs
+    (steps, lanes, 4): frees every block of the first slab but two, at chunks
+    c1 and c2 off the multiples of 8 (c2 at 1..7 mod 16 and not c1 + 8);
+    then one lane allocates `big` B (8 chunks, same chunk size), which takes
+    the now sparse slab as its head; then lanes 0 and 1 of one wave free the
+    two blocks: two runs of one block whose starts are not multiples of the
+    slab's block length.  Returns (ops, slab, step of the `big` malloc)."""
+    nsteps, lanes = recs.shape[:2]
+    sb = recs[:, :, 1].reshape(-1)
+    ch = recs[:, :, 2].reshape(-1).astype(np.int64)
+    inx = np.nonzero(sb == sb[0])[0]
+    k1 = next(int(i) for i in inx if ch[i] % 8)
+    k2 = next(int(i) for i in inx if 1 <= ch[i] % 16 <= 7 and ch[i] != ch[k1] + 8 and i != k1)
+    frees = [int(i) for i in inx if i not in (k1, k2)]
+    nfree = (len(frees) + lanes - 1) // lanes
+    ops = np.zeros((nfree + 2, lanes, 2), dtype=np.uint32)
+    for j, i in enumerate(frees):
+        ops[j // lanes, j % lanes] = (2, i)
+    ops[nfree, 0] = (1, big)
+    ops[nfree + 1, 0] = (2, k1)
+    ops[nfree + 1, 1] = (2, k2)
+    return ops, int(sb[0]), nsteps + nfree
+
+
 def random_batch_ops(rng, nsteps, lanes, sizes, max_count=16, p_free=0.4):
     """Random batched replay program: [kind, arg, count, 0] per (step, lane)."""
     ops = np.zeros((nsteps, lanes, 4), dtype=np.uint32)

@@ -13,9 +13,9 @@ import pytest
 
 import halloc_amd as H
 from oracle import model as M
-from tests.helpers import (crowded_program, BLOCK_SIZES, LARGE_SB, REC_NULL, assert_disjoint, graph_sizes, large_class_of,
-                           large_state, large_units, live_intervals, pressure_program, random_batch_ops, random_ops,
-                           sweep_sizes, wave_refree_ops)
+from tests.helpers import (crowded_program, fill_program, mixed_sizes_tail, BLOCK_SIZES, LARGE_SB, REC_NULL,
+                           assert_disjoint, graph_sizes, large_class_of, large_state, large_units, live_intervals,
+                           pressure_program, random_batch_ops, random_ops, sweep_sizes, wave_refree_ops)
 
 pytestmark = pytest.mark.gpu
 
@@ -139,6 +139,42 @@ def test_replay_pressure_exhaustion_and_reuse(halloc):
     phase1 = set(recs_g[:2500, :, 1][ops[:2500, :, 0] == 1].tolist()) - {REC_NULL}
     phase3 = set(recs_g[5500:, :, 1][ops[5500:, :, 0] == 1].tolist()) - {REC_NULL}
     assert phase3 and phase3 <= phase1
+
+
+@pytest.mark.parametrize("batch", [False, True])
+def test_replay_mixed_sizes_two_run_free(halloc, batch):
+    # a slab keeps two 16 B blocks at chunks off the multiples of 8 and is
+    # then taken from the sparse set as the 128 B head (8 chunks a block);
+    # one wave frees both from two lanes: the two-run free path, whose run
+    # starts are then not multiples of the slab's block length, must clear
+    # both blocks' lengths (bit-exact state against the model)
+    say(f"replay mixed sizes two-run free batch={batch}")
+    opts = halloc(H.HallocOpts(memory=32 << 20, sb_sz_sh=21, nheads=1, busy_fraction=0.1, roomy_fraction=0.05))
+    fill = fill_program(16, 260)
+    heap = M.ModelHeap(model_opts(opts))
+    if not batch:
+        plan, _ = M.ModelHeap(model_opts(opts)).replay(fill)
+        tail, sb, big_step = mixed_sizes_tail(plan)
+        ops = np.concatenate([fill, tail])
+        recs_g, _ = H.Halloc.replay(ops)
+        recs_m, _ = heap.replay(ops)
+        big = recs_g[big_step, 0]
+    else:
+        def as_batch(o):
+            b = np.zeros(o.shape[:2] + (4,), dtype=np.uint32)
+            b[..., :2] = o
+            b[..., 2] = o[..., 0] == 1
+            return b
+        plan = M.ModelHeap(model_opts(opts)).replay_batch(as_batch(fill))
+        tail, sb, big_step = mixed_sizes_tail(plan[:, :, 0])
+        ops = as_batch(np.concatenate([fill, tail]))
+        recs_g, _ = H.Halloc.replay_batch(ops)
+        recs_m = heap.replay_batch(ops)
+        big = recs_g[big_step, 0, 0]
+    # the 128 B block went to the slab that holds the two 16 B blocks
+    assert big[0] == 6 and big[1] == sb
+    assert (recs_g == recs_m).all()
+    assert_same_state(heap, opts)
 
 
 def test_replay_crowded_head_replaced_below_its_limit(halloc):
