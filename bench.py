@@ -7,6 +7,9 @@ launch in which every lane does hamalloc(16), writes a 16 B pattern, reads it
 back and calls hafree (tst/perf/priv-throughput/priv-throughput.cu:14-27 with
 the write/verify of tst/corr/test/test.cu:84-95).  W untimed steps, then K
 timed steps; the device is synchronized before and after the timed region.
+The default W is 10: each launch's tickets sweep an eighth of every head
+slab, so the first 8 launches meet the slabs' bit and nibble lines cold and
+run about 20% slower than the steady state that follows.
 
 Also reported (each on its own freshly initialized heap):
   batching         4096 random-graph builds (n_i log-uniform on [1k, 64k]
@@ -178,7 +181,7 @@ def bench_phase(H, memory=512 << 20):
     (the share of threads holding a block at the end of an allocation and of
     a free phase, and the share acting per step), which fix the chain's
     probabilities (tst/common/common.cu:314-355).  The heap uses the library
-    defaults: 16 head sets, adaptive."""
+    defaults: 32 head sets, adaptive."""
     from halloc_amd import harness
     o = harness.defaults_for("phase-throughput")
     o.parse(["-n1048576", "-s16", "-S256", "-duniform", "-l1", "-i8", "-t10", "-f0.95", "-F0.05",
@@ -191,7 +194,7 @@ def bench_phase(H, memory=512 << 20):
     ms = r["malloc_ms"] + r["free_ms"]
     pairs = (r["mallocs"] + r["frees"]) / 2
     return {"workload": "phase-throughput -n1048576 -s16 -S256 -l1 -i8 -t10 -f0.95 -F0.05 -e0.91 -g5, "
-                        f"{memory >> 20} MiB pool, library defaults (16 head sets, adaptive)",
+                        f"{memory >> 20} MiB pool, library defaults (32 head sets, adaptive)",
             "chain_probabilities": [[float(x) for x in row] for row in o.probabs],
             "pairs_per_s": pairs / (ms / 1e3), "mallocs": r["mallocs"], "frees": r["frees"],
             "nulls": r["nulls"], "kernel_ms": ms, "violations": r["nulls"] + r["checks_failed"]}
@@ -268,9 +271,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
-    # head sets of the plain 16 B kernel: 32 (the plain form's tuned count; one
-    # size class on 96 slabs leaves room for them), against the library's 16
+    ap.add_argument("--warmup", type=int, default=10)
+    # head sets of the plain 16 B kernel: 32, the library default and the plain
+    # form's tuned count (one size class on 96 slabs leaves room for them)
     ap.add_argument("--nheads", type=int, default=32)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds for the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")

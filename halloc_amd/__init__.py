@@ -161,7 +161,7 @@ class HallocOpts:
     roomy_fraction: float = 0.6
     sparse_fraction: float = 0.012
     sb_sz_sh: int = 22
-    nheads: int = 16
+    nheads: int = 32
     hash_step_frac: float = 0.0
     cached_arena: int = 0
     fixed_sets: int = 0

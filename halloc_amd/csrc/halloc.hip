@@ -1845,7 +1845,7 @@ void ha_default_opts(ha_opts_c* o) {
   o->roomy_fraction = d.roomy_fraction;
   o->sparse_fraction = d.sparse_fraction;
   o->sb_sz_sh = d.sb_sz_sh;
-  o->nheads = 16;
+  o->nheads = 32;
   o->hash_step_frac = 0.0;
   o->cached_arena = 0;
   o->fixed_sets = 0;

@@ -34,7 +34,7 @@ typedef struct ha_opts_c {
   double roomy_fraction;     /* default 0.6 */
   double sparse_fraction;    /* default 0.012 */
   int32_t sb_sz_sh;          /* default 22 (4 MiB slabs), 21..26 */
-  int32_t nheads;            /* head-slab sets (1..64), default 16: each (set, size class) pair
+  int32_t nheads;            /* head-slab sets (1..64), default 32: each (set, size class) pair
                                 has its own head slab, ticket and lock.  A wave's full-spread set
                                 is its XCD's (xcc_id % nheads); above 8 (a multiple of 8), one of
                                 its XCD's nheads / 8 sets, picked by the wave's CU.  0: the
@@ -57,7 +57,7 @@ typedef struct ha_opts_c {
                                 1: every wave keeps its full-spread set. */
 } ha_opts_c;
 
-/* Fills *o with the defaults of src/halloc.h:30-32, nheads = 16 and adaptive
+/* Fills *o with the defaults of src/halloc.h:30-32, nheads = 32 and adaptive
    sets. */
 void ha_default_opts(ha_opts_c *o);
 

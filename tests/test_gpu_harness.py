@@ -139,7 +139,7 @@ def test_typed_new_and_delete():
     # new (halloc) T(args) / hadelete (include/halloc.h under HALLOC_CPP): every
     # object constructed once and destroyed once, arrays intact, nothing left
     say("sample objects")
-    H.Halloc.init(H.HallocOpts(memory=4 << 30))  # up to 16 classes with 16 head sets each
+    H.Halloc.init(H.HallocOpts(memory=4 << 30))  # up to 16 classes; the adaptive head sets fold to 16
     try:
         for n, hi, seed in ((1 << 20, 64, 1), (300000, 700, 2), (1000, 1, 3)):
             r = H.Halloc.run_sample("objects", n, max_len=hi, seed=seed)
@@ -150,8 +150,8 @@ def test_typed_new_and_delete():
 
 
 def test_sample_string_lengths_vary():
-    # strings of 1..600 characters reach 14 size classes: with 16 head sets
-    # each, the pool needs room for that many heads (4 GiB: 768 slabs)
+    # strings of 1..600 characters reach 14 size classes: the head sets fold to
+    # 16 on 4 GiB (768 slabs), which leaves room for that many heads
     say("sample add-strings 1..600")
     H.Halloc.init(H.HallocOpts(memory=4 << 30))
     try:

@@ -278,7 +278,7 @@ def test_replay_adaptive_sets(halloc, nheads):
 
 
 def test_adaptive_sets_many_classes_default_options(halloc):
-    # the drop-in defaults (16 sets, adaptive) at 512 MiB with requests over
+    # the drop-in defaults (32 sets, adaptive) at 512 MiB with requests over
     # all 16 classes, in kernels of 20 000 until one returns NULL: the sets
     # in use have folded to 2, no request failed before the requested bytes
     # passed half of the slab heap, and when one did no slab was free
