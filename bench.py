@@ -28,7 +28,10 @@ Also reported (each on its own freshly initialized heap):
   sweep            2^19 lanes, U{1..8} requests log-uniform on [16 B, 3 KiB]
                    plus 1% on (3 KiB, 64 KiB], random-order frees, 1 GiB
                    pool; per size class; the large path timed alone
-  roofline         dominant kernel (the plain workload kernel) against HBM:
+  arena            both arena types (uncached default, cached_arena 1): a
+                   caller's kernel making 64 passes of stores, loads and
+                   updates over 2^18 blocks of 64 B, and the plain kernel
+  roofline        dominant kernel (the plain workload kernel) against HBM:
                    algorithmic bytes per pair = 4.05 reference-scheme atomics
                    x 8 B + 32 B payload = 64.4 B; traffic from the committed
                    rocprofv3 counter summary when present
@@ -244,6 +247,29 @@ def bench_sweep(H, memory=1 << 30, seed=0, rounds=8):
     return out
 
 
+def bench_arena(H, memory=512 << 20, n=1 << 18, words=8, iters=64, reps=5):
+    """What the arena's memory type costs the caller's own kernels, and the
+    plain 16 B kernel, in both arena types: n lanes each own a block of
+    8 * words bytes (16 MiB in all, so the blocks fit the eight L2s) and make
+    `iters` passes of stores, loads or updates over its words
+    (ha_run_touch)."""
+    out = {"workload": f"{n} lanes x {8 * words} B blocks, {iters} passes of stores / loads / updates per "
+                       f"launch, mean of {reps} launches; plain 16 B kernel, 2^20 lanes, 10 launches; "
+                       f"{memory >> 20} MiB pool, library defaults"}
+    for name, cached in (("uncached", 0), ("cached", 1)):
+        H.Halloc.init(H.HallocOpts(memory=memory, cached_arena=cached))
+        r = H.Halloc.run_touch(n, words, iters, reps)
+        p = H.Halloc.run_plain(warmup=10, launches=10)
+        H.Halloc.shutdown()
+        nbytes = n * words * 8 * iters
+        out[name] = {"store_ms": r["store_ms"], "load_ms": r["load_ms"], "update_ms": r["update_ms"],
+                     "store_gbps": nbytes / (r["store_ms"] * 1e6), "load_gbps": nbytes / (r["load_ms"] * 1e6),
+                     "update_gbps": 2 * nbytes / (r["update_ms"] * 1e6),
+                     "plain_16B_us": p["total_ms"] * 1e3 / 10,
+                     "violations": r["nulls"] + p["errors"] + p["nulls"]}
+    return out
+
+
 def plain_counters():
     """The committed counter summary of the plain kernel (profiles/), or {}."""
     p = os.path.join(ROOT, "profiles", "plain_16B_counters.json")
@@ -279,7 +305,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-probe", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="only the plain 16 B workload")
-    ap.add_argument("--extras", default="batching,serving,phase,sweep",
+    ap.add_argument("--extras", default="batching,serving,phase,sweep,arena",
                     help="comma-separated workloads to run after the plain one")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -358,7 +384,7 @@ def main():
                         "probe_ms_by_rounds": {str(k): v for k, v in chain.items()}}
     extras = [] if args.no_extras else [e for e in args.extras.split(",") if e]
     for e in extras:
-        if e not in ("batching", "serving", "phase", "sweep"):
+        if e not in ("batching", "serving", "phase", "sweep", "arena"):
             print(f"unknown workload {e!r}", file=sys.stderr)
             sys.exit(2)
     if "batching" in extras:
@@ -371,6 +397,8 @@ def main():
         out["phase"] = bench_phase(H)
     if "sweep" in extras:
         out["sweep"] = bench_sweep(H)
+    if "arena" in extras:
+        out["arena"] = bench_arena(H)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_budget)
     print(json.dumps(out))

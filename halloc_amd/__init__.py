@@ -32,7 +32,7 @@ EXPORTS = ("ha_default_opts", "ha_init_c", "ha_shutdown_c", "ha_extfrag_c", "ha_
            "ha_run_plain", "ha_malloc_sizes", "ha_free_ptrs", "ha_read_words", "ha_probe_atomics",
            "ha_run_lists", "ha_run_phase", "ha_replay_batch", "ha_run_graphs", "ha_run_serve",
            "ha_probe_copy", "ha_run_test", "ha_run_sample", "ha_run_corr_test0", "ha_probe_chain",
-           "ha_free_bytes_noalloc_c")
+           "ha_free_bytes_noalloc_c", "ha_run_touch")
 
 
 class HallocError(RuntimeError):
@@ -203,6 +203,7 @@ def lib() -> ctypes.CDLL:
         "ha_free_bytes_noalloc_c": (i32, [P(ctypes.c_uint64)]),
         "ha_free_ptrs": (i32, [vp, i32, i32]),
         "ha_read_words": (i32, [vp, i32, vp]),
+        "ha_run_touch": (i32, [i32, i32, i32, i32, vp, vp, vp]),
         "ha_probe_atomics": (i32, [vp]),
         "ha_run_lists": (i32, [vp, i32, i32, vp, i32, vp, P(_ListResult)]),
         "ha_run_phase": (i32, [P(_PhaseArgs), P(_PhaseResult), vp, i32]),
@@ -346,6 +347,18 @@ class Halloc:
         out = np.zeros(len(p), dtype=np.uint32)
         _check(lib().ha_read_words(p.ctypes.data, len(p), out.ctypes.data))
         return out
+
+    @staticmethod
+    def run_touch(n=1 << 18, words=8, iters=64, reps=5) -> dict:
+        """A caller's kernel over n blocks of 8*words bytes (ha_run_touch):
+        mean launch ms and word sums of the store, load and update modes."""
+        ms = np.zeros(3, dtype=np.float64)
+        sums = np.zeros(3, dtype=np.uint64)
+        nulls = np.zeros(1, dtype=np.uint64)
+        _check(lib().ha_run_touch(n, words, iters, reps, ms.ctypes.data, sums.ctypes.data, nulls.ctypes.data))
+        return {"store_ms": float(ms[2]), "load_ms": float(ms[0]), "update_ms": float(ms[1]),
+                "store_sum": int(sums[2]), "load_sum": int(sums[0]), "update_sum": int(sums[1]),
+                "nulls": int(nulls[0])}
 
     @staticmethod
     def probe_atomics() -> np.ndarray:

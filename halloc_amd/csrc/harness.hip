@@ -259,6 +259,50 @@ __global__ void read_words_k(void* const* ptrs, int n, u32* out) {
   out[i] = ptrs[i] ? *(const u32*)ptrs[i] : 0u;
 }
 
+// ------------------------------------------------- a caller's own accesses
+// What the arena's memory type costs a kernel that uses its blocks: lane i
+// owns the `words` 64-bit words of block ptrs[i] and makes `iters` passes
+// over them.  Mode 0 loads and sums, 1 loads, updates and stores, 2 stores.
+// The empty asm between passes keeps every pass's accesses in the code.
+constexpr u64 TOUCH_MUL = 6364136223846793005ull;
+__global__ void __launch_bounds__(256) touch_alloc_k(u64** ptrs, int n, int words, unsigned long long* nulls) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  u64* p = (u64*)hamalloc((size_t)words * 8);
+  ptrs[i] = p;
+  if (!p) atomicAdd(nulls, 1ull);
+}
+__global__ void __launch_bounds__(256) touch_k(u64* const* ptrs, int n, int words, int iters, int mode,
+                                               u64* sink) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  u64* b = ptrs[i];
+  if (!b) return;
+  u64 acc = 0;
+  for (int it = 0; it < iters; ++it) {
+    for (int w = 0; w < words; ++w) {
+      if (mode == 0) acc += b[w];
+      else if (mode == 1) b[w] = b[w] * TOUCH_MUL + (u64)it;
+      else b[w] = (u64)it * (u64)words + (u64)w + (u64)i;
+    }
+    asm volatile("" ::: "memory");
+  }
+  if (mode == 0) sink[i] = acc;
+}
+// lane i: the sum of its block's words (mode 0: what touch_k stored in sink)
+__global__ void __launch_bounds__(256) touch_sum_k(u64* const* ptrs, int n, int words, int mode, u64* sink) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || mode == 0) return;
+  u64 s = 0;
+  if (const u64* b = ptrs[i])
+    for (int w = 0; w < words; ++w) s += b[w];
+  sink[i] = s;
+}
+__global__ void __launch_bounds__(256) touch_free_k(u64* const* ptrs, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) hafree(ptrs[i]);
+}
+
 // ------------------------------------------ tst/corr/test/test.cu test 0
 // malloc_k (:47-55): thread t of nthreads allocates ntimes 16 B blocks at
 // ptrs[t], ptrs[t + nthreads], ... (a grid-stride loop)
@@ -1813,6 +1857,53 @@ int ha_read_words(const uint64_t* ptrs, int n, uint32_t* out) {
   read_words_k<<<(n + 255) / 256, 256>>>((void* const*)dp.p, n, (u32*)dout.p);
   HH(hipGetLastError());
   HH(hipMemcpy(out, dout.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+  return HA_OK;
+}
+
+int ha_run_touch(int n, int words, int iters, int reps, double* ms, uint64_t* sums, uint64_t* nulls) {
+  if (!ha::host_params().initialized) return hfail(HA_E_STATE, "allocator not initialized");
+  if (n < 1 || words < 1 || words > 384 || iters < 1 || reps < 1 || !ms || !sums || !nulls)
+    return hfail(HA_E_INVALID, "bad touch arguments");
+  const int bs = 256, grid = (n + bs - 1) / bs;
+  DBuf dp, ds, dn;
+  HH(dp.alloc((size_t)n * 8));
+  HH(ds.alloc((size_t)n * 8));
+  HH(dn.alloc(8));
+  HH(hipMemset(dn.p, 0, 8));
+  Stream s;
+  HH(hipStreamCreate(&s.s));
+  hipEvent_t e0, e1;
+  HH(hipEventCreate(&e0));
+  HH(hipEventCreate(&e1));
+  u64** ptrs = (u64**)dp.p;
+  u64* sink = (u64*)ds.p;
+  touch_alloc_k<<<grid, bs, 0, s.s>>>(ptrs, n, words, (unsigned long long*)dn.p);
+  HH(hipGetLastError());
+  std::vector<u64> h((size_t)n);
+  // stores first, so that the loads and updates start from known values
+  const int order[3] = {2, 0, 1};
+  for (int m : order) {
+    touch_k<<<grid, bs, 0, s.s>>>(ptrs, n, words, iters, m, sink);  // untimed first launch
+    HH(hipEventRecord(e0, s.s));
+    for (int r = 0; r < reps; ++r) touch_k<<<grid, bs, 0, s.s>>>(ptrs, n, words, iters, m, sink);
+    HH(hipEventRecord(e1, s.s));
+    HH(hipEventSynchronize(e1));
+    float t = 0;
+    HH(hipEventElapsedTime(&t, e0, e1));
+    ms[m] = (double)t / reps;
+    touch_sum_k<<<grid, bs, 0, s.s>>>(ptrs, n, words, m, sink);
+    HH(hipMemcpyAsync(h.data(), sink, (size_t)n * 8, hipMemcpyDeviceToHost, s.s));
+    HH(hipStreamSynchronize(s.s));
+    u64 sum = 0;
+    for (u64 v : h) sum += v;
+    sums[m] = sum;
+  }
+  touch_free_k<<<grid, bs, 0, s.s>>>(ptrs, n);
+  HH(hipGetLastError());
+  HH(hipMemcpyAsync(nulls, dn.p, 8, hipMemcpyDeviceToHost, s.s));
+  HH(hipStreamSynchronize(s.s));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return HA_OK;
 }
 

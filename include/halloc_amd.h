@@ -152,6 +152,19 @@ int ha_free_ptrs(const uint64_t *ptrs, int n, int bs);
 /* Reads back the first 32-bit word of each block (0 for NULL). */
 int ha_read_words(const uint64_t *ptrs, int n, uint32_t *out);
 
+/* A caller's own kernel over heap blocks, to measure what the arena's memory
+   type costs it (no counterpart in the reference).  n lanes each hamalloc a
+   block of 8 * words bytes (words in 1..384); lane i then makes `iters`
+   passes over its block's 64-bit words in three modes, each launched once
+   untimed and then `reps` times: 2 stores it * words + w + i into word w,
+   0 loads and sums the words, 1 replaces each word x by
+   x * 6364136223846793005 + it (mod 2^64).  ms[m] is mode m's mean launch
+   time (HIP events), sums[m] the sum (mod 2^64) over lanes of the block's
+   words after mode m (mode 0: of the per-lane sums of its last launch);
+   *nulls counts the lanes whose hamalloc returned NULL (they skip the rest).
+   The blocks are freed at the end. */
+int ha_run_touch(int n, int words, int iters, int reps, double *ms, uint64_t *sums, uint64_t *nulls);
+
 /* tst/corr/test/test.cu test 0 (:84-132): nthreads threads (blocks of bs)
    each allocate nmallocs 16 B blocks in the grid-stride order of its malloc_k
    (:47-55); every block gets two u64 holding its own address, read back after

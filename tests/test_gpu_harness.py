@@ -1,6 +1,7 @@
 """The reference test harness on the device (halloc_amd/harness.py over
 ha_run_test): the run-all-tests sweep, shortened, passes; the perf tests run
 without NULLs; a failing check is reported, not hidden."""
+import numpy as np
 import pytest
 
 import halloc_amd as H
@@ -159,6 +160,39 @@ def test_sample_string_lengths_vary():
         assert r["nulls"] == 0 and r["errors"] == 0, r
         r = H.Halloc.run_sample("grid-points", 200000, ncells=5, seed=4)
         assert r["nulls"] == 0 and r["errors"] == 0, r
+        assert H.Halloc.free_bytes() == H.Halloc.geometry()["arena_bytes"]
+    finally:
+        H.Halloc.shutdown()
+
+
+def touch_expected(n, words, iters, reps):
+    """The word sums ha_run_touch must report: stores leave word w of lane i
+    at (iters-1)*words + w + i; a load pass sums the block; each of the
+    1 + reps update launches applies x -> x*M + it for it in 0..iters-1."""
+    u = np.uint64
+    x = ((iters - 1) * words + np.arange(words, dtype=np.uint64)[None, :]
+         + np.arange(n, dtype=np.uint64)[:, None]).astype(np.uint64)
+    store = int(x.sum(dtype=np.uint64))
+    load = (iters * store) % (1 << 64)
+    with np.errstate(over="ignore"):
+        for _ in range(1 + reps):
+            for it in range(iters):
+                x = x * u(6364136223846793005) + u(it)
+    return store, load, int(x.sum(dtype=np.uint64))
+
+
+@pytest.mark.parametrize("cached", [0, 1])
+def test_caller_block_accesses(cached):
+    # a caller's kernel over its blocks in both arena types: stores, loads and
+    # updates see exactly their own values, and every block is freed
+    say(f"caller block accesses cached_arena={cached}")
+    H.Halloc.init(H.HallocOpts(memory=64 << 20, cached_arena=cached))
+    try:
+        n, words, iters, reps = 5000, 8, 4, 2
+        r = H.Halloc.run_touch(n, words, iters, reps)
+        store, load, update = touch_expected(n, words, iters, reps)
+        assert r["nulls"] == 0
+        assert (r["store_sum"], r["load_sum"], r["update_sum"]) == (store, load, update)
         assert H.Halloc.free_bytes() == H.Halloc.geometry()["arena_bytes"]
     finally:
         H.Halloc.shutdown()
