@@ -232,14 +232,19 @@ HA_DEV u32 share_head(u32 ihead, u32 size_id, u32 chunk_id) {
 }
 
 // Takes over empty heads and cached heads of other (head, class) pairs until
-// one slab becomes free; false if none does.  Caller holds the head lock of
-// (ihead, size_id); the victims' head locks are only try-locked.
-HA_DEV bool steal_idle_head(u32 ihead, u32 size_id) {
+// one slab becomes free; false if none does.  other_idle: only slabs another
+// class holds without using them -- its cached heads, and its empty heads on
+// pairs outside the spread, which no wave serves.  Caller holds the head lock
+// of (ihead, size_id); the victims' head locks are only try-locked.
+HA_DEV bool steal_idle_head(u32 ihead, u32 size_id, bool other_idle = false) {
+  const u32 sp = ha_p.fixed_sets ? ha_p.nheads : ld32(ha_p.spread);
   for (u32 ih = 0; ih < ha_p.nheads; ++ih) {
     for (u32 c = 0; c < NSIZES; ++c) {
       if (ih == ihead && c == size_id) continue;
+      if (other_idle && c == size_id) continue;
       u32* lp = hot(ha_p.head_lock, ih, c);
       for (int which = 0; which < 2; ++which) {
+        if (other_idle && which == 1 && ih < sp) continue;
         u32* wp = hot(which == 0 ? ha_p.cached : ha_p.head, ih, c);
         u32 w = ld32(wp);
         if (w == SB_NONE) continue;
@@ -286,11 +291,14 @@ HA_DEV u32 adopt_head(u32 ihead, u32 size_id);
 // for a head needed now, first (after sparse) a head of the class that a pair
 // outside the spread holds, and at the end another XCD's head of the class,
 // then an idle head of another class.  A cache refill (now == false) skips the free set
-// when free slabs are scarce; a head needed now then tries sharing first.
+// when free slabs are scarce; a head needed now then first frees one slab
+// that another class holds idle (steal_idle_head, other_idle) and, if none,
+// tries sharing: pairs that share a head put two ticket sequences into one
+// slab and crowd it.
 HA_DEV u32 find_sb_for_size(u32 size_id, u32 chunk_id, u32 ihead, bool now) {
   const SizeInfo& s = ha_p.si[size_id];
   u32 sb;
-  bool tried_share = false;
+  bool tried_share = false, tried_idle = false;
   u32 nsteal = 0;
 again:
   while ((sb = set_pop(set_roomy(size_id))) != SB_NONE) {
@@ -334,7 +342,12 @@ again:
   }
   const bool scarce = free_scarce();
   if (scarce && !now) return SB_NONE;
-  if (scarce && now && !tried_share) {
+  bool freed_idle = false;
+  if (scarce && now && !tried_idle) {
+    tried_idle = true;
+    freed_idle = steal_idle_head(ihead, size_id, true);
+  }
+  if (scarce && now && !freed_idle && !tried_share) {
     tried_share = true;
     u32 shared = share_head(ihead, size_id, chunk_id);
     if (shared != SB_NONE) return shared;

@@ -55,7 +55,9 @@
 //   * with several head sets, free slabs are "scarce" below the largest of
 //     nheads, half the (head set, class) pairs holding a head, and a quarter of
 //     the heap: then a cached head is not refilled from the free set, and a new
-//     head shares its class's head before taking a free slab.
+//     head first frees one slab another class holds idle (its cached head, or
+//     its empty head on a pair outside the spread) and, when there is none,
+//     shares its class's head before taking a free slab.
 //   * one head search per class at a time (a class lock); a search frees at
 //     most 4 idle heads of other pairs; waiters on a head lock whose holder's
 //     search failed return NULL instead of searching again.
@@ -496,14 +498,19 @@ class Heap {
   }
 
   // take over empty heads and cached heads of other (head, class) pairs until
-  // one slab becomes free; false if none does.  Caller holds the head lock of
+  // one slab becomes free; false if none does.  other_idle: only slabs
+  // another class holds without using them (its cached heads, and its empty
+  // heads on pairs outside the spread).  Caller holds the head lock of
   // (ihead, size_id).
-  bool steal_idle_head(u32 ihead, u32 size_id) {
+  bool steal_idle_head(u32 ihead, u32 size_id, bool other_idle = false) {
+    const u32 sp = opts.fixed_sets ? nheads : spread.load();
     for (u32 ih = 0; ih < nheads; ++ih) {
       for (u32 c = 0; c < NSIZES; ++c) {
         if (ih == ihead && c == size_id) continue;
+        if (other_idle && c == size_id) continue;
         size_t hi = (size_t)ih * NSIZES + c;
         for (int which = 0; which < 2; ++which) {
+          if (other_idle && which == 1 && ih < sp) continue;
           std::atomic<u32>& word = which == 0 ? cached[hi] : head[hi];
           u32 w = word.load();
           if (w == SB_NONE) continue;
@@ -547,11 +554,12 @@ class Heap {
   // find_sb_for_size (src/slab.cuh:259-372): roomy -> sparse -> free -> busy;
   // then, for a head needed now, another XCD's head of the class, then an
   // idle head of another class.  A cache refill (now == false) skips the free
-  // set when free slabs are scarce; a head needed now then tries sharing first.
+  // set when free slabs are scarce; a head needed now then first frees one
+  // slab another class holds idle and, if none, tries sharing
   u32 find_sb_for_size(u32 size_id, u32 chunk_id, u32 ihead = 0, bool now = false) {
     const SizeInfo& s = si[size_id];
     u32 sb;
-    bool tried_share = false;
+    bool tried_share = false, tried_idle = false;
     u32 nsteal = 0;
   again:
     while ((sb = set_pop(set_roomy(size_id))) != SB_NONE) {
@@ -589,7 +597,12 @@ class Heap {
     }
     const bool scarce = free_scarce();
     if (scarce && !now) return SB_NONE;
-    if (scarce && now && !tried_share) {
+    bool freed_idle = false;
+    if (scarce && now && !tried_idle) {
+      tried_idle = true;
+      freed_idle = steal_idle_head(ihead, size_id, true);
+    }
+    if (scarce && now && !freed_idle && !tried_share) {
       tried_share = true;
       u32 shared = share_head(ihead, size_id, chunk_id);
       if (shared != SB_NONE) return shared;

@@ -248,3 +248,28 @@ def live_large_blocks(ops, recs):
                     if u == int(r[2]):
                         del alive[j]
     return list(alive.values())
+
+
+def class_switch_program(nsets=32, first=64, then=16, nw=64):
+    """One wave per step on head set step % nsets: nw steps of `first` B
+    blocks, frees of all of them, then nw steps of `then` B blocks.
+    Returns (ops, head set per step)."""
+    lanes = 64
+    a = np.zeros((nw, lanes, 2), dtype=np.uint32)
+    a[:, :, 0] = 1
+    a[:, :, 1] = first
+    f = np.zeros((nw, lanes, 2), dtype=np.uint32)
+    f[:, :, 0] = 2
+    f[:, :, 1] = np.arange(nw * lanes).reshape(nw, lanes)
+    b = a.copy()
+    b[:, :, 1] = then
+    ops = np.concatenate([a, f, b])
+    return ops, (np.arange(3 * nw) % nsets).astype(np.uint32)
+
+
+def head_words(dump, nsbs, nheads):
+    """(heads, cached heads) of a state dump: nheads x 16 slab words."""
+    from oracle.model import NSETS
+    hb = 3 * nsbs + NSETS * ((nsbs + 63) // 64)
+    n = nheads * 16
+    return dump[hb:hb + n].reshape(nheads, 16), dump[hb + n:hb + 2 * n].reshape(nheads, 16)

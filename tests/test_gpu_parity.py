@@ -13,9 +13,10 @@ import pytest
 
 import halloc_amd as H
 from oracle import model as M
-from tests.helpers import (crowded_program, fill_program, mixed_sizes_tail, BLOCK_SIZES, LARGE_SB, REC_NULL,
-                           assert_disjoint, graph_sizes, large_class_of, large_state, large_units, live_intervals,
-                           pressure_program, random_batch_ops, random_ops, sweep_sizes, wave_refree_ops)
+from tests.helpers import (class_switch_program, crowded_program, fill_program, head_words, mixed_sizes_tail,
+                           BLOCK_SIZES, LARGE_SB, REC_NULL, assert_disjoint, graph_sizes, large_class_of, large_state,
+                           large_units, live_intervals, pressure_program, random_batch_ops, random_ops, sweep_sizes,
+                           wave_refree_ops)
 
 pytestmark = pytest.mark.gpu
 
@@ -175,6 +176,36 @@ def test_replay_mixed_sizes_two_run_free(halloc, batch):
     assert big[0] == 6 and big[1] == sb
     assert (recs_g == recs_m).all()
     assert_same_state(heap, opts)
+
+
+def test_replay_class_switch_takes_idle_slabs(halloc):
+    # 32 sets on 96 slabs (the defaults): 64 B blocks on every set, all freed,
+    # then 16 B blocks: the 16 B heads come from the slabs the 64 B class holds
+    # idle, one per pair (bit-exact against the model)
+    say("replay class switch, 32 sets")
+    opts = halloc(H.HallocOpts(memory=512 << 20, nheads=32))
+    ops, ih = class_switch_program()
+    recs_g, recs_m, heap = replay_both(opts, ops, ih)
+    assert (recs_g == recs_m).all()
+    assert_same_state(heap, opts)
+    heads, _ = head_words(H.Halloc.dump(), H.Halloc.geometry()["nsbs"], 32)
+    col = heads[:, 0][heads[:, 0] != 0xFFFFFFFF] & 0x7FFFFFFF
+    assert len(col) == 16 and len(set(col.tolist())) == 16
+
+
+def test_plain_kernel_after_a_class_switch(halloc):
+    # the plain 16 B kernel on a heap whose every set first served 64 B
+    # blocks (caller kernel, all freed): about 50 us per launch, as on a fresh
+    # heap; when the 16 B pairs shared one head instead, launches took 9-27 ms
+    say("plain 16 B after 64 B blocks, library defaults")
+    halloc(H.HallocOpts(memory=512 << 20))
+    t = H.Halloc.run_touch(1 << 18, 8, 4, 1)
+    assert t["nulls"] == 0
+    p = H.Halloc.run_plain(warmup=2, launches=5)
+    say(f"per-launch ms {[round(float(x), 3) for x in p['ms']]}")
+    assert p["nulls"] == 0 and p["errors"] == 0
+    assert float(np.max(p["ms"])) < 1.0
+    assert H.Halloc.free_bytes() == H.Halloc.geometry()["arena_bytes"]
 
 
 def test_replay_crowded_head_replaced_below_its_limit(halloc):

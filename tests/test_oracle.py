@@ -9,8 +9,8 @@ import numpy as np
 import pytest
 
 from oracle import model as M
-from tests.helpers import (REC_NULL, crowded_program, BLOCK_SIZES, assert_disjoint, live_intervals, pressure_program, random_batch_ops,
-                           random_ops, wave_refree_ops)
+from tests.helpers import (REC_NULL, BLOCK_SIZES, assert_disjoint, class_switch_program, crowded_program, head_words,
+                           live_intervals, pressure_program, random_batch_ops, random_ops, wave_refree_ops)
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -572,15 +572,20 @@ def test_adaptive_spread_follows_the_classes_seen():
     before = heads()
     outside = int((before[2:] != 0xFFFFFFFF).sum())
     assert outside > 0                               # taken before the spread shrank
-    # more 16 B blocks than the two inner heads and their cached heads hold:
-    # each inner pair then adopts a 16 B head of an outer pair instead of
-    # taking a free slab, and no outer pair takes a new head
+    # the inner pairs' 16 B cached heads went to the other classes' head
+    # searches while free slabs were scarce (a cached head is a slab its
+    # class holds idle)
+    cached = h.dump()[hb + 16 * 16:hb + 2 * 16 * 16].reshape(16, 16)
+    assert (cached[:2, 0] == 0xFFFFFFFF).all()
+    # more 16 B blocks than two inner heads hold: each inner pair then adopts
+    # 16 B heads of outer pairs (two each in 250 waves) instead of taking a
+    # free slab, and no outer pair takes a new head
     ops[:, :, 1] = 16
     for _ in range(250):
         h.replay(ops, np.arange(nw, dtype=np.uint32) % 16)
     after = heads()
     adopted = set(before[2:, 0].tolist()) & set(after[:2, 0].tolist())
-    assert len(adopted) == 2 and (after[2:, 0] == 0xFFFFFFFF).sum() == 2
+    assert len(adopted) == 2 and (after[2:, 0] == 0xFFFFFFFF).sum() == 4
     assert ((after[2:] != 0xFFFFFFFF) <= (before[2:] != 0xFFFFFFFF)).all()
     assert M.check_dump(opts, h.dump())[0] == 0
 
@@ -666,3 +671,21 @@ def test_stale_waves_outside_the_spread_are_served_and_exhaust_cleanly():
     hb = 3 * h.nsbs + M.NSETS * nsw
     assert (d[hb + 13 * 16:hb + 14 * 16] == 0xFFFFFFFF).all()
     assert M.check_dump(opts, d)[0] == 0
+
+
+def test_class_switch_frees_idle_heads_instead_of_sharing():
+    # 32 sets on 96 slabs: 64 B blocks on every set leave 32 heads and 32
+    # cached heads of that class idle once they are freed.  The 16 B heads
+    # that follow (spread 16 for two classes) come from those idle slabs, one
+    # slab per pair; sharing one head among the pairs instead (what a scarce
+    # free set led to) puts 16 ticket sequences into one slab, whose crowded
+    # head is then replaced by sharing again
+    opts = M.make_opts(nheads=32)
+    h = M.ModelHeap(opts)
+    ops, ih = class_switch_program()
+    recs, _ = h.replay(ops, ih)
+    assert (recs[:, :, 0] != M.REC_NULL).all()
+    heads, _ = head_words(h.dump(), h.nsbs, 32)
+    col = heads[:, 0][heads[:, 0] != 0xFFFFFFFF] & 0x7FFFFFFF
+    assert len(col) == 16 and len(set(col.tolist())) == 16
+    assert M.check_dump(opts, h.dump())[0] == 0
