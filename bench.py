@@ -124,9 +124,13 @@ def bench_batching(H, njobs=4096, memory=4 << 30, seed=0):
     H.Halloc.init(autotune.opts_of(pcfg, memory))
     p = H.Halloc.run_graphs(nn, run_batch=False, repeats=3)
     H.Halloc.shutdown()
+    # the library's default allocator options at the tuned launch shape (blocks
+    # per call, resident waves: the workload's choice), and at the tuner's
+    # starting shape
     d0 = autotune.DEFAULT
     H.Halloc.init(H.HallocOpts(memory=memory))
-    d = H.Halloc.run_graphs(nn, batch=d0["batch"], waves=d0["waves"], repeats=3)
+    d = H.Halloc.run_graphs(nn, batch=cfg["batch"], waves=cfg["waves"], repeats=3)
+    d1 = H.Halloc.run_graphs(nn, batch=d0["batch"], waves=d0["waves"], run_plain=False, repeats=3)
     H.Halloc.shutdown()
     pairs = 2 * t["nodes"]  # a header and an adjacency array per node
     plain_best = min(t["plain_ms"], p["plain_ms"], d["plain_ms"])
@@ -143,8 +147,12 @@ def bench_batching(H, njobs=4096, memory=4 << 30, seed=0):
         "tuned_opts": {"batched_ms": t["batch_ms"], "plain_ms": t["plain_ms"]},
         "plain_tuned_opts": {"plain_ms": p["plain_ms"]},
         "default_opts": {"batched_ms": d["batch_ms"], "plain_ms": d["plain_ms"],
-                         "batched_vs_plain": d["plain_ms"] / d["batch_ms"]},
-        "violations": t["nulls"] + t["errors"] + p["nulls"] + p["errors"] + d["nulls"] + d["errors"],
+                         "batched_vs_plain": d["plain_ms"] / d["batch_ms"],
+                         "launch_shape": {"batch": cfg["batch"], "waves": cfg["waves"]},
+                         "batched_ms_at_start_shape": d1["batch_ms"],
+                         "start_shape": {"batch": d0["batch"], "waves": d0["waves"]}},
+        "violations": t["nulls"] + t["errors"] + p["nulls"] + p["errors"] + d["nulls"] + d["errors"]
+                      + d1["nulls"] + d1["errors"],
     }
 
 

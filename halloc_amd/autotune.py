@@ -36,7 +36,8 @@ SPACE = {
     "sparse_fraction": [0.005, 0.012, 0.05],
     "hash_step_frac": [0.0, 0.01, 0.05, 0.11, 0.25],
 }
-DEFAULT = {"batch": 8, "waves": 2048, "sb_sz_sh": 22, "nheads": 16, "busy_fraction": 0.835,
+# the library's default options and the search's starting launch shape
+DEFAULT = {"batch": 8, "waves": 2048, "sb_sz_sh": 22, "nheads": 32, "busy_fraction": 0.835,
            "roomy_fraction": 0.6, "sparse_fraction": 0.012, "hash_step_frac": 0.0}
 OPT_KEYS = ("sb_sz_sh", "nheads", "busy_fraction", "roomy_fraction", "sparse_fraction", "hash_step_frac")
 SPACE_PLAIN = {k: v for k, v in SPACE.items() if k in OPT_KEYS}  # the plain form has no launch shape
