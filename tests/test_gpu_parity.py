@@ -198,14 +198,18 @@ def test_plain_kernel_after_a_class_switch(halloc):
     # blocks (caller kernel, all freed): about 50 us per launch, as on a fresh
     # heap; when the 16 B pairs shared one head instead, launches took 9-27 ms
     say("plain 16 B after 64 B blocks, library defaults")
-    halloc(H.HallocOpts(memory=512 << 20))
+    opts = halloc(H.HallocOpts(memory=512 << 20))
     t = H.Halloc.run_touch(1 << 18, 8, 4, 1)
     assert t["nulls"] == 0
     p = H.Halloc.run_plain(warmup=2, launches=5)
     say(f"per-launch ms {[round(float(x), 3) for x in p['ms']]}")
     assert p["nulls"] == 0 and p["errors"] == 0
     assert float(np.max(p["ms"])) < 1.0
+    # a third class on top, then the whole state: every invariant holds
+    t = H.Halloc.run_touch(1 << 17, 32, 2, 1)
+    assert t["nulls"] == 0
     assert H.Halloc.free_bytes() == H.Halloc.geometry()["arena_bytes"]
+    assert M.check_dump(model_opts(opts), H.Halloc.dump())[0] == 0
 
 
 def test_replay_crowded_head_replaced_below_its_limit(halloc):
