@@ -507,14 +507,14 @@ HA_DEV u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen, bool force = false)
     if (try_lock(lp)) {
       cur = ld32(hp);
       u32 nh;
+      bool refill = false;
       // below: a forced replacement of a head that still has room
       const bool below = cur != SB_NONE && ctr_count(ld64(ctr_of(cur & ~HEAD_EXT))) < head_limit(cur, s);
       const u32 fe = ld32(ha_p.free_ep);
       if (below && ld32(fp) == cur && ld32(fp + 1) == fe) {
         nh = cur;  // this head's forced search failed, and no slab became free since
       } else if (cur == SB_NONE || !below || (force && cur == seen)) {
-        nh = ld32(cp);
-        st32(cp, SB_NONE);
+        nh = a_xchg32(cp, SB_NONE);
         if (nh == SB_NONE) {
           // one head search per class at a time: a pair that follows sees the
           // head the one before it took and can share it instead of taking
@@ -543,12 +543,26 @@ HA_DEV u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen, bool force = false)
             detach_head(csb);
             unlock(l);
           }
-          st32(cp, find_sb_for_size(size_id, s.chunk_id, ihead, false));
+          refill = true;
         }
       } else {
         nh = cur;
       }
       unlock(lp);
+      // the cache refill (a slab search) runs after the pair's lock is
+      // released, so waves needing the next replacement do not wait for it; a
+      // refill that finds the cache filled meanwhile gives its slab back
+      if (refill) {
+        const u32 c2 = find_sb_for_size(size_id, s.chunk_id, ihead, false);
+        u32 expect = SB_NONE;
+        if (c2 != SB_NONE &&
+            !__hip_atomic_compare_exchange_strong(cp, &expect, c2, __ATOMIC_RELAXED, __ATOMIC_RELAXED, HA_SCOPE)) {
+          u32* l = ha_p.sb_lock + (c2 & ~HEAD_EXT);
+          lock(l);
+          detach_head(c2 & ~HEAD_EXT);
+          unlock(l);
+        }
+      }
       return nh;
     }
     while (ld32(hp) == cur && ld32(lp) != 0) __builtin_amdgcn_s_sleep(2);
