@@ -235,13 +235,20 @@ HA_DEV u32 share_head(u32 ihead, u32 size_id, u32 chunk_id) {
 // one slab becomes free; false if none does.  other_idle: only slabs another
 // class holds without using them -- its cached heads, and its empty heads on
 // pairs outside the spread, which no wave serves.  Caller holds the head lock
-// of (ihead, size_id); the victims' head locks are only try-locked.
+// of (ihead, size_id); the victims' head locks are only try-locked.  A class
+// that never took a head holds none, so only the classes seen are visited
+// (in the same set-major order): with one class the idle search costs
+// nothing.
 HA_DEV bool steal_idle_head(u32 ihead, u32 size_id, bool other_idle = false) {
-  const u32 sp = ha_p.fixed_sets ? ha_p.nheads : ld32(ha_p.spread);
-  for (u32 ih = 0; ih < ha_p.nheads; ++ih) {
-    for (u32 c = 0; c < NSIZES; ++c) {
+  const u32 nh = ha_p.nheads;
+  const u32 sp = ha_p.fixed_sets ? nh : ld32(ha_p.spread);
+  u32 cls = ha_p.fixed_sets ? (1u << NSIZES) - 1 : ld32(ha_p.class_seen) & ((1u << NSIZES) - 1);
+  if (other_idle) cls &= ~(1u << size_id);
+  if (!cls) return false;
+  for (u32 ih = 0; ih < nh; ++ih) {
+    for (u32 rem = cls; rem; rem &= rem - 1) {
+      const u32 c = (u32)__builtin_ctz(rem);
       if (ih == ihead && c == size_id) continue;
-      if (other_idle && c == size_id) continue;
       u32* lp = hot(ha_p.head_lock, ih, c);
       for (int which = 0; which < 2; ++which) {
         if (other_idle && which == 1 && ih < sp) continue;
@@ -507,14 +514,14 @@ HA_DEV u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen, bool force = false)
     if (try_lock(lp)) {
       cur = ld32(hp);
       u32 nh;
-      bool refill = false;
       // below: a forced replacement of a head that still has room
       const bool below = cur != SB_NONE && ctr_count(ld64(ctr_of(cur & ~HEAD_EXT))) < head_limit(cur, s);
       const u32 fe = ld32(ha_p.free_ep);
       if (below && ld32(fp) == cur && ld32(fp + 1) == fe) {
         nh = cur;  // this head's forced search failed, and no slab became free since
       } else if (cur == SB_NONE || !below || (force && cur == seen)) {
-        nh = a_xchg32(cp, SB_NONE);
+        nh = ld32(cp);
+        st32(cp, SB_NONE);
         if (nh == SB_NONE) {
           // one head search per class at a time: a pair that follows sees the
           // head the one before it took and can share it instead of taking
@@ -543,26 +550,12 @@ HA_DEV u32 new_sb_for_size(u32 size_id, u32 ihead, u32 seen, bool force = false)
             detach_head(csb);
             unlock(l);
           }
-          refill = true;
+          st32(cp, find_sb_for_size(size_id, s.chunk_id, ihead, false));
         }
       } else {
         nh = cur;
       }
       unlock(lp);
-      // the cache refill (a slab search) runs after the pair's lock is
-      // released, so waves needing the next replacement do not wait for it; a
-      // refill that finds the cache filled meanwhile gives its slab back
-      if (refill) {
-        const u32 c2 = find_sb_for_size(size_id, s.chunk_id, ihead, false);
-        u32 expect = SB_NONE;
-        if (c2 != SB_NONE &&
-            !__hip_atomic_compare_exchange_strong(cp, &expect, c2, __ATOMIC_RELAXED, __ATOMIC_RELAXED, HA_SCOPE)) {
-          u32* l = ha_p.sb_lock + (c2 & ~HEAD_EXT);
-          lock(l);
-          detach_head(c2 & ~HEAD_EXT);
-          unlock(l);
-        }
-      }
       return nh;
     }
     while (ld32(hp) == cur && ld32(lp) != 0) __builtin_amdgcn_s_sleep(2);

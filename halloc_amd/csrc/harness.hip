@@ -213,11 +213,10 @@ u64 dump_words_of(const ha::Params& P) {
 }
 
 // ---------------------------------------------------------------- plain
-// Built for 7 waves per SIMD (72 VGPRs, 6 spilled to 16 B of scratch per
-// lane).  An 8-wave build (64 VGPRs) spills more: with the cache refill
-// after the head lock it ran 6% slower per launch, and about 2% faster before
-// that change.
-__global__ void __launch_bounds__(256, 7) plain_k(int nthreads, int size, int iters, unsigned long long* errs,
+// Built for 8 waves per SIMD (64 VGPRs; the inlined allocator alone would
+// take 67, and 7 waves): the kernel waits on dependent atomics, so one more
+// resident wave per SIMD is worth the few spills it costs (about 2% per launch).
+__global__ void __launch_bounds__(256, 8) plain_k(int nthreads, int size, int iters, unsigned long long* errs,
                         unsigned long long* nulls) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nthreads) return;
