@@ -196,3 +196,28 @@ def test_caller_block_accesses(cached):
         assert H.Halloc.free_bytes() == H.Halloc.geometry()["arena_bytes"]
     finally:
         H.Halloc.shutdown()
+
+
+def test_long_loop_192B_at_16_sets():
+    # tst/perf/priv-throughput at 192 B with 32 alloc/free iterations per
+    # launch on 16 head sets: the crowded-head and forced replacements run
+    # under full concurrency; no NULL, no check failure, every block freed
+    # and every heap invariant holds afterwards
+    import dataclasses
+    from oracle import model as M
+    say("priv-throughput 192 B, 16 sets, -i32")
+    o = harness.defaults_for("priv-throughput")
+    o.parse(["-n1048576", "-s192", "-l1", "-i32", "-t2", "-m536870912"])
+    opts = dataclasses.replace(o.halloc_opts(), nheads=16)
+    H.Halloc.init(opts)
+    try:
+        r = harness._kernel("priv-throughput", o)
+        assert r["nulls"] == 0 and r["checks_failed"] == 0, r
+        assert H.Halloc.free_bytes() == H.Halloc.geometry()["arena_bytes"]
+        mo = M.make_opts(memory=opts.memory, halloc_fraction=opts.halloc_fraction,
+                         busy_fraction=opts.busy_fraction, roomy_fraction=opts.roomy_fraction,
+                         sparse_fraction=opts.sparse_fraction, sb_sz_sh=opts.sb_sz_sh, nheads=opts.nheads,
+                         hash_mode=1, hash_step_frac=opts.hash_step_frac, fixed_sets=opts.fixed_sets)
+        assert M.check_dump(mo, H.Halloc.dump())[0] == 0
+    finally:
+        H.Halloc.shutdown()
