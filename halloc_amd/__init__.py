@@ -304,6 +304,9 @@ class Halloc:
 
     @staticmethod
     def run_plain(nthreads=1 << 20, bs=256, size=16, iters=1, warmup=1, launches=10):
+        """Plain hamalloc/hafree kernel: `launches` timed launches back to back
+        (total_ms between events at both ends, wall_ms by the host clock), then
+        as many again, each between its own event pair (ms)."""
         ms = np.zeros(launches, dtype=np.float32)
         r = _PlainResult()
         _check(lib().ha_run_plain(nthreads, bs, size, iters, warmup, launches, ms.ctypes.data,

@@ -62,6 +62,13 @@ struct Stream {
   }
 };
 
+struct Events {
+  std::vector<hipEvent_t> e;
+  ~Events() {
+    for (hipEvent_t x : e) (void)hipEventDestroy(x);
+  }
+};
+
 // ------------------------------------------------------------------ replay
 // Record of a malloc of `nbytes` that returned p: a slab block is (size class,
 // slab, chunk); a large block is (16 + its class, LARGE_SB, unit).
@@ -1735,41 +1742,51 @@ int ha_run_plain(int nthreads, int bs, int size, int iters, int warmup, int laun
   HH(hipMemset(dc.p, 0, 16));
   Stream s;
   HH(hipStreamCreate(&s.s));
-  hipEvent_t e0, e1;
-  HH(hipEventCreate(&e0));
-  HH(hipEventCreate(&e1));
+  // ev[0], ev[1]: the timed region; ev[2 + 2l], ev[3 + 2l]: launch l of the
+  // per-launch pass
+  Events ev;
+  for (int i = 0; i < 2 + (ms ? 2 * launches : 0); ++i) {
+    hipEvent_t e;
+    HH(hipEventCreate(&e));
+    ev.e.push_back(e);
+  }
   const int grid = (nthreads + bs - 1) / bs;
   unsigned long long* c = (unsigned long long*)dc.p;
   for (int w = 0; w < warmup; ++w) plain_k<<<grid, bs, 0, s.s>>>(nthreads, size, iters, c, c + 1);
   HH(hipGetLastError());
   HH(hipStreamSynchronize(s.s));
   HH(hipMemsetAsync(dc.p, 0, 16, s.s));
-  double total = 0;
-  int rc = HA_OK;
   HH(hipStreamSynchronize(s.s));
+  // the timed launches are queued back to back, as a caller's stream of
+  // launches runs: no host synchronization and no event between them (an
+  // event pair around every launch lengthened each step by about 6 us)
   const auto w0 = std::chrono::steady_clock::now();
-  for (int l = 0; l < launches; ++l) {
-    HH(hipEventRecord(e0, s.s));
-    plain_k<<<grid, bs, 0, s.s>>>(nthreads, size, iters, c, c + 1);
-    HH(hipEventRecord(e1, s.s));
-    HH(hipEventSynchronize(e1));
-    float t = 0;
-    HH(hipEventElapsedTime(&t, e0, e1));
-    if (ms) ms[l] = t;
-    total += t;
-  }
+  HH(hipEventRecord(ev.e[0], s.s));
+  for (int l = 0; l < launches; ++l) plain_k<<<grid, bs, 0, s.s>>>(nthreads, size, iters, c, c + 1);
+  HH(hipEventRecord(ev.e[1], s.s));
   HH(hipStreamSynchronize(s.s));
   const auto w1 = std::chrono::steady_clock::now();
   HH(hipGetLastError());
+  float total = 0;
+  HH(hipEventElapsedTime(&total, ev.e[0], ev.e[1]));
+  if (ms) {
+    // the per-launch pass, after the timed region
+    for (int l = 0; l < launches; ++l) {
+      HH(hipEventRecord(ev.e[2 + 2 * l], s.s));
+      plain_k<<<grid, bs, 0, s.s>>>(nthreads, size, iters, c, c + 1);
+      HH(hipEventRecord(ev.e[3 + 2 * l], s.s));
+    }
+    HH(hipStreamSynchronize(s.s));
+    HH(hipGetLastError());
+    for (int l = 0; l < launches; ++l) HH(hipEventElapsedTime(&ms[l], ev.e[2 + 2 * l], ev.e[3 + 2 * l]));
+  }
   unsigned long long h[2];
   HH(hipMemcpy(h, dc.p, 16, hipMemcpyDeviceToHost));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   res->errors = h[0];
   res->nulls = h[1];
   res->total_ms = total;
   res->wall_ms = std::chrono::duration<double, std::milli>(w1 - w0).count();
-  return rc;
+  return HA_OK;
 }
 
 int ha_malloc_sizes(const uint32_t* sizes, int n, int bs, uint64_t* ptrs) {

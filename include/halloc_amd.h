@@ -130,10 +130,14 @@ int ha_dump(uint64_t *out, int64_t nwords);
 /* Plain 16 B throughput (tst/perf/priv-throughput/priv-throughput.cu:14-27):
    nthreads lanes in blocks of bs (at most 256); each lane does iters x { p = hamalloc(size);
    write a 16 B pattern {tid, ~tid, tid ^ 0x9E3779B9, iter}; read it back;
-   hafree(p) }; `launches` timed launches after `warmup` untimed ones.
-   ms[i] receives launch i's duration (HIP events); total_ms is their sum and
-   wall_ms the host clock around the timed launches, the stream synchronized
-   before and after.  errors counts pattern mismatches, nulls NULL returns. */
+   hafree(p) }; `launches` timed launches after `warmup` untimed ones, queued
+   back to back on one stream (no host synchronization and no event between
+   them).  total_ms is the time between HIP events recorded before the first
+   and after the last, wall_ms the host clock around them, the stream
+   synchronized before and after.  If ms is not NULL, a second pass of
+   `launches` launches follows, each between its own pair of events, and
+   ms[i] receives launch i's duration.  errors counts pattern mismatches and
+   nulls NULL returns, over both passes. */
 typedef struct ha_plain_result {
   uint64_t errors;
   uint64_t nulls;

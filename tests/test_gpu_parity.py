@@ -202,9 +202,9 @@ def test_plain_kernel_after_a_class_switch(halloc):
     t = H.Halloc.run_touch(1 << 18, 8, 4, 1)
     assert t["nulls"] == 0
     p = H.Halloc.run_plain(warmup=2, launches=5)
-    say(f"per-launch ms {[round(float(x), 3) for x in p['ms']]}")
+    say(f"ms per launch: {p['total_ms'] / 5:.3f} timed, {[round(float(x), 3) for x in p['ms']]} one by one")
     assert p["nulls"] == 0 and p["errors"] == 0
-    assert float(np.max(p["ms"])) < 1.0
+    assert p["total_ms"] / 5 < 1.0 and float(np.max(p["ms"])) < 1.0
     # a third class on top, then the whole state: every invariant holds
     t = H.Halloc.run_touch(1 << 17, 32, 2, 1)
     assert t["nulls"] == 0
@@ -315,8 +315,11 @@ def test_replay_adaptive_sets(halloc, nheads):
 def test_adaptive_sets_many_classes_default_options(halloc):
     # the drop-in defaults (32 sets, adaptive) at 512 MiB with requests over
     # all 16 classes, in kernels of 20 000 until one returns NULL: the sets
-    # in use have folded to 2, no request failed before the requested bytes
-    # passed half of the slab heap, and when one did no slab was free
+    # in use have folded to 2, the heap then serves more than half of its
+    # slab bytes in requested bytes, and no slab is free. The requests of one
+    # kernel run concurrently, so the fill is what the heap holds after the
+    # kernel with the first NULL (the requests it served included), not the
+    # bytes of whole kernels before it
     say("adaptive sets: mixed classes until exhaustion, default options")
     opts = halloc(H.HallocOpts())
     g = H.Halloc.geometry()
@@ -326,10 +329,11 @@ def test_adaptive_sets_many_classes_default_options(halloc):
         sizes = rng.integers(8, 3073, 20000).astype(np.uint32)
         p = H.Halloc.malloc_sizes(sizes, bs=128)
         held.append(p)
+        total += int(sizes[p != 0].sum())
         if (p == 0).any():
             break
-        total += int(sizes.sum())
-    say(f"first NULL in round {rnd}: {total / g['arena_bytes']:.3f} of the slab bytes requested before")
+    say(f"first NULL in kernel {rnd} ({int((p == 0).sum())} of 20000): the heap holds "
+        f"{total / g['arena_bytes']:.3f} of its slab bytes in requested bytes")
     assert (p == 0).any() and total > 0.5 * g["arena_bytes"]
     d = H.Halloc.dump()
     nsw = g["nsw"]
