@@ -47,7 +47,10 @@ typedef struct ha_opts_c {
                                 across the XCDs' L2s, and kept for reuse after ha_shutdown;
                                 1: L2-cached (hipMalloc) arena, and every hafree performs an
                                 agent-scope release before the blocks can move to another XCD */
-  int32_t fixed_sets;        /* 0 (default): adaptive sets.  Waves use the first `spread` sets
+  int32_t fixed_sets;        /* 0 (default): adaptive sets.  nheads is first cut to the largest
+                                power of two s with 3 * s <= slabs (what one class can hold heads
+                                in: 32 at 512 MiB, 8 at 128 MiB; ha_geometry reports the count).
+                                Waves use the first `spread` sets
                                 only; the spread starts at nheads and shrinks, as size classes take
                                 their first head, to the largest power of two s with
                                 3 * s * classes seen <= slabs, and the pairs above it give their

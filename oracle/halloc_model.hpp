@@ -96,9 +96,10 @@
 //     The model states the field search with loops over aligned positions, the
 //     device with word-parallel bit arithmetic (ha_bits.h).
 //   * adaptive head sets (fixed_sets 0): waves fold their set onto the first
-//     `spread` sets; the spread starts at nheads and shrinks when a class takes
-//     its first head, to the largest power of two s with 3 * s * classes seen
-//     <= slabs.  A pair that needs a head adopts a head of its class from a
+//     `spread` sets; nheads is at most the largest power of two s with
+//     3 * s <= slabs, the spread starts at nheads and shrinks when a class
+//     takes its first head, to the largest power of two s with
+//     3 * s * classes seen <= slabs.  A pair that needs a head adopts a head of its class from a
 //     pair outside the spread (after the roomy and sparse sets, before a free
 //     slab); a pair outside the spread is served from the pair it folds onto.
 #pragma once
@@ -329,6 +330,13 @@ class Heap {
     u64 halloc_memory = (u64)(o.halloc_fraction * (double)memory);
     nsbs = (u32)(halloc_memory / sb_sz);
     nheads = (u32)(o.nheads < 1 ? 1 : o.nheads);
+    // adaptive sets: at most the sets one size class can hold heads in
+    // (halloc.hip ha_init_c, the same clamp)
+    if (!o.fixed_sets && nsbs / 3 < nheads) {
+      u32 s = 1;
+      while (2 * s <= nsbs / 3) s *= 2;
+      nheads = s;
+    }
     if (!build_size_infos(sb_sz, o, si) || nsbs == 0) return;
     nsw = (nsbs + 63) / 64;
     bit_words = sb_sz / BLOCK_STEP / 64;
@@ -652,9 +660,9 @@ class Heap {
 
   // ------------------------------------------------------------ set spread
   // Adaptive sets (halloc.hip note_class / adopt_head): the spread starts at
-  // nheads; when a size class takes its first head, it becomes the largest
-  // power of two s (at most nheads) with 3 * s * (classes seen) <= nsbs, if
-  // that is smaller.  Pairs at and above it keep their heads until a pair
+  // nheads (at most what one class allows); when a size class takes its
+  // first head, it becomes the largest power of two s (at most nheads) with
+  // 3 * s * (classes seen) <= nsbs, if that is smaller.  Pairs at and above it keep their heads until a pair
   // below adopts them (or steals them when empty).
   u32 fold_set(u32 full, u32 s) const {
     if (s >= nheads) return full;

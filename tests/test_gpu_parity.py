@@ -193,6 +193,23 @@ def test_replay_class_switch_takes_idle_slabs(halloc):
     assert len(col) == 16 and len(set(col.tolist())) == 16
 
 
+def test_plain_kernel_with_more_sets_than_the_heap_holds(halloc):
+    # 64 adaptive sets asked for on 96 slabs: the heap runs the 32 that one
+    # class can hold heads in, and the 16 B kernel runs as fast as at 32 sets.
+    # Counting the 32 sets no wave could use, free slabs looked scarce, pairs
+    # shared heads, and launches took 4-6x as long
+    say("plain 16 B at 64 adaptive sets on 512 MiB")
+    us = {}
+    for nh in (32, 64):
+        halloc(H.HallocOpts(memory=512 << 20, nheads=nh))
+        assert H.Halloc.geometry()["nheads"] == 32
+        p = H.Halloc.run_plain(nthreads=1 << 20, bs=256, size=16, iters=1, warmup=10, launches=10)
+        assert p["nulls"] == 0 and p["errors"] == 0
+        us[nh] = p["total_ms"] / 10 * 1e3
+    say(f"us per launch: {us}")
+    assert us[64] < 2 * us[32]
+
+
 def test_plain_kernel_after_a_class_switch(halloc):
     # the plain 16 B kernel on a heap whose every set first served 64 B
     # blocks (caller kernel, all freed): about 50 us per launch, as on a fresh

@@ -673,6 +673,25 @@ def test_stale_waves_outside_the_spread_are_served_and_exhaust_cleanly():
     assert M.check_dump(opts, d)[0] == 0
 
 
+def test_adaptive_sets_limited_to_what_one_class_can_hold():
+    # 96 slabs hold heads for at most 32 sets (3 * 32 <= 96): asked for 64
+    # adaptive sets, the heap runs 32, so a program gives the same records and
+    # the same state as at 32 sets. Counting the 32 sets that no wave can use,
+    # free slabs looked scarce with 32 of them free, and pairs shared heads.
+    # Fixed sets keep the count asked for.
+    ops, ih = class_switch_program()
+    runs = {}
+    for nh in (32, 64):
+        h = M.ModelHeap(M.make_opts(nheads=nh))
+        recs, _ = h.replay(ops, ih)
+        runs[nh] = (recs, h.dump())
+        assert M.check_dump(M.make_opts(nheads=nh), runs[nh][1])[0] == 0
+    assert (runs[32][0] == runs[64][0]).all()
+    assert len(runs[32][1]) == len(runs[64][1]) and (runs[32][1] == runs[64][1]).all()
+    fixed = M.ModelHeap(M.make_opts(nheads=64, fixed_sets=1)).dump()
+    assert len(fixed) - len(runs[32][1]) == 32 * 16 * 3   # 3 dump words per (set, class)
+
+
 def test_class_switch_frees_idle_heads_instead_of_sharing():
     # 32 sets on 96 slabs: 64 B blocks on every set leave 32 heads and 32
     # cached heads of that class idle once they are freed.  The 16 B heads
