@@ -1656,12 +1656,14 @@ static int init_locked(const ha_opts_c& o) {
     while (2 * P.nheads <= 16 && 2 * P.nheads * NSIZES * 3 <= P.nsbs) P.nheads *= 2;
   }
   P.fixed_sets = (u32)o.fixed_sets;
-  // adaptive sets: no more sets than one size class can hold heads in (the
-  // spread never exceeds that), so that free_scarce and the head searches
-  // count only sets that can be in use. With 64 sets on 96 slabs, the 32
+  // no more sets than one size class can hold heads in (an adaptive spread
+  // never exceeds that), so that free_scarce and the head searches count
+  // only sets that can be in use. With 64 adaptive sets on 96 slabs, the 32
   // that could never be used made 32 free slabs look scarce: pairs shared
-  // heads instead of taking free slabs and the 16 B kernel ran 4-6x slower.
-  if (!P.fixed_sets && P.nsbs / 3u < P.nheads) {
+  // heads instead of taking free slabs and the 16 B kernel ran 4-6x slower;
+  // 64 fixed sets there, each pair wanting a head and a cached head, took
+  // 86-362 ms per launch.
+  if (P.nsbs / 3u < P.nheads) {
     u32 s = 1;
     while (2 * s <= P.nsbs / 3u) s *= 2;
     P.nheads = s;

@@ -34,7 +34,10 @@ typedef struct ha_opts_c {
   double roomy_fraction;     /* default 0.6 */
   double sparse_fraction;    /* default 0.012 */
   int32_t sb_sz_sh;          /* default 22 (4 MiB slabs), 21..26 */
-  int32_t nheads;            /* head-slab sets (1..64), default 32: each (set, size class) pair
+  int32_t nheads;            /* head-slab sets (1..64), default 32, cut at init to the largest
+                                power of two s with 3 * s <= slabs where it is above (what one
+                                class can hold heads in: 32 at 512 MiB, 8 at 128 MiB;
+                                ha_geometry reports the count): each (set, size class) pair
                                 has its own head slab, ticket and lock.  A wave's full-spread set
                                 is its XCD's (xcc_id % nheads); above 8 (a multiple of 8), one of
                                 its XCD's nheads / 8 sets, picked by the wave's CU.  0: the
@@ -47,10 +50,7 @@ typedef struct ha_opts_c {
                                 across the XCDs' L2s, and kept for reuse after ha_shutdown;
                                 1: L2-cached (hipMalloc) arena, and every hafree performs an
                                 agent-scope release before the blocks can move to another XCD */
-  int32_t fixed_sets;        /* 0 (default): adaptive sets.  nheads is first cut to the largest
-                                power of two s with 3 * s <= slabs (what one class can hold heads
-                                in: 32 at 512 MiB, 8 at 128 MiB; ha_geometry reports the count).
-                                Waves use the first `spread` sets
+  int32_t fixed_sets;        /* 0 (default): adaptive sets.  Waves use the first `spread` sets
                                 only; the spread starts at nheads and shrinks, as size classes take
                                 their first head, to the largest power of two s with
                                 3 * s * classes seen <= slabs, and the pairs above it give their

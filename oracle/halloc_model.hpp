@@ -96,8 +96,8 @@
 //     The model states the field search with loops over aligned positions, the
 //     device with word-parallel bit arithmetic (ha_bits.h).
 //   * adaptive head sets (fixed_sets 0): waves fold their set onto the first
-//     `spread` sets; nheads is at most the largest power of two s with
-//     3 * s <= slabs, the spread starts at nheads and shrinks when a class
+//     `spread` sets; nheads is cut to at most the largest power of two s
+//     with 3 * s <= slabs (fixed sets too), the spread starts at nheads and shrinks when a class
 //     takes its first head, to the largest power of two s with
 //     3 * s * classes seen <= slabs.  A pair that needs a head adopts a head of its class from a
 //     pair outside the spread (after the roomy and sparse sets, before a free
@@ -330,9 +330,9 @@ class Heap {
     u64 halloc_memory = (u64)(o.halloc_fraction * (double)memory);
     nsbs = (u32)(halloc_memory / sb_sz);
     nheads = (u32)(o.nheads < 1 ? 1 : o.nheads);
-    // adaptive sets: at most the sets one size class can hold heads in
-    // (halloc.hip ha_init_c, the same clamp)
-    if (!o.fixed_sets && nsbs / 3 < nheads) {
+    // at most the sets one size class can hold heads in (halloc.hip
+    // ha_init_c, the same cut)
+    if (nsbs / 3 < nheads) {
       u32 s = 1;
       while (2 * s <= nsbs / 3) s *= 2;
       nheads = s;

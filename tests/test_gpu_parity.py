@@ -194,20 +194,21 @@ def test_replay_class_switch_takes_idle_slabs(halloc):
 
 
 def test_plain_kernel_with_more_sets_than_the_heap_holds(halloc):
-    # 64 adaptive sets asked for on 96 slabs: the heap runs the 32 that one
-    # class can hold heads in, and the 16 B kernel runs as fast as at 32 sets.
-    # Counting the 32 sets no wave could use, free slabs looked scarce, pairs
-    # shared heads, and launches took 4-6x as long
-    say("plain 16 B at 64 adaptive sets on 512 MiB")
+    # 64 sets asked for on 96 slabs: the heap runs the 32 that one class can
+    # hold heads in, and the 16 B kernel runs as fast as at 32 sets. Counting
+    # the 32 sets no wave could use, free slabs looked scarce, pairs shared
+    # heads, and launches took 4-6x as long (adaptive sets) or 86-362 ms
+    # (fixed sets)
+    say("plain 16 B at 64 sets on 512 MiB")
     us = {}
-    for nh in (32, 64):
-        halloc(H.HallocOpts(memory=512 << 20, nheads=nh))
+    for nh, fixed in ((32, 0), (64, 0), (64, 1)):
+        halloc(H.HallocOpts(memory=512 << 20, nheads=nh, fixed_sets=fixed))
         assert H.Halloc.geometry()["nheads"] == 32
         p = H.Halloc.run_plain(nthreads=1 << 20, bs=256, size=16, iters=1, warmup=10, launches=10)
         assert p["nulls"] == 0 and p["errors"] == 0
-        us[nh] = p["total_ms"] / 10 * 1e3
+        us[(nh, fixed)] = p["total_ms"] / 10 * 1e3
     say(f"us per launch: {us}")
-    assert us[64] < 2 * us[32]
+    assert us[(64, 0)] < 2 * us[(32, 0)] and us[(64, 1)] < 2 * us[(32, 0)]
 
 
 def test_plain_kernel_after_a_class_switch(halloc):

@@ -678,7 +678,7 @@ def test_adaptive_sets_limited_to_what_one_class_can_hold():
     # adaptive sets, the heap runs 32, so a program gives the same records and
     # the same state as at 32 sets. Counting the 32 sets that no wave can use,
     # free slabs looked scarce with 32 of them free, and pairs shared heads.
-    # Fixed sets keep the count asked for.
+    # Fixed sets are cut the same way; 24 slabs keep 8 fixed sets.
     ops, ih = class_switch_program()
     runs = {}
     for nh in (32, 64):
@@ -689,7 +689,12 @@ def test_adaptive_sets_limited_to_what_one_class_can_hold():
     assert (runs[32][0] == runs[64][0]).all()
     assert len(runs[32][1]) == len(runs[64][1]) and (runs[32][1] == runs[64][1]).all()
     fixed = M.ModelHeap(M.make_opts(nheads=64, fixed_sets=1)).dump()
-    assert len(fixed) - len(runs[32][1]) == 32 * 16 * 3   # 3 dump words per (set, class)
+    assert len(fixed) == len(runs[32][1])
+    small = dict(memory=64 << 20, sb_sz_sh=21, fixed_sets=1)   # 24 slabs
+    d8 = M.ModelHeap(M.make_opts(nheads=8, **small)).dump()
+    d1 = M.ModelHeap(M.make_opts(nheads=1, **small)).dump()
+    d16 = M.ModelHeap(M.make_opts(nheads=16, **small)).dump()
+    assert len(d8) - len(d1) == 7 * 16 * 3 and len(d16) == len(d8)   # 3 dump words per (set, class)
 
 
 def test_class_switch_frees_idle_heads_instead_of_sharing():
