@@ -304,7 +304,9 @@ CHAIN_ROUNDS = 6
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    # 100 launches of about 38 us: the synchronization that ends the timed
+    # region (about 14 us) weighs on the per-step time as 1/steps
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     # head sets of the plain 16 B kernel: 32, the library default and the plain
     # form's tuned count (one size class on 96 slabs leaves room for them)
